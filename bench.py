@@ -1,0 +1,160 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the direct randomization summator on one MI355X.
+
+Workload: 3-D unstructured field, N = 10,000,000 points uniform in [0,500]^3,
+M = 1000 Gaussian-model modes (l = 20), seeded (gstools_core.workloads.c2),
+FP64 inputs resident in HBM, FP64 output.  One step = one full summation
+(mode-table preparation + the direct kernel) over all points through the
+device entry point gsc_summate_dev, on torch's current HIP stream.
+
+Prints ONE JSON line: metric, value (point*modes/s), roofline of the direct
+kernel (transcendental-issue bound) and the CPU FP64 oracle timed on a bounded
+sample of the same workload on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gstools-core_amd"))
+sys.path.insert(0, ROOT)
+
+METRIC = "point·modes/s: direct vs factored-MFMA summator at ≤1e-5·σ err, 1 GPU"
+# gfx950 FP32 transcendental issue peak: 256 CUs x 32 lanes/clk (half rate) x 2.4 GHz
+TRANS_PEAK = 256 * 32 * 2.4e9
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(w, n_sample: int) -> dict:
+    import numpy as np
+
+    import oracle
+
+    md = w.modes
+    pos = np.ascontiguousarray(w.pos[:, :n_sample])
+    threads = oracle.max_threads()
+    t0 = time.perf_counter()
+    oracle.summate(md.cov_samples, md.z1, md.z2, pos, num_threads=threads)
+    dt = time.perf_counter() - t0
+    return {
+        "value": n_sample * md.m / dt,
+        "unit": "point·modes/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"first {n_sample} points of the workload x {md.m} modes, FP64 oracle "
+                  f"(C restatement of field.rs:53-62, OpenMP), {dt:.2f} s wall",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-sample", type=int, default=2_000_000,
+                    help="points of the workload timed on the CPU oracle (0 = skip)")
+    ap.add_argument("--n", type=int, default=10_000_000)
+    ap.add_argument("--m", type=int, default=1000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the summation shards by points with no exchange; see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from gstools_core import device, workloads
+
+    log(f"generating workload c2: n={args.n} m={args.m}")
+    w = workloads.c2(n=args.n, m=args.m)
+    md = w.modes
+    n, m, d = w.n_points, md.m, md.dim
+    dev = torch.device("cuda:0")
+    cov = torch.from_numpy(np.ascontiguousarray(md.cov_samples)).to(dev)
+    z1 = torch.from_numpy(md.z1).to(dev)
+    z2 = torch.from_numpy(md.z2).to(dev)
+    pos = torch.from_numpy(np.ascontiguousarray(w.pos)).to(dev)
+    out = torch.empty(n, dtype=torch.float64, device=dev)
+    nb = device.workspace_bytes(d, m)
+    work = torch.empty(nb, dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+
+    def step():
+        device.summate_dev(d, m, n, cov.data_ptr(), z1.data_ptr(), z2.data_ptr(), pos.data_ptr(),
+                           n, out.data_ptr(), work.data_ptr(), nb, stream=sh)
+
+    log(f"warmup {args.warmup}")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    log(f"timing {args.steps} steps")
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    for i in range(args.steps):
+        step()
+        ev[i + 1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per = [ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps)]
+    total_ms = ev[0].elapsed_time(ev[-1])
+    ms_per_step = total_ms / args.steps
+    value = n * m / (ms_per_step * 1e-3)
+    log(f"ms/step {ms_per_step:.3f} (min {min(per):.3f}, wall {wall * 1e3 / args.steps:.3f})")
+
+    # roofline of the dominant kernel: one hardware sine per point-mode
+    # (amplitude-phase form), i.e. N*M transcendentals per launch
+    avg_launch_s = ms_per_step * 1e-3
+    achieved = n * m / avg_launch_s / 1e12
+    roofline = {
+        "bound": "valu-transcendental",
+        "achieved": round(achieved, 4),
+        "peak": round(TRANS_PEAK / 1e12, 4),
+        "unit": "Tsin/s",
+        "frac": round(achieved / (TRANS_PEAK / 1e12), 4),
+        "traffic": None,
+        "kernel": "k_direct_scalar<3,8,double>",
+        "work_per_launch": f"{n}*{m} point-modes, 1 v_sin_f32 + 3 v_fma_f64 + fract + cvt + "
+                           "1 v_fma_f32 each",
+    }
+
+    cpu = None
+    if args.cpu_sample > 0:
+        log(f"cpu baseline on {args.cpu_sample} points")
+        cpu = cpu_baseline(w, min(args.cpu_sample, n))
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "point·modes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64-phase/f32-sin",
+        "data": "synthetic",
+        "config": {"workload": f"3-D unstructured summate, N={n} uniform in [0,500]^3, "
+                               f"M={m} Gaussian modes (l=20), seed 19031977",
+                   "path": "direct", "n_points": n, "n_modes": m, "dim": d},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

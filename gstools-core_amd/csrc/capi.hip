@@ -1,0 +1,363 @@
+// C-ABI entry points (include/gstools_core_amd.h).
+//
+// The host-pointer functions mirror the reference's contract
+// (src/lib.rs:33-84 -> src/field.rs:37-249): read-only strided inputs, a
+// freshly computed result, shape checks first.  Where the reference panics
+// (field.rs:44-46, 104-106, 180; panic = "abort" in Cargo.toml:21-22) these
+// return a status code and leave a message for gsc_last_error().
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "gsc_internal.h"
+#include "gstools_core_amd.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+#define GSC_HIP(call)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(GSC_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));         \
+  } while (0)
+
+// Per-device state for the synchronous host-pointer path: one stream and one
+// growing device buffer, guarded by a mutex (calls on one device serialise).
+struct DeviceCtx {
+  std::mutex mu;
+  bool checked = false;
+  bool ok = false;
+  hipStream_t stream = nullptr;
+  void* buf = nullptr;
+  size_t cap = 0;
+};
+constexpr int kMaxDevices = 64;
+DeviceCtx g_dev[kMaxDevices];
+
+const gsc_opts* opts_or_default(const gsc_opts* o, gsc_opts* tmp) {
+  if (o) return o;
+  gsc_opts_init(tmp);
+  return tmp;
+}
+
+int resolve_device(const gsc_opts* o, int* dev) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    return fail(GSC_ERR_NODEVICE, "no HIP device is visible");
+  int d = o->device;
+  if (d < 0) {
+    if (hipGetDevice(&d) != hipSuccess) d = 0;
+  }
+  if (d >= n || d >= kMaxDevices)
+    return fail(GSC_ERR_INVALID, "device %d out of range (%d visible)", d, n);
+  *dev = d;
+  return GSC_OK;
+}
+
+// Checks once per device that it is a gfx950 (the only code object built).
+int check_arch(int dev) {
+  DeviceCtx& c = g_dev[dev];
+  if (!c.checked) {
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, dev) != hipSuccess)
+      return fail(GSC_ERR_NODEVICE, "cannot query device %d", dev);
+    c.ok = std::strncmp(p.gcnArchName, "gfx950", 6) == 0;
+    c.checked = true;
+    if (!c.ok) return fail(GSC_ERR_NODEVICE, "device %d is %s, this library is built for gfx950",
+                           dev, p.gcnArchName);
+  }
+  return c.ok ? GSC_OK : fail(GSC_ERR_NODEVICE, "device %d is not a gfx950", dev);
+}
+
+int validate_common(int dim, int64_t M, int64_t N, const double* cov, const double* z1,
+                    const double* z2, const double* pos, const void* out) {
+  if (dim < 1) return fail(GSC_ERR_INVALID, "dim must be >= 1 (got %d)", dim);
+  if (dim > gsc::kMaxDim)
+    return fail(GSC_ERR_DIM, "dim %d is above the supported maximum %d", dim, gsc::kMaxDim);
+  if (M < 0 || N < 0) return fail(GSC_ERR_INVALID, "negative size (M=%lld, N=%lld)",
+                                  (long long)M, (long long)N);
+  if (M > (int64_t)0x7fffffff) return fail(GSC_ERR_INVALID, "M=%lld exceeds 2^31-1", (long long)M);
+  if (M > 0 && (!cov || !z1 || !z2)) return fail(GSC_ERR_INVALID, "null mode array");
+  if (N > 0 && (!pos || !out)) return fail(GSC_ERR_INVALID, "null position or output array");
+  return GSC_OK;
+}
+
+int validate_opts(const gsc_opts* o) {
+  if (o->out_dtype != GSC_F64 && o->out_dtype != GSC_F32)
+    return fail(GSC_ERR_INVALID, "out_dtype must be GSC_F64 or GSC_F32");
+  return GSC_OK;
+}
+
+// Gathers a strided 2-D host array (rows, cols) into dense row-major `dst`.
+void gather2(const double* src, int64_t rows, int64_t cols, int64_t sr, int64_t sc, double* dst,
+             int nthreads) {
+  if (sc == 1 && sr == cols) {
+    std::memcpy(dst, src, sizeof(double) * rows * cols);
+    return;
+  }
+  auto body = [&](int64_t c0, int64_t c1) {
+    for (int64_t r = 0; r < rows; ++r)
+      for (int64_t c = c0; c < c1; ++c) dst[r * cols + c] = src[r * sr + c * sc];
+  };
+  const int64_t work = rows * cols;
+  int nt = nthreads > 0 ? nthreads : (int)std::max(1u, std::thread::hardware_concurrency());
+  if (work < (1 << 20) || nt <= 1) {
+    body(0, cols);
+    return;
+  }
+  nt = (int)std::min<int64_t>(nt, 64);
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; ++t)
+    th.emplace_back(body, cols * t / nt, cols * (t + 1) / nt);
+  for (auto& t : th) t.join();
+}
+
+void gather1(const double* src, int64_t n, int64_t s, double* dst) {
+  if (s == 1) {
+    std::memcpy(dst, src, sizeof(double) * n);
+    return;
+  }
+  for (int64_t i = 0; i < n; ++i) dst[i] = src[i * s];
+}
+
+size_t align256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+
+// Shared body of the three host-pointer summators.
+//   kind: gsc::KIND_*; for KIND_INCOMPR `out` is (dim, N) with strides
+//   (out_sd, out_sp), otherwise N values with stride out_s = out_sp.
+int host_summate(int kind, int dim, int64_t M, int64_t N, const double* sf, int64_t sf_s,
+                 const double* cov, int64_t cov_sd, int64_t cov_sm, const double* z1,
+                 int64_t z1_s, const double* z2, int64_t z2_s, const double* pos, int64_t pos_sd,
+                 int64_t pos_sp, void* out, int64_t out_sd, int64_t out_sp,
+                 const gsc_opts* opts_in) {
+  gsc_opts tmp;
+  const gsc_opts* o = opts_or_default(opts_in, &tmp);
+  int rc = validate_common(dim, M, N, cov, z1, z2, pos, out);
+  if (rc) return rc;
+  if ((rc = validate_opts(o))) return rc;
+  if (kind == gsc::KIND_INCOMPR && dim != 2 && dim != 3)
+    return fail(GSC_ERR_DIM, "Only two- and three-dimensional problems are supported.");
+  if (kind == gsc::KIND_FOURIER && M > 0 && !sf)
+    return fail(GSC_ERR_INVALID, "null spectrum_factor array");
+  if (N == 0) return GSC_OK;
+
+  const int ncomp = kind == gsc::KIND_INCOMPR ? dim : 1;
+  const size_t elt = o->out_dtype == GSC_F32 ? 4 : 8;
+  try {
+    // Dense host staging of the (small) mode arrays, and of pos when strided.
+    const int64_t Mm = std::max<int64_t>(M, 1);
+    std::vector<double> modes((size_t)(dim * Mm + 3 * Mm), 0.0);
+    if (M > 0) {
+      gather2(cov, dim, M, cov_sd, cov_sm, modes.data(), 1);
+      gather1(z1, M, z1_s, modes.data() + dim * M);
+      gather1(z2, M, z2_s, modes.data() + (dim + 1) * M);
+      if (kind == gsc::KIND_FOURIER) gather1(sf, M, sf_s, modes.data() + (dim + 2) * M);
+    }
+    std::vector<double> pos_dense;
+    const double* pos_src = pos;
+    if (!(pos_sp == 1 && pos_sd == N)) {
+      pos_dense.resize((size_t)(dim * N));
+      gather2(pos, dim, N, pos_sd, pos_sp, pos_dense.data(), o->num_threads);
+      pos_src = pos_dense.data();
+    }
+    const bool out_dense = kind == gsc::KIND_INCOMPR ? (out_sp == 1 && out_sd == N) : out_sp == 1;
+
+    int dev = 0;
+    if ((rc = resolve_device(o, &dev))) return rc;
+    DeviceCtx& c = g_dev[dev];
+    std::lock_guard<std::mutex> lock(c.mu);
+    GSC_HIP(hipSetDevice(dev));
+    if ((rc = check_arch(dev))) return rc;
+    if (!c.stream) GSC_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+
+    const size_t b_table = gsc::mode_table_bytes(dim, M);
+    const size_t b_modes = align256(sizeof(double) * modes.size());
+    const size_t b_pos = align256(sizeof(double) * dim * N);
+    const size_t b_out = align256(elt * ncomp * N);
+    const size_t need = b_table + b_modes + b_pos + b_out;
+    if (need > c.cap) {
+      if (c.buf) GSC_HIP(hipFree(c.buf));
+      c.buf = nullptr;
+      c.cap = 0;
+      if (hipMalloc(&c.buf, need) != hipSuccess)
+        return fail(GSC_ERR_ALLOC, "hipMalloc of %zu bytes failed", need);
+      c.cap = need;
+    }
+    char* base = static_cast<char*>(c.buf);
+    void* d_table = base;
+    double* d_modes = reinterpret_cast<double*>(base + b_table);
+    double* d_pos = reinterpret_cast<double*>(base + b_table + b_modes);
+    void* d_out = base + b_table + b_modes + b_pos;
+
+    GSC_HIP(hipMemcpyAsync(d_modes, modes.data(), sizeof(double) * modes.size(),
+                           hipMemcpyHostToDevice, c.stream));
+    GSC_HIP(hipMemcpyAsync(d_pos, pos_src, sizeof(double) * dim * N, hipMemcpyHostToDevice,
+                           c.stream));
+    GSC_HIP(gsc::launch_prep(kind, dim, M, d_modes, d_modes + dim * M, d_modes + (dim + 1) * M,
+                             kind == gsc::KIND_FOURIER ? d_modes + (dim + 2) * M : nullptr,
+                             d_table, c.stream));
+    if (kind == gsc::KIND_INCOMPR)
+      GSC_HIP(gsc::launch_direct_incompr(dim, M, N, d_table, d_pos, N, d_out, N, o->out_dtype,
+                                         o->scale, c.stream));
+    else
+      GSC_HIP(gsc::launch_direct_scalar(dim, M, N, d_table, d_pos, N, d_out, o->out_dtype,
+                                        o->scale, c.stream));
+
+    std::vector<char> out_tmp;
+    void* host_dst = out;
+    if (!out_dense) {
+      out_tmp.resize(elt * ncomp * N);
+      host_dst = out_tmp.data();
+    }
+    GSC_HIP(hipMemcpyAsync(host_dst, d_out, elt * ncomp * N, hipMemcpyDeviceToHost, c.stream));
+    GSC_HIP(hipStreamSynchronize(c.stream));
+    if (!out_dense) {
+      for (int comp = 0; comp < ncomp; ++comp)
+        for (int64_t j = 0; j < N; ++j) {
+          const int64_t src = comp * N + j;
+          const int64_t dst = kind == gsc::KIND_INCOMPR ? comp * out_sd + j * out_sp : j * out_sp;
+          if (elt == 8)
+            static_cast<double*>(out)[dst] = reinterpret_cast<const double*>(out_tmp.data())[src];
+          else
+            static_cast<float*>(out)[dst] = reinterpret_cast<const float*>(out_tmp.data())[src];
+        }
+    }
+  } catch (const std::bad_alloc&) {
+    return fail(GSC_ERR_ALLOC, "host allocation failed");
+  } catch (...) {
+    return fail(GSC_ERR_INVALID, "unexpected host exception");
+  }
+  return GSC_OK;
+}
+
+int dev_common(int kind, int dim, int64_t M, int64_t N, const double* sf, const double* cov,
+               const double* z1, const double* z2, const double* pos, int64_t pos_ld, void* out,
+               int64_t out_ld, void* work, size_t work_bytes, const gsc_opts* opts_in,
+               void* stream) {
+  gsc_opts tmp;
+  const gsc_opts* o = opts_or_default(opts_in, &tmp);
+  int rc = validate_common(dim, M, N, cov, z1, z2, pos, out);
+  if (rc) return rc;
+  if ((rc = validate_opts(o))) return rc;
+  if (kind == gsc::KIND_INCOMPR && dim != 2 && dim != 3)
+    return fail(GSC_ERR_DIM, "Only two- and three-dimensional problems are supported.");
+  if (kind == gsc::KIND_FOURIER && M > 0 && !sf)
+    return fail(GSC_ERR_INVALID, "null spectrum_factor array");
+  if (pos_ld < N) return fail(GSC_ERR_INVALID, "pos_ld (%lld) < N (%lld)", (long long)pos_ld,
+                              (long long)N);
+  if (kind == gsc::KIND_INCOMPR && out_ld < N)
+    return fail(GSC_ERR_INVALID, "out_ld (%lld) < N (%lld)", (long long)out_ld, (long long)N);
+  const size_t need = gsc::mode_table_bytes(dim, M);
+  if (!work || work_bytes < need)
+    return fail(GSC_ERR_WORKSPACE, "workspace of %zu bytes is below the %zu required", work_bytes,
+                need);
+  if (N == 0) return GSC_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  GSC_HIP(gsc::launch_prep(kind, dim, M, cov, z1, z2, sf, work, s));
+  if (kind == gsc::KIND_INCOMPR)
+    GSC_HIP(gsc::launch_direct_incompr(dim, M, N, work, pos, pos_ld, out, out_ld, o->out_dtype,
+                                       o->scale, s));
+  else
+    GSC_HIP(gsc::launch_direct_scalar(dim, M, N, work, pos, pos_ld, out, o->out_dtype, o->scale,
+                                      s));
+  return GSC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void gsc_opts_init(gsc_opts* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->out_dtype = GSC_F64;
+  o->path = GSC_PATH_AUTO;
+  o->precision = GSC_PREC_AUTO;
+  o->device = -1;
+  o->num_threads = 0;
+  o->scale = 1.0;
+}
+
+const char* gsc_version(void) { return "1.1.0-mi355x.1"; }
+
+const char* gsc_last_error(void) { return g_last_error.c_str(); }
+
+int gsc_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+size_t gsc_workspace_bytes(int dim, int64_t M) {
+  if (dim < 1 || M < 0) return 0;
+  return gsc::mode_table_bytes(dim, M);
+}
+
+int gsc_summate(int dim, int64_t M, int64_t N, const double* cov, int64_t cov_sd, int64_t cov_sm,
+                const double* z1, int64_t z1_s, const double* z2, int64_t z2_s, const double* pos,
+                int64_t pos_sd, int64_t pos_sp, void* out, int64_t out_s, const gsc_opts* opts) {
+  return host_summate(gsc::KIND_SCALAR, dim, M, N, nullptr, 0, cov, cov_sd, cov_sm, z1, z1_s, z2,
+                      z2_s, pos, pos_sd, pos_sp, out, 0, out_s, opts);
+}
+
+int gsc_summate_incompr(int dim, int64_t M, int64_t N, const double* cov, int64_t cov_sd,
+                        int64_t cov_sm, const double* z1, int64_t z1_s, const double* z2,
+                        int64_t z2_s, const double* pos, int64_t pos_sd, int64_t pos_sp, void* out,
+                        int64_t out_sd, int64_t out_sp, const gsc_opts* opts) {
+  return host_summate(gsc::KIND_INCOMPR, dim, M, N, nullptr, 0, cov, cov_sd, cov_sm, z1, z1_s,
+                      z2, z2_s, pos, pos_sd, pos_sp, out, out_sd, out_sp, opts);
+}
+
+int gsc_summate_fourier(int dim, int64_t M, int64_t N, const double* sf, int64_t sf_s,
+                        const double* modes, int64_t md_sd, int64_t md_sm, const double* z1,
+                        int64_t z1_s, const double* z2, int64_t z2_s, const double* pos,
+                        int64_t pos_sd, int64_t pos_sp, void* out, int64_t out_s,
+                        const gsc_opts* opts) {
+  return host_summate(gsc::KIND_FOURIER, dim, M, N, sf, sf_s, modes, md_sd, md_sm, z1, z1_s, z2,
+                      z2_s, pos, pos_sd, pos_sp, out, 0, out_s, opts);
+}
+
+int gsc_summate_dev(int dim, int64_t M, int64_t N, const double* cov, const double* z1,
+                    const double* z2, const double* pos, int64_t pos_ld, void* out, void* work,
+                    size_t work_bytes, const gsc_opts* opts, void* stream) {
+  return dev_common(gsc::KIND_SCALAR, dim, M, N, nullptr, cov, z1, z2, pos, pos_ld, out, 0, work,
+                    work_bytes, opts, stream);
+}
+
+int gsc_summate_incompr_dev(int dim, int64_t M, int64_t N, const double* cov, const double* z1,
+                            const double* z2, const double* pos, int64_t pos_ld, void* out,
+                            int64_t out_ld, void* work, size_t work_bytes, const gsc_opts* opts,
+                            void* stream) {
+  return dev_common(gsc::KIND_INCOMPR, dim, M, N, nullptr, cov, z1, z2, pos, pos_ld, out, out_ld,
+                    work, work_bytes, opts, stream);
+}
+
+int gsc_summate_fourier_dev(int dim, int64_t M, int64_t N, const double* sf, const double* modes,
+                            const double* z1, const double* z2, const double* pos, int64_t pos_ld,
+                            void* out, void* work, size_t work_bytes, const gsc_opts* opts,
+                            void* stream) {
+  return dev_common(gsc::KIND_FOURIER, dim, M, N, sf, modes, z1, z2, pos, pos_ld, out, 0, work,
+                    work_bytes, opts, stream);
+}
+
+}  // extern "C"

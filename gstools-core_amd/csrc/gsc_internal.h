@@ -1,0 +1,50 @@
+// Internal declarations shared by the HIP kernels and the C-ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace gsc {
+
+// One spectral mode, prepared once per call in FP64 and read wave-uniformly
+// (scalar loads) by the summation kernels.
+//   kap[i] = k_i / (2 pi)          phase in revolutions: t = kap . x + th
+//   th     = atan2(z1, z2) / (2 pi)
+//   amp[c] = per-component amplitude of A sin(2 pi t), A = hypot(z1, z2):
+//            scalar summator: amp[0] = A; Fourier: amp[0] = sf * A;
+//            incompressible:  amp[c] = (delta_c0 - k_c k_0 / |k|^2) * A.
+// z1 cos(phi) + z2 sin(phi) = A sin(phi + atan2(z1, z2)), so one sine per
+// point-mode replaces the reference's cos + sin pair (field.rs:58).
+struct alignas(16) ModeRec {
+  double kap[3];
+  double th;
+  float amp[4];
+};
+static_assert(sizeof(ModeRec) == 48, "ModeRec layout");
+
+// For 4 <= dim <= kMaxDim the table is (dim + 2) doubles per mode:
+// [th, amp, kap_0 .. kap_{dim-1}].
+constexpr int kMaxDim = 16;
+
+enum Kind { KIND_SCALAR = 0, KIND_INCOMPR = 1, KIND_FOURIER = 2 };
+
+size_t mode_table_bytes(int dim, int64_t M);
+
+// Builds the mode table from dense device arrays cov (dim, M) row-major,
+// z1, z2 (M) and, for KIND_FOURIER, sf (M).
+hipError_t launch_prep(int kind, int dim, int64_t M, const double* cov, const double* z1,
+                       const double* z2, const double* sf, void* table, hipStream_t stream);
+
+// out[j] (N values of out_dtype) = scale * sum_m amp0_m sin(2 pi (kap_m . x_j + th_m)).
+hipError_t launch_direct_scalar(int dim, int64_t M, int64_t N, const void* table,
+                                const double* pos, int64_t pos_ld, void* out, int out_dtype,
+                                double scale, hipStream_t stream);
+
+// out[c * out_ld + j] = scale * sum_m amp_{c,m} sin(2 pi (kap_m . x_j + th_m)), c < dim.
+hipError_t launch_direct_incompr(int dim, int64_t M, int64_t N, const void* table,
+                                 const double* pos, int64_t pos_ld, void* out, int64_t out_ld,
+                                 int out_dtype, double scale, hipStream_t stream);
+
+}  // namespace gsc

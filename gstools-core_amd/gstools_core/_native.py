@@ -1,0 +1,147 @@
+"""ctypes binding of libgstools_core_amd.so (declarations in include/gstools_core_amd.h).
+
+The library is loaded lazily on first use.  There is no CPU fallback: if the
+library is missing or no gfx950 device is visible, the call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgstools_core_amd.so")
+
+GSC_OK = 0
+GSC_ERR_INVALID = 1
+GSC_ERR_DIM = 2
+GSC_ERR_NODEVICE = 3
+GSC_ERR_HIP = 4
+GSC_ERR_ALLOC = 5
+GSC_ERR_WORKSPACE = 6
+
+GSC_F64 = 0
+GSC_F32 = 1
+
+GSC_PATH_AUTO = 0
+GSC_PATH_DIRECT = 1
+GSC_PATH_FACTORED = 2
+
+PRECISIONS = {
+    "auto": 0, "fp16x3": 1, "bf16x3": 2, "bf16x6": 3, "fp16x1": 4, "bf16x1": 5,
+}
+
+
+class GscOpts(ctypes.Structure):
+    _fields_ = [
+        ("out_dtype", ctypes.c_int32),
+        ("path", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("device", ctypes.c_int32),
+        ("num_threads", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("scale", ctypes.c_double),
+    ]
+
+
+_i64 = ctypes.c_int64
+_int = ctypes.c_int
+_vp = ctypes.c_void_p
+_sz = ctypes.c_size_t
+_optp = ctypes.POINTER(GscOpts)
+
+# name -> (restype, argtypes); every function declared in the public header.
+SIGNATURES = {
+    "gsc_opts_init": (None, [_optp]),
+    "gsc_version": (ctypes.c_char_p, []),
+    "gsc_last_error": (ctypes.c_char_p, []),
+    "gsc_device_count": (_int, []),
+    "gsc_workspace_bytes": (_sz, [_int, _i64]),
+    "gsc_summate": (_int, [_int, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64,
+                           _i64, _vp, _i64, _optp]),
+    "gsc_summate_incompr": (_int, [_int, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp,
+                                   _i64, _i64, _vp, _i64, _i64, _optp]),
+    "gsc_summate_fourier": (_int, [_int, _i64, _i64, _vp, _i64, _vp, _i64, _i64, _vp, _i64, _vp,
+                                   _i64, _vp, _i64, _i64, _vp, _i64, _optp]),
+    "gsc_summate_dev": (_int, [_int, _i64, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _sz, _optp,
+                               _vp]),
+    "gsc_summate_incompr_dev": (_int, [_int, _i64, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _i64, _vp,
+                                       _sz, _optp, _vp]),
+    "gsc_summate_fourier_dev": (_int, [_int, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp,
+                                       _sz, _optp, _vp]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+def _preload_hip_runtime() -> None:
+    """Makes the process use ONE HIP runtime when PyTorch is installed.
+
+    PyTorch-ROCm ships its own libamdhip64.so with the same SONAME
+    (libamdhip64.so.7) as /opt/rocm's.  Whichever is loaded first serves every
+    later request for that SONAME, and torch does not initialise on a runtime
+    other than its own.  Loading torch's copy first (RTLD_GLOBAL) lets this
+    library and torch share it in either import order.  GSC_HIP_RUNTIME=system
+    keeps the system runtime instead.
+    """
+    if os.environ.get("GSC_HIP_RUNTIME", "auto") == "system":
+        return
+    import importlib.util
+    spec = importlib.util.find_spec("torch")
+    if spec is None or not spec.submodule_search_locations:
+        return
+    cand = os.path.join(list(spec.submodule_search_locations)[0], "lib", "libamdhip64.so")
+    if os.path.exists(cand):
+        ctypes.CDLL(cand, mode=ctypes.RTLD_GLOBAL)
+
+
+def lib():
+    """Returns the loaded library, loading it on first use (raises if absent)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(
+                    f"native library {LIB_PATH} is not built; run "
+                    "`python gstools-core_amd/build_native.py` (hipcc, gfx950)")
+            _preload_hip_runtime()
+            handle = ctypes.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(handle, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = handle
+    return _lib
+
+
+def make_opts(out_dtype=GSC_F64, scale=1.0, device=-1, num_threads=0, path=GSC_PATH_AUTO,
+              precision=0) -> GscOpts:
+    o = GscOpts()
+    lib().gsc_opts_init(ctypes.byref(o))
+    o.out_dtype = out_dtype
+    o.scale = float(scale)
+    o.device = int(device)
+    o.num_threads = int(num_threads)
+    o.path = int(path)
+    o.precision = int(precision)
+    return o
+
+
+def last_error() -> str:
+    msg = lib().gsc_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(rc: int) -> None:
+    """Maps a gsc_status to the Python exception the drop-in raises."""
+    if rc == GSC_OK:
+        return
+    msg = last_error()
+    if rc in (GSC_ERR_INVALID, GSC_ERR_DIM, GSC_ERR_WORKSPACE):
+        raise ValueError(msg)
+    if rc == GSC_ERR_ALLOC:
+        raise MemoryError(msg)
+    raise RuntimeError(msg or f"gstools_core native call failed with status {rc}")

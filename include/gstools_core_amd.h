@@ -1,0 +1,166 @@
+/*
+ * gstools_core_amd.h -- C-ABI of the MI355X (gfx950) randomization-method
+ * summators.
+ *
+ * The reference crate (GSTools-Core 1.1.0) exposes these computations as Rust
+ * functions over ndarray views and as PyO3 functions of the Python module
+ * `gstools_core`.  Each entry point below replaces one of them; its comment
+ * names the reference interface (file:line) it stands in for.  The
+ * signatures use plain pointers, sizes and signed element strides only, so a
+ * Rust `extern "C"` block, ctypes or cgo can bind them directly.
+ *
+ * Conventions
+ *  - Arrays are float64 unless stated.  A 2-D array (rows, cols) is addressed
+ *    as p[r * s_row + c * s_col] with signed strides counted in elements, so
+ *    numpy/ndarray views of any layout (including negative strides) can be
+ *    passed without copying on the caller side.
+ *  - `cov`  is the (dim, M) matrix of spectral samples k_m (one mode per
+ *    column), `z1`, `z2` the M standard-normal weights, `pos` the (dim, N)
+ *    matrix of points (one point per column).
+ *  - Every function returns a gsc_status.  Nothing unwinds or aborts across
+ *    this boundary; the message of the last failure on the calling thread is
+ *    available from gsc_last_error().
+ *  - Results are NOT scaled (as in the reference) unless gsc_opts.scale is
+ *    set; GSTools' RandMeth multiplies by sqrt(var / M) itself.
+ *  - The host-pointer entry points are synchronous and thread-safe (one lock
+ *    per device).  The *_dev entry points take device pointers, enqueue work
+ *    on the given HIP stream (a hipStream_t passed as void*, NULL = the null
+ *    stream) and return without synchronising.
+ */
+#ifndef GSTOOLS_CORE_AMD_H
+#define GSTOOLS_CORE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum gsc_status {
+  GSC_OK = 0,
+  GSC_ERR_INVALID = 1,   /* bad sizes, shapes, strides, null pointers or options */
+  GSC_ERR_DIM = 2,       /* unsupported spatial dimension */
+  GSC_ERR_NODEVICE = 3,  /* no usable gfx950 device */
+  GSC_ERR_HIP = 4,       /* a HIP runtime call failed */
+  GSC_ERR_ALLOC = 5,     /* host or device allocation failed */
+  GSC_ERR_WORKSPACE = 6  /* caller workspace too small */
+} gsc_status;
+
+typedef enum gsc_dtype { GSC_F64 = 0, GSC_F32 = 1 } gsc_dtype;
+
+/* Evaluation path for structured grids (gsc_summate_grid*). */
+typedef enum gsc_path {
+  GSC_PATH_AUTO = 0,      /* tuner's choice */
+  GSC_PATH_DIRECT = 1,    /* per-point kernel (FP64 phase, FP32 sin) */
+  GSC_PATH_FACTORED = 2   /* per-axis tables contracted on the MFMA cores */
+} gsc_path;
+
+/* Split-precision format of the factored path's MFMA products. */
+typedef enum gsc_precision {
+  GSC_PREC_AUTO = 0,      /* most accurate format that meets 1e-5 sigma: fp16x3 */
+  GSC_PREC_F16X3 = 1,     /* fp16 hi/lo parts, 3 products, FP32 accumulate */
+  GSC_PREC_BF16X3 = 2,    /* bf16 hi/lo parts, 3 products */
+  GSC_PREC_BF16X6 = 3,    /* bf16 hi/mid/lo parts, 6 products */
+  GSC_PREC_F16X1 = 4,     /* fp16, 1 product (fast, ~1e-3 sigma) */
+  GSC_PREC_BF16X1 = 5     /* bf16, 1 product (fast, ~1e-2 sigma) */
+} gsc_precision;
+
+typedef struct gsc_opts {
+  int32_t out_dtype;    /* gsc_dtype of `out` (default GSC_F64, as the reference) */
+  int32_t path;         /* gsc_path, structured entry points only */
+  int32_t precision;    /* gsc_precision, factored path only */
+  int32_t device;       /* HIP device ordinal, -1 = current device */
+  int32_t num_threads;  /* host threads for staging strided arrays, 0 = all */
+  int32_t reserved;
+  double scale;         /* multiplies every output value (1.0 = reference) */
+} gsc_opts;
+
+/* Fills *opts with the defaults: f64 output, auto path/precision, device -1, scale 1. */
+void gsc_opts_init(gsc_opts* opts);
+
+/* Library version string ("1.1.0-mi355x.<n>"); the reference exports
+ * gstools_core.__version__ (lib.rs:31). */
+const char* gsc_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* gsc_last_error(void);
+
+/* Number of visible HIP devices (0 when none; never fails). */
+int gsc_device_count(void);
+
+/*
+ * Scalar randomization summator.
+ *   out[j] = scale * sum_{m<M} z1[m] cos(k_m . x_j) + z2[m] sin(k_m . x_j)
+ * Replaces field::summator (src/field.rs:37-65) and the PyO3 function
+ * gstools_core.summate (src/lib.rs:33-48).  dim >= 1 (1..3 use specialised
+ * kernels, 4..16 a generic one).  Host pointers; `out` has N elements of
+ * opts->out_dtype with stride out_s.  opts may be NULL.
+ */
+int gsc_summate(int dim, int64_t M, int64_t N,
+                const double* cov, int64_t cov_sd, int64_t cov_sm,
+                const double* z1, int64_t z1_s,
+                const double* z2, int64_t z2_s,
+                const double* pos, int64_t pos_sd, int64_t pos_sp,
+                void* out, int64_t out_s, const gsc_opts* opts);
+
+/*
+ * Incompressible (divergence-free) vector summator, dim in {2, 3}:
+ *   out[i][j] = scale * sum_m (delta_i0 - k_mi k_m0 / |k_m|^2)
+ *                        (z1[m] cos(k_m . x_j) + z2[m] sin(k_m . x_j))
+ * Replaces field::summator_incompr (src/field.rs:97-182) and
+ * gstools_core.summate_incompr (src/lib.rs:50-65).  `out` is (dim, N) with
+ * strides (out_sd, out_sp); the reference returns Fortran order, i.e.
+ * out_sd = 1, out_sp = dim.  Returns GSC_ERR_DIM for other dimensions (the
+ * reference panics, field.rs:180).
+ */
+int gsc_summate_incompr(int dim, int64_t M, int64_t N,
+                        const double* cov, int64_t cov_sd, int64_t cov_sm,
+                        const double* z1, int64_t z1_s,
+                        const double* z2, int64_t z2_s,
+                        const double* pos, int64_t pos_sd, int64_t pos_sp,
+                        void* out, int64_t out_sd, int64_t out_sp, const gsc_opts* opts);
+
+/*
+ * Fourier-method summator:
+ *   out[j] = scale * sum_m sf[m] (z1[m] cos(k_m . x_j) + z2[m] sin(k_m . x_j))
+ * with k_m the columns of `modes`.  Replaces field::summator_fourier
+ * (src/field.rs:219-249) and gstools_core.summate_fourier (src/lib.rs:67-84).
+ */
+int gsc_summate_fourier(int dim, int64_t M, int64_t N,
+                        const double* sf, int64_t sf_s,
+                        const double* modes, int64_t md_sd, int64_t md_sm,
+                        const double* z1, int64_t z1_s,
+                        const double* z2, int64_t z2_s,
+                        const double* pos, int64_t pos_sd, int64_t pos_sp,
+                        void* out, int64_t out_s, const gsc_opts* opts);
+
+/*
+ * Device-resident forms of the three summators (same arithmetic).  All
+ * arrays are device pointers in dense row-major layout: cov/modes (dim, M),
+ * z1/z2/sf (M), pos (dim, N) with row pitch pos_ld >= N elements, out (N) for
+ * the scalar forms and (dim, N) with row pitch out_ld >= N for incompr.
+ * `work` is device scratch of at least gsc_workspace_bytes(dim, M) bytes
+ * (the per-mode table); it must stay untouched until the stream has run the
+ * call.  `sf` is used by the Fourier form only and may be NULL otherwise.
+ * These have no counterpart in the reference (which has no device side);
+ * they are the device half of the three functions above.
+ */
+size_t gsc_workspace_bytes(int dim, int64_t M);
+int gsc_summate_dev(int dim, int64_t M, int64_t N, const double* cov, const double* z1,
+                    const double* z2, const double* pos, int64_t pos_ld, void* out,
+                    void* work, size_t work_bytes, const gsc_opts* opts, void* stream);
+int gsc_summate_incompr_dev(int dim, int64_t M, int64_t N, const double* cov, const double* z1,
+                            const double* z2, const double* pos, int64_t pos_ld, void* out,
+                            int64_t out_ld, void* work, size_t work_bytes, const gsc_opts* opts,
+                            void* stream);
+int gsc_summate_fourier_dev(int dim, int64_t M, int64_t N, const double* sf, const double* modes,
+                            const double* z1, const double* z2, const double* pos, int64_t pos_ld,
+                            void* out, void* work, size_t work_bytes, const gsc_opts* opts,
+                            void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GSTOOLS_CORE_AMD_H */

@@ -1,0 +1,180 @@
+"""CPU tier: the C-ABI library and the Python drop-in's host logic (no GPU needed).
+
+No compute call reaches a device here: argument and shape errors are raised
+before the library touches HIP, and a valid call without a GPU must fail
+loudly (there is no CPU fallback).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+import gstools_core
+from gstools_core import _native, device, workloads
+
+HEADER = os.path.join(ROOT, "include", "gstools_core_amd.h")
+
+
+def header_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gsc_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_header_symbol():
+    lib = _native.lib()
+    names = header_functions()
+    assert "gsc_summate" in names and "gsc_summate_incompr" in names
+    for n in names:
+        assert hasattr(lib, n), f"{n} declared in the header but not exported"
+    # the ctypes signature table covers exactly the header
+    assert set(_native.SIGNATURES) == set(names)
+
+
+def test_version_and_defaults():
+    lib = _native.lib()
+    assert lib.gsc_version().decode().startswith("1.1.0")
+    o = _native.GscOpts()
+    lib.gsc_opts_init(ctypes.byref(o))
+    assert (o.out_dtype, o.path, o.precision, o.device, o.scale) == (0, 0, 0, -1, 1.0)
+    assert gstools_core.__version__ == "1.1.0"
+
+
+def test_workspace_bytes():
+    assert device.workspace_bytes(3, 1000) >= 48 * 1000
+    assert device.workspace_bytes(3, 1000) % 256 == 0
+    assert device.workspace_bytes(5, 10) >= 7 * 8 * 10
+    assert device.workspace_bytes(0, 10) == 0
+
+
+def _ok_args(d=3, m=4, n=5):
+    rng = np.random.default_rng(0)
+    return rng.normal(size=(d, m)), rng.normal(size=m), rng.normal(size=m), rng.normal(size=(d, n))
+
+
+@pytest.mark.parametrize("fn", [gstools_core.summate, gstools_core.summate_incompr])
+def test_type_errors_like_pyo3(fn):
+    cov, z1, z2, pos = _ok_args()
+    with pytest.raises(TypeError):
+        fn(cov.tolist(), z1, z2, pos)
+    with pytest.raises(TypeError):
+        fn(cov.astype(np.float32), z1, z2, pos)
+    with pytest.raises(TypeError):
+        fn(cov, z1[None, :], z2, pos)
+    with pytest.raises(TypeError):
+        fn(cov, z1, z2, pos.astype(">f8"))
+    with pytest.raises(TypeError):
+        fn(cov, z1, z2, pos, 2.0)
+    with pytest.raises(OverflowError):
+        fn(cov, z1, z2, pos, -1)
+
+
+@pytest.mark.parametrize("fn", [gstools_core.summate, gstools_core.summate_incompr])
+def test_shape_mismatch_raises_value_error(fn):
+    cov, z1, z2, pos = _ok_args()
+    with pytest.raises(ValueError):
+        fn(cov[:2], z1, z2, pos)
+    with pytest.raises(ValueError):
+        fn(cov, z1[:3], z2, pos)
+    with pytest.raises(ValueError):
+        fn(cov, z1, z2[:3], pos)
+
+
+def test_incompr_dimension_check():
+    cov, z1, z2, pos = _ok_args(d=4)
+    with pytest.raises(ValueError, match="two- and three-dimensional"):
+        gstools_core.summate_incompr(cov, z1, z2, pos)
+    cov, z1, z2, pos = _ok_args(d=1)
+    with pytest.raises(ValueError):
+        gstools_core.summate_incompr(cov, z1, z2, pos)
+
+
+def test_fourier_shape_checks():
+    cov, z1, z2, pos = _ok_args()
+    with pytest.raises(ValueError):
+        gstools_core.summate_fourier(np.ones(3), cov, z1, z2, pos)
+    with pytest.raises(TypeError):
+        gstools_core.summate_fourier([1.0] * 4, cov, z1, z2, pos)
+
+
+def test_empty_point_set_needs_no_device():
+    cov, z1, z2, _ = _ok_args()
+    r = gstools_core.summate(cov, z1, z2, np.zeros((3, 0)))
+    assert r.shape == (0,) and r.dtype == np.float64
+    r = gstools_core.summate_incompr(cov, z1, z2, np.zeros((3, 0)))
+    assert r.shape == (3, 0)
+
+
+def test_capi_validation_before_device():
+    lib = _native.lib()
+    o = _native.make_opts()
+    buf = np.zeros(16)
+    p = ctypes.c_void_p(buf.ctypes.data)
+    # incompr with d = 4: GSC_ERR_DIM, no device touched
+    rc = lib.gsc_summate_incompr(4, 2, 2, p, 2, 1, p, 1, p, 1, p, 2, 1, p, 1, 4, ctypes.byref(o))
+    assert rc == _native.GSC_ERR_DIM
+    assert "two- and three-dimensional" in _native.last_error()
+    # negative sizes, null pointers, bad dtype
+    assert lib.gsc_summate(3, -1, 2, p, 1, 1, p, 1, p, 1, p, 1, 1, p, 1, None) == \
+        _native.GSC_ERR_INVALID
+    assert lib.gsc_summate(3, 2, 2, None, 1, 1, p, 1, p, 1, p, 1, 1, p, 1, None) == \
+        _native.GSC_ERR_INVALID
+    bad = _native.make_opts()
+    bad.out_dtype = 7
+    assert lib.gsc_summate(3, 2, 2, p, 2, 1, p, 1, p, 1, p, 2, 1, p, 1, ctypes.byref(bad)) == \
+        _native.GSC_ERR_INVALID
+    assert lib.gsc_summate(0, 2, 2, p, 2, 1, p, 1, p, 1, p, 2, 1, p, 1, None) == \
+        _native.GSC_ERR_INVALID
+    assert lib.gsc_summate(17, 2, 2, p, 2, 1, p, 1, p, 1, p, 2, 1, p, 1, None) == \
+        _native.GSC_ERR_DIM
+    # device form: workspace too small, pitch below N
+    assert lib.gsc_summate_dev(3, 100, 10, p, p, p, p, 10, p, p, 16, None, None) == \
+        _native.GSC_ERR_WORKSPACE
+    assert lib.gsc_summate_dev(3, 1, 10, p, p, p, p, 5, p, p, 4096, None, None) == \
+        _native.GSC_ERR_INVALID
+
+
+def test_no_silent_cpu_fallback_without_gpu():
+    if _native.lib().gsc_device_count() > 0:
+        pytest.skip("a GPU is visible; the GPU tier covers this path")
+    cov, z1, z2, pos = _ok_args()
+    with pytest.raises(RuntimeError):
+        gstools_core.summate(cov, z1, z2, pos)
+
+
+def test_workload_generators_are_seeded_and_shaped():
+    a = workloads.c1(n=1000)
+    b = workloads.c1(n=1000)
+    np.testing.assert_array_equal(a.modes.cov_samples, b.modes.cov_samples)
+    np.testing.assert_array_equal(a.pos, b.pos)
+    assert a.modes.cov_samples.shape == (2, 1000) and a.pos.shape == (2, 1000)
+    assert 0.0 <= a.pos.min() and a.pos.max() <= 100.0
+    # k ~ N(0, 1/l^2): sample std close to 1/l = 0.1
+    assert abs(a.modes.cov_samples.std() - 0.1) < 0.01
+    c2 = workloads.c2(n=10, m=1000)
+    assert c2.modes.cov_samples.shape == (3, 1000)
+    assert abs(c2.modes.cov_samples.std() - 0.05) < 0.005
+    c5 = workloads.c5(side=8, m=4000)
+    assert c5.n_points == 512
+    assert np.abs(c5.modes.cov_samples).max() > 10.0  # heavy tail
+    ens = workloads.c4(count=6)
+    assert len(ens) == 6
+    for w in ens:
+        assert all(64 <= len(ax) <= 1024 for ax in w.axes)
+        assert 5.0 <= w.modes.len_scale <= 50.0
+    # different realisations draw different modes
+    assert not np.array_equal(ens[0].modes.z1, ens[1].modes.z1)
+
+
+def test_grid_point_expansion():
+    axes = [np.arange(3.0), np.arange(4.0) * 2, np.arange(5.0) - 1]
+    idx = np.array([0, 7, 59])
+    pts = workloads.grid_points(axes, idx)
+    full = np.stack(np.meshgrid(*axes, indexing="ij")).reshape(3, -1)
+    np.testing.assert_array_equal(pts, full[:, idx])
