@@ -53,6 +53,71 @@ def cpu_baseline(w, n_sample: int) -> dict:
     }
 
 
+def time_steps(step, steps: int, warmup: int, stream):
+    """Per-step times (ms) of `step` on `stream`, HIP events around each step."""
+    import torch
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    ev[0].record(stream)
+    for i in range(steps):
+        step()
+        ev[i + 1].record(stream)
+    torch.cuda.synchronize()
+    return [ev[i].elapsed_time(ev[i + 1]) for i in range(steps)], ev[0].elapsed_time(ev[-1])
+
+
+def grid_leg(steps: int, warmup: int) -> dict:
+    """Structured 256^3 grid, M = 1000 (workload c3): factored MFMA vs direct kernel."""
+    import numpy as np
+    import torch
+
+    from gstools_core import grid, workloads
+
+    w = workloads.c3()
+    md = w.modes
+    dev = torch.device("cuda:0")
+    cov = torch.from_numpy(np.ascontiguousarray(md.cov_samples)).to(dev)
+    z1 = torch.from_numpy(md.z1).to(dev)
+    z2 = torch.from_numpy(md.z2).to(dev)
+    axes = [torch.from_numpy(a).to(dev) for a in w.axes]
+    shape = [len(a) for a in w.axes]
+    n = w.n_points
+    out = torch.empty(n, dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream()
+    res = {"workload": f"3-D grid {shape[0]}x{shape[1]}x{shape[2]}, M={md.m} Gaussian (l=16), "
+                       "FP32 output", "n_points": n, "n_modes": md.m}
+    for path, prec, acc in (("factored", "fp16x3", "f64"), ("factored", "fp16x3", "f32x2"),
+                            ("factored", "fp16x3", "f32"), ("factored", "bf16x6", "f64"),
+                            ("factored", "fp16x1", "f32"), ("direct", "auto", "f64")):
+        nb = grid.grid_workspace_bytes(3, md.m, shape, path=path, precision=prec)
+        work = torch.empty(nb, dtype=torch.uint8, device=dev)
+
+        def step():
+            grid.summate_grid_dev(3, md.m, cov.data_ptr(), z1.data_ptr(), z2.data_ptr(),
+                                  [a.data_ptr() for a in axes], shape, out.data_ptr(),
+                                  work.data_ptr(), nb, path=path, precision=prec, out_f32=True,
+                                  stream=stream.cuda_stream, accum=acc)
+
+        st = steps if path == "factored" else max(3, steps // 4)
+        per, total = time_steps(step, st, warmup, stream)
+        ms = total / st
+        key = f"{path}_{prec}_{acc}" if path == "factored" else "direct"
+        res[key] = {"ms_per_step": ms, "point_modes_per_s": n * md.m / (ms * 1e-3)}
+        if path == "factored":
+            nprod = {"fp16x3": 3, "bf16x6": 6, "fp16x1": 1}[prec]
+            # 2 flops per real MAC, K = 2M -> 4 M flops per point per product
+            tflops = 4.0 * n * md.m * nprod / (ms * 1e-3) / 1e12
+            res[key]["mfma_tflops"] = tflops
+            res[key]["mfma_frac_of_2500"] = tflops / 2500.0
+        del work
+    res["factored_fp16x3_over_direct"] = (res["direct"]["ms_per_step"] /
+                                          res["factored_fp16x3_f64"]["ms_per_step"])
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -62,6 +127,7 @@ def main() -> int:
                     help="points of the workload timed on the CPU oracle (0 = skip)")
     ap.add_argument("--n", type=int, default=10_000_000)
     ap.add_argument("--m", type=int, default=1000)
+    ap.add_argument("--no-grid", action="store_true", help="skip the structured-grid leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -128,6 +194,11 @@ def main() -> int:
                            "1 v_fma_f32 each",
     }
 
+    grid_res = None
+    if not args.no_grid:
+        log("grid leg (256^3, factored vs direct)")
+        grid_res = grid_leg(args.steps, args.warmup)
+
     cpu = None
     if args.cpu_sample > 0:
         log(f"cpu baseline on {args.cpu_sample} points")
@@ -151,6 +222,7 @@ def main() -> int:
                    "path": "direct", "n_points": n, "n_modes": m, "dim": d},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "structured_grid": grid_res,
     }
     print(json.dumps(line), flush=True)
     return 0

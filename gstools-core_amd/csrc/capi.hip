@@ -17,6 +17,7 @@
 #include <thread>
 #include <vector>
 
+#include "gsc_factored.h"
 #include "gsc_internal.h"
 #include "gstools_core_amd.h"
 
@@ -283,6 +284,63 @@ int dev_common(int kind, int dim, int64_t M, int64_t N, const double* sf, const 
   return GSC_OK;
 }
 
+// ---- structured grids -------------------------------------------------------
+
+int validate_grid(int dim, int64_t M, const int64_t* shape, const void* axes) {
+  if (dim < 1 || dim > 3)
+    return fail(GSC_ERR_DIM, "grid entry points support dim 1..3 (got %d)", dim);
+  if (M < 0) return fail(GSC_ERR_INVALID, "negative M");
+  if (M > (int64_t)0x7fffffff) return fail(GSC_ERR_INVALID, "M=%lld exceeds 2^31-1", (long long)M);
+  if (!shape || !axes) return fail(GSC_ERR_INVALID, "null shape or axes");
+  for (int i = 0; i < dim; ++i)
+    if (shape[i] < 0) return fail(GSC_ERR_INVALID, "negative axis length");
+  return GSC_OK;
+}
+
+int64_t grid_points(int dim, const int64_t* shape) {
+  int64_t n = 1;
+  for (int i = 0; i < dim; ++i) n *= shape[i];
+  return n;
+}
+
+// Path choice: the factored contraction needs a 2-D or 3-D grid; below a
+// 16-point last axis or 16 rows its 128 x 128 tiles are mostly padding.
+int choose_grid_path(int dim, const int64_t* shape, const gsc_opts* o) {
+  if (dim == 1) return GSC_PATH_DIRECT;
+  if (o->path == GSC_PATH_DIRECT || o->path == GSC_PATH_FACTORED) return o->path;
+  const int64_t cols = shape[dim - 1];
+  const int64_t rows = grid_points(dim, shape) / (cols > 0 ? cols : 1);
+  return (cols >= 16 && rows >= 16) ? GSC_PATH_FACTORED : GSC_PATH_DIRECT;
+}
+
+size_t grid_work_bytes(int dim, int64_t M, const int64_t* shape, const gsc_opts* o, int path) {
+  if (path == GSC_PATH_FACTORED) return gsc::plan_factored(dim, M, shape, o->precision).bytes;
+  return gsc::mode_table_bytes(3, M);
+}
+
+int grid_launch(int dim, int64_t M, const double* cov, const double* z1, const double* z2,
+                const double* const* axes, const int64_t* shape, void* out, void* work,
+                const gsc_opts* o, int path, hipStream_t s) {
+  GSC_HIP(gsc::launch_prep(gsc::KIND_SCALAR, dim, M, cov, z1, z2, nullptr, work, s));
+  if (path == GSC_PATH_FACTORED) {
+    const gsc::FactoredPlan p = gsc::plan_factored(dim, M, shape, o->precision);
+    GSC_HIP(gsc::launch_factored(p, M, axes, work, out, o->out_dtype, o->scale, o->accum, s));
+  } else {
+    GSC_HIP(gsc::launch_direct_grid(dim, M, shape, axes, work, out, o->out_dtype, o->scale, s));
+  }
+  return GSC_OK;
+}
+
+int validate_precision(const gsc_opts* o) {
+  if (o->precision < GSC_PREC_AUTO || o->precision > GSC_PREC_BF16X1)
+    return fail(GSC_ERR_INVALID, "unknown precision %d", o->precision);
+  if (o->path < GSC_PATH_AUTO || o->path > GSC_PATH_FACTORED)
+    return fail(GSC_ERR_INVALID, "unknown path %d", o->path);
+  if (o->accum < GSC_ACCUM_F64 || o->accum > GSC_ACCUM_F32)
+    return fail(GSC_ERR_INVALID, "unknown accumulation mode %d", o->accum);
+  return GSC_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -358,6 +416,128 @@ int gsc_summate_fourier_dev(int dim, int64_t M, int64_t N, const double* sf, con
                             void* stream) {
   return dev_common(gsc::KIND_FOURIER, dim, M, N, sf, modes, z1, z2, pos, pos_ld, out, 0, work,
                     work_bytes, opts, stream);
+}
+
+int gsc_grid_path(int dim, int64_t M, const int64_t* shape, const gsc_opts* opts) {
+  gsc_opts tmp;
+  const gsc_opts* o = opts_or_default(opts, &tmp);
+  if (validate_grid(dim, M, shape, shape) || validate_precision(o)) return -1;
+  const int path = choose_grid_path(dim, shape, o);
+  if (path == GSC_PATH_FACTORED && dim < 2) return -1;
+  return path;
+}
+
+size_t gsc_grid_workspace_bytes(int dim, int64_t M, const int64_t* shape, const gsc_opts* opts) {
+  gsc_opts tmp;
+  const gsc_opts* o = opts_or_default(opts, &tmp);
+  if (validate_grid(dim, M, shape, shape) || validate_precision(o)) return 0;
+  return grid_work_bytes(dim, M, shape, o, choose_grid_path(dim, shape, o));
+}
+
+int gsc_summate_grid(int dim, int64_t M, const double* cov, int64_t cov_sd, int64_t cov_sm,
+                     const double* z1, int64_t z1_s, const double* z2, int64_t z2_s,
+                     const double* const* axes, const int64_t* axis_s, const int64_t* shape,
+                     void* out, const gsc_opts* opts) {
+  gsc_opts tmp;
+  const gsc_opts* o = opts_or_default(opts, &tmp);
+  int rc = validate_grid(dim, M, shape, axes);
+  if (rc || (rc = validate_opts(o)) || (rc = validate_precision(o))) return rc;
+  if (M > 0 && (!cov || !z1 || !z2)) return fail(GSC_ERR_INVALID, "null mode array");
+  const int path = choose_grid_path(dim, shape, o);
+  if (path == GSC_PATH_FACTORED && dim < 2)
+    return fail(GSC_ERR_INVALID, "the factored path needs a 2-D or 3-D grid");
+  const int64_t N = grid_points(dim, shape);
+  if (N == 0) return GSC_OK;
+  if (!out) return fail(GSC_ERR_INVALID, "null output");
+  for (int i = 0; i < dim; ++i)
+    if (!axes[i]) return fail(GSC_ERR_INVALID, "null axis %d", i);
+  const size_t elt = o->out_dtype == GSC_F32 ? 4 : 8;
+  try {
+    const int64_t Mm = std::max<int64_t>(M, 1);
+    std::vector<double> modes((size_t)((dim + 2) * Mm), 0.0);
+    if (M > 0) {
+      gather2(cov, dim, M, cov_sd, cov_sm, modes.data(), 1);
+      gather1(z1, M, z1_s, modes.data() + dim * M);
+      gather1(z2, M, z2_s, modes.data() + (dim + 1) * M);
+    }
+    int64_t ax_total = 0;
+    for (int i = 0; i < dim; ++i) ax_total += shape[i];
+    std::vector<double> ax((size_t)std::max<int64_t>(ax_total, 1));
+    int64_t off = 0;
+    for (int i = 0; i < dim; ++i) {
+      gather1(axes[i], shape[i], axis_s ? axis_s[i] : 1, ax.data() + off);
+      off += shape[i];
+    }
+    int dev = 0;
+    if ((rc = resolve_device(o, &dev))) return rc;
+    DeviceCtx& c = g_dev[dev];
+    std::lock_guard<std::mutex> lock(c.mu);
+    GSC_HIP(hipSetDevice(dev));
+    if ((rc = check_arch(dev))) return rc;
+    if (!c.stream) GSC_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    const size_t b_work = align256(grid_work_bytes(dim, M, shape, o, path));
+    const size_t b_modes = align256(sizeof(double) * modes.size());
+    const size_t b_ax = align256(sizeof(double) * ax.size());
+    const size_t b_out = align256(elt * N);
+    const size_t need = b_work + b_modes + b_ax + b_out;
+    if (need > c.cap) {
+      if (c.buf) GSC_HIP(hipFree(c.buf));
+      c.buf = nullptr;
+      c.cap = 0;
+      if (hipMalloc(&c.buf, need) != hipSuccess)
+        return fail(GSC_ERR_ALLOC, "hipMalloc of %zu bytes failed", need);
+      c.cap = need;
+    }
+    char* base = static_cast<char*>(c.buf);
+    void* d_work = base;
+    double* d_modes = reinterpret_cast<double*>(base + b_work);
+    double* d_ax = reinterpret_cast<double*>(base + b_work + b_modes);
+    void* d_out = base + b_work + b_modes + b_ax;
+    GSC_HIP(hipMemcpyAsync(d_modes, modes.data(), sizeof(double) * modes.size(),
+                           hipMemcpyHostToDevice, c.stream));
+    GSC_HIP(hipMemcpyAsync(d_ax, ax.data(), sizeof(double) * ax.size(), hipMemcpyHostToDevice,
+                           c.stream));
+    const double* d_axes[3] = {nullptr, nullptr, nullptr};
+    off = 0;
+    for (int i = 0; i < dim; ++i) {
+      d_axes[i] = d_ax + off;
+      off += shape[i];
+    }
+    if ((rc = grid_launch(dim, M, d_modes, d_modes + dim * M, d_modes + (dim + 1) * M, d_axes,
+                          shape, d_out, d_work, o, path, c.stream)))
+      return rc;
+    GSC_HIP(hipMemcpyAsync(out, d_out, elt * N, hipMemcpyDeviceToHost, c.stream));
+    GSC_HIP(hipStreamSynchronize(c.stream));
+  } catch (const std::bad_alloc&) {
+    return fail(GSC_ERR_ALLOC, "host allocation failed");
+  } catch (...) {
+    return fail(GSC_ERR_INVALID, "unexpected host exception");
+  }
+  return GSC_OK;
+}
+
+int gsc_summate_grid_dev(int dim, int64_t M, const double* cov, const double* z1,
+                         const double* z2, const double* const* axes, const int64_t* shape,
+                         void* out, void* work, size_t work_bytes, const gsc_opts* opts,
+                         void* stream) {
+  gsc_opts tmp;
+  const gsc_opts* o = opts_or_default(opts, &tmp);
+  int rc = validate_grid(dim, M, shape, axes);
+  if (rc || (rc = validate_opts(o)) || (rc = validate_precision(o))) return rc;
+  if (M > 0 && (!cov || !z1 || !z2)) return fail(GSC_ERR_INVALID, "null mode array");
+  const int path = choose_grid_path(dim, shape, o);
+  if (path == GSC_PATH_FACTORED && dim < 2)
+    return fail(GSC_ERR_INVALID, "the factored path needs a 2-D or 3-D grid");
+  const size_t need = grid_work_bytes(dim, M, shape, o, path);
+  if (!work || work_bytes < need)
+    return fail(GSC_ERR_WORKSPACE, "workspace of %zu bytes is below the %zu required", work_bytes,
+                need);
+  if (grid_points(dim, shape) == 0) return GSC_OK;
+  if (!out) return fail(GSC_ERR_INVALID, "null output");
+  for (int i = 0; i < dim; ++i)
+    if (!axes[i]) return fail(GSC_ERR_INVALID, "null axis %d", i);
+  return grid_launch(dim, M, cov, z1, z2, axes, shape, out, work, o, path,
+                     static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

@@ -15,6 +15,7 @@
 // as operands for free.  Positions are (dim, N) row-major: coalesced FP64
 // loads, once per point, kept in VGPRs for the whole mode loop.
 #include "gsc_internal.h"
+#include "gsc_factored.h"
 
 #include <hip/hip_runtime.h>
 
@@ -88,23 +89,13 @@ __global__ void k_prep_generic(int kind, int dim, int64_t M, const double* __res
 // K1: scalar summator.  P points per thread (strided by the block size so
 // every load and store is coalesced); grid = ceil(N / (256 P)) blocks.
 // ---------------------------------------------------------------------------
-template <int D, int P, typename OutT>
-__global__ void __launch_bounds__(kBlock)
-k_direct_scalar(const ModeRec* __restrict__ modes, int M, const double* __restrict__ pos,
-                int64_t pos_ld, int64_t N, OutT* __restrict__ out, double scale) {
-  const int64_t base = static_cast<int64_t>(blockIdx.x) * (kBlock * P) + threadIdx.x;
-  double x[P][D];
-#pragma unroll
-  for (int p = 0; p < P; ++p) {
-    int64_t j = base + p * kBlock;
-    j = j < N ? j : N - 1;  // tail lanes recompute the last point, never store it
-#pragma unroll
-    for (int i = 0; i < D; ++i) x[p][i] = pos[i * pos_ld + j];
-  }
-  double tot[P];
+// Mode loop shared by the point-list and grid forms: tot[p] = sum over all
+// modes of amp0 sin(2 pi (kap . x_p + th)).
+template <int D, int P>
+__device__ __forceinline__ void sum_modes_scalar(const ModeRec* __restrict__ modes, int M,
+                                                 const double (&x)[P][D], double (&tot)[P]) {
 #pragma unroll
   for (int p = 0; p < P; ++p) tot[p] = 0.0;
-
   // Software-pipelined mode loads: the record for m + 1 is requested while
   // mode m is being summed, so the scalar-load latency hides behind VALU work.
   ModeRec nxt;
@@ -128,6 +119,57 @@ k_direct_scalar(const ModeRec* __restrict__ modes, int M, const double* __restri
 #pragma unroll
     for (int p = 0; p < P; ++p) tot[p] += static_cast<double>(acc[p]);
   }
+}
+
+template <int D, int P, typename OutT>
+__global__ void __launch_bounds__(kBlock)
+k_direct_scalar(const ModeRec* __restrict__ modes, int M, const double* __restrict__ pos,
+                int64_t pos_ld, int64_t N, OutT* __restrict__ out, double scale) {
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * (kBlock * P) + threadIdx.x;
+  double x[P][D];
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    int64_t j = base + p * kBlock;
+    j = j < N ? j : N - 1;  // tail lanes recompute the last point, never store it
+#pragma unroll
+    for (int i = 0; i < D; ++i) x[p][i] = pos[i * pos_ld + j];
+  }
+  double tot[P];
+  sum_modes_scalar<D, P>(modes, M, x, tot);
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    const int64_t j = base + p * kBlock;
+    if (j < N) store(out + j, tot[p] * scale);
+  }
+}
+
+// Grid form: point j of the C-order grid (s0, s1, s2) has coordinates
+// (ax0[j / (s1 s2)], ax1[(j / s2) % s1], ax2[j % s2]) (fewer axes for D < 3).
+template <int D, int P, typename OutT>
+__global__ void __launch_bounds__(kBlock)
+k_direct_grid(const ModeRec* __restrict__ modes, int M, const double* __restrict__ ax0,
+              const double* __restrict__ ax1, const double* __restrict__ ax2, int64_t s1,
+              int64_t s2, int64_t N, OutT* __restrict__ out, double scale) {
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * (kBlock * P) + threadIdx.x;
+  double x[P][D];
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    int64_t j = base + p * kBlock;
+    j = j < N ? j : N - 1;
+    if constexpr (D == 1) {
+      x[p][0] = ax0[j];
+    } else if constexpr (D == 2) {
+      x[p][0] = ax0[j / s1];
+      x[p][1] = ax1[j % s1];
+    } else {
+      const int64_t i0 = j / (s1 * s2), r = j - i0 * (s1 * s2);
+      x[p][0] = ax0[i0];
+      x[p][1] = ax1[r / s2];
+      x[p][2] = ax2[r % s2];
+    }
+  }
+  double tot[P];
+  sum_modes_scalar<D, P>(modes, M, x, tot);
 #pragma unroll
   for (int p = 0; p < P; ++p) {
     const int64_t j = base + p * kBlock;
@@ -305,6 +347,30 @@ hipError_t launch_direct_incompr(int dim, int64_t M, int64_t N, const void* tabl
                : incompr_t<2, double>(M, N, table, pos, pos_ld, out, out_ld, scale, s);
   return f32 ? incompr_t<3, float>(M, N, table, pos, pos_ld, out, out_ld, scale, s)
              : incompr_t<3, double>(M, N, table, pos, pos_ld, out, out_ld, scale, s);
+}
+
+hipError_t launch_direct_grid(int dim, int64_t M, const int64_t* shape, const double* const* axes,
+                              const void* table, void* out, int out_dtype, double scale,
+                              hipStream_t s) {
+  int64_t N = 1;
+  for (int i = 0; i < dim; ++i) N *= shape[i];
+  if (N <= 0) return hipSuccess;
+  constexpr int P = 8;
+  const unsigned g = grid_for(N, kBlock * P);
+  const ModeRec* md = static_cast<const ModeRec*>(table);
+  const int Mi = static_cast<int>(M);
+  const int64_t s1 = dim > 1 ? shape[1] : 1, s2 = dim > 2 ? shape[2] : 1;
+  const double* a0 = axes[0];
+  const double* a1 = dim > 1 ? axes[1] : nullptr;
+  const double* a2 = dim > 2 ? axes[2] : nullptr;
+#define GSC_GRID(D, T) \
+  k_direct_grid<D, P, T><<<g, kBlock, 0, s>>>(md, Mi, a0, a1, a2, s1, s2, N, static_cast<T*>(out), scale)
+  const bool f32 = out_dtype == 1;
+  if (dim == 1) { if (f32) GSC_GRID(1, float); else GSC_GRID(1, double); }
+  else if (dim == 2) { if (f32) GSC_GRID(2, float); else GSC_GRID(2, double); }
+  else { if (f32) GSC_GRID(3, float); else GSC_GRID(3, double); }
+#undef GSC_GRID
+  return hipGetLastError();
 }
 
 }  // namespace gsc

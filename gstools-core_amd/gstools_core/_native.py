@@ -39,7 +39,7 @@ class GscOpts(ctypes.Structure):
         ("precision", ctypes.c_int32),
         ("device", ctypes.c_int32),
         ("num_threads", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("accum", ctypes.c_int32),
         ("scale", ctypes.c_double),
     ]
 
@@ -49,6 +49,8 @@ _int = ctypes.c_int
 _vp = ctypes.c_void_p
 _sz = ctypes.c_size_t
 _optp = ctypes.POINTER(GscOpts)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_vpp = ctypes.POINTER(ctypes.c_void_p)
 
 # name -> (restype, argtypes); every function declared in the public header.
 SIGNATURES = {
@@ -69,6 +71,12 @@ SIGNATURES = {
                                        _sz, _optp, _vp]),
     "gsc_summate_fourier_dev": (_int, [_int, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp,
                                        _sz, _optp, _vp]),
+    "gsc_grid_path": (_int, [_int, _i64, _i64p, _optp]),
+    "gsc_grid_workspace_bytes": (_sz, [_int, _i64, _i64p, _optp]),
+    "gsc_summate_grid": (_int, [_int, _i64, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _vpp, _i64p,
+                                _i64p, _vp, _optp]),
+    "gsc_summate_grid_dev": (_int, [_int, _i64, _vp, _vp, _vp, _vpp, _i64p, _vp, _vp, _sz, _optp,
+                                    _vp]),
 }
 
 _lib = None
@@ -117,8 +125,11 @@ def lib():
     return _lib
 
 
+ACCUM = {"f64": 0, "f32x2": 1, "f32": 2}
+
+
 def make_opts(out_dtype=GSC_F64, scale=1.0, device=-1, num_threads=0, path=GSC_PATH_AUTO,
-              precision=0) -> GscOpts:
+              precision=0, accum=0) -> GscOpts:
     o = GscOpts()
     lib().gsc_opts_init(ctypes.byref(o))
     o.out_dtype = out_dtype
@@ -127,6 +138,7 @@ def make_opts(out_dtype=GSC_F64, scale=1.0, device=-1, num_threads=0, path=GSC_P
     o.num_threads = int(num_threads)
     o.path = int(path)
     o.precision = int(precision)
+    o.accum = int(accum)
     return o
 
 

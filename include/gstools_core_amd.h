@@ -66,13 +66,21 @@ typedef enum gsc_precision {
   GSC_PREC_BF16X1 = 5     /* bf16, 1 product (fast, ~1e-2 sigma) */
 } gsc_precision;
 
+/* Accumulation of the factored path's FP32 MFMA partial sums, which are
+ * flushed every 64 modes into a second accumulator set. */
+typedef enum gsc_accum {
+  GSC_ACCUM_F64 = 0,    /* second set in FP64 (default, most accurate) */
+  GSC_ACCUM_F32X2 = 1,  /* second set in FP32 (fewer registers) */
+  GSC_ACCUM_F32 = 2     /* one FP32 set over all modes (error grows with M) */
+} gsc_accum;
+
 typedef struct gsc_opts {
   int32_t out_dtype;    /* gsc_dtype of `out` (default GSC_F64, as the reference) */
   int32_t path;         /* gsc_path, structured entry points only */
   int32_t precision;    /* gsc_precision, factored path only */
   int32_t device;       /* HIP device ordinal, -1 = current device */
   int32_t num_threads;  /* host threads for staging strided arrays, 0 = all */
-  int32_t reserved;
+  int32_t accum;        /* gsc_accum, factored path only */
   double scale;         /* multiplies every output value (1.0 = reference) */
 } gsc_opts;
 
@@ -158,6 +166,37 @@ int gsc_summate_fourier_dev(int dim, int64_t M, int64_t N, const double* sf, con
                             const double* z1, const double* z2, const double* pos, int64_t pos_ld,
                             void* out, void* work, size_t work_bytes, const gsc_opts* opts,
                             void* stream);
+
+/*
+ * Structured grids.  The field on the C-order grid axes[0] x ... x
+ * axes[dim-1] (shape[i] coordinates on axis i, dim in {1, 2, 3}) equals
+ * gsc_summate on the expanded point list
+ *   pos[:, j] = (axes[0][i0], axes[1][i1], axes[2][i2]),  j = (i0 * shape[1] + i1) * shape[2] + i2
+ * which is how GSTools calls summate for structured meshes (it expands the
+ * mesh in Python first).  These entry points have no counterpart in the
+ * reference: they are the grid front-end of gsc_summate that avoids the
+ * expansion.  opts->path selects the per-point kernel (GSC_PATH_DIRECT) or
+ * the factored MFMA contraction (GSC_PATH_FACTORED, dim 2 or 3, precision
+ * from opts->precision); GSC_PATH_AUTO lets the library choose.  `out` is the
+ * dense C-order grid (prod(shape) values of opts->out_dtype).
+ *
+ * gsc_grid_path returns the path (GSC_PATH_DIRECT / GSC_PATH_FACTORED) the
+ * library takes for these arguments, or -1 on invalid arguments.
+ */
+int gsc_grid_path(int dim, int64_t M, const int64_t* shape, const gsc_opts* opts);
+size_t gsc_grid_workspace_bytes(int dim, int64_t M, const int64_t* shape, const gsc_opts* opts);
+int gsc_summate_grid(int dim, int64_t M,
+                     const double* cov, int64_t cov_sd, int64_t cov_sm,
+                     const double* z1, int64_t z1_s,
+                     const double* z2, int64_t z2_s,
+                     const double* const* axes, const int64_t* axis_s, const int64_t* shape,
+                     void* out, const gsc_opts* opts);
+/* Device form: cov (dim, M), z1, z2, every axes[i] and out are device
+ * pointers (`axes` itself is a host array of dim device pointers). */
+int gsc_summate_grid_dev(int dim, int64_t M, const double* cov, const double* z1,
+                         const double* z2, const double* const* axes, const int64_t* shape,
+                         void* out, void* work, size_t work_bytes, const gsc_opts* opts,
+                         void* stream);
 
 #ifdef __cplusplus
 }
