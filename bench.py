@@ -199,6 +199,17 @@ def main() -> int:
         log("grid leg (256^3, factored vs direct)")
         grid_res = grid_leg(args.steps, args.warmup)
 
+    # the matrix-core-phase variant of the same kernel, for comparison
+    def step_mfma():
+        device.summate_dev(d, m, n, cov.data_ptr(), z1.data_ptr(), z2.data_ptr(), pos.data_ptr(),
+                           n, out.data_ptr(), work.data_ptr(), nb, stream=sh, variant="mfma")
+
+    _, tot_mfma = time_steps(step_mfma, max(3, args.steps // 2), 1, stream)
+    mfma_ms = tot_mfma / max(3, args.steps // 2)
+    direct_variants = {"valu_phase": {"ms_per_step": ms_per_step, "point_modes_per_s": value},
+                       "mfma_phase": {"ms_per_step": mfma_ms,
+                                      "point_modes_per_s": n * m / (mfma_ms * 1e-3)}}
+
     cpu = None
     if args.cpu_sample > 0:
         log(f"cpu baseline on {args.cpu_sample} points")
@@ -222,6 +233,7 @@ def main() -> int:
                    "path": "direct", "n_points": n, "n_modes": m, "dim": d},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "direct_variants": direct_variants,
         "structured_grid": grid_res,
     }
     print(json.dumps(line), flush=True)

@@ -28,9 +28,15 @@ FLAGS = [
 ]
 
 
+# Per-file flags.  direct.hip: MFMA results go to arch VGPRs (its FP64 phase
+# tiles feed VALU work directly; AGPR results would cost two v_accvgpr_read per
+# phase).  The factored GEMM needs more than 256 registers and keeps AGPRs.
+FILE_FLAGS = {"direct.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(OBJDIR, os.path.basename(src).replace(".hip", ".o"))
-    cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
+    cmd = [HIPCC, *FLAGS, *FILE_FLAGS.get(os.path.basename(src), []), "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")

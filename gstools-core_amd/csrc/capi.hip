@@ -106,6 +106,8 @@ int validate_common(int dim, int64_t M, int64_t N, const double* cov, const doub
 int validate_opts(const gsc_opts* o) {
   if (o->out_dtype != GSC_F64 && o->out_dtype != GSC_F32)
     return fail(GSC_ERR_INVALID, "out_dtype must be GSC_F64 or GSC_F32");
+  if (o->direct_variant != GSC_DIRECT_MFMA && o->direct_variant != GSC_DIRECT_VALU)
+    return fail(GSC_ERR_INVALID, "unknown direct_variant %d", o->direct_variant);
   return GSC_OK;
 }
 
@@ -219,10 +221,10 @@ int host_summate(int kind, int dim, int64_t M, int64_t N, const double* sf, int6
                              d_table, c.stream));
     if (kind == gsc::KIND_INCOMPR)
       GSC_HIP(gsc::launch_direct_incompr(dim, M, N, d_table, d_pos, N, d_out, N, o->out_dtype,
-                                         o->scale, c.stream));
+                                         o->scale, o->direct_variant, c.stream));
     else
       GSC_HIP(gsc::launch_direct_scalar(dim, M, N, d_table, d_pos, N, d_out, o->out_dtype,
-                                        o->scale, c.stream));
+                                        o->scale, o->direct_variant, c.stream));
 
     std::vector<char> out_tmp;
     void* host_dst = out;
@@ -277,10 +279,10 @@ int dev_common(int kind, int dim, int64_t M, int64_t N, const double* sf, const 
   GSC_HIP(gsc::launch_prep(kind, dim, M, cov, z1, z2, sf, work, s));
   if (kind == gsc::KIND_INCOMPR)
     GSC_HIP(gsc::launch_direct_incompr(dim, M, N, work, pos, pos_ld, out, out_ld, o->out_dtype,
-                                       o->scale, s));
+                                       o->scale, o->direct_variant, s));
   else
     GSC_HIP(gsc::launch_direct_scalar(dim, M, N, work, pos, pos_ld, out, o->out_dtype, o->scale,
-                                      s));
+                                      o->direct_variant, s));
   return GSC_OK;
 }
 
@@ -315,7 +317,7 @@ int choose_grid_path(int dim, const int64_t* shape, const gsc_opts* o) {
 
 size_t grid_work_bytes(int dim, int64_t M, const int64_t* shape, const gsc_opts* o, int path) {
   if (path == GSC_PATH_FACTORED) return gsc::plan_factored(dim, M, shape, o->precision).bytes;
-  return gsc::mode_table_bytes(3, M);
+  return gsc::mode_table_bytes(dim, M);
 }
 
 int grid_launch(int dim, int64_t M, const double* cov, const double* z1, const double* z2,
@@ -326,7 +328,8 @@ int grid_launch(int dim, int64_t M, const double* cov, const double* z1, const d
     const gsc::FactoredPlan p = gsc::plan_factored(dim, M, shape, o->precision);
     GSC_HIP(gsc::launch_factored(p, M, axes, work, out, o->out_dtype, o->scale, o->accum, s));
   } else {
-    GSC_HIP(gsc::launch_direct_grid(dim, M, shape, axes, work, out, o->out_dtype, o->scale, s));
+    GSC_HIP(gsc::launch_direct_grid(dim, M, shape, axes, work, out, o->out_dtype, o->scale,
+                                    o->direct_variant, s));
   }
   return GSC_OK;
 }
@@ -353,6 +356,8 @@ void gsc_opts_init(gsc_opts* o) {
   o->precision = GSC_PREC_AUTO;
   o->device = -1;
   o->num_threads = 0;
+  o->accum = GSC_ACCUM_F64;
+  o->direct_variant = GSC_DIRECT_VALU;
   o->scale = 1.0;
 }
 

@@ -40,34 +40,54 @@ __device__ __forceinline__ void store(OutT* p, double v) { *p = static_cast<OutT
 // ---------------------------------------------------------------------------
 // Mode preparation (FP64).  One thread per mode.
 // ---------------------------------------------------------------------------
-__global__ void k_prep_small(int kind, int dim, int64_t M, const double* __restrict__ cov,
-                             const double* __restrict__ z1, const double* __restrict__ z2,
-                             const double* __restrict__ sf, ModeRec* __restrict__ table) {
+__global__ void k_prep_small(int kind, int dim, int64_t M, int ntiles,
+                             const double* __restrict__ cov, const double* __restrict__ z1,
+                             const double* __restrict__ z2, const double* __restrict__ sf,
+                             ModeRec* __restrict__ table, double* __restrict__ a_tab,
+                             float* __restrict__ amp_tab) {
   const int64_t m = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (m >= M) return;
-  double k[3] = {0.0, 0.0, 0.0};
-  for (int i = 0; i < dim; ++i) k[i] = cov[i * M + m];
-  const double a = z1[m], b = z2[m];
-  const double A = hypot(a, b);
+  if (m >= static_cast<int64_t>(ntiles) * 16) return;
   ModeRec r;
-  for (int i = 0; i < 3; ++i) r.kap[i] = k[i] * kInv2Pi;
-  r.th = atan2(a, b) * kInv2Pi;
+  r.kap[0] = r.kap[1] = r.kap[2] = 0.0;
+  r.th = 0.0;
   r.amp[0] = r.amp[1] = r.amp[2] = r.amp[3] = 0.0f;
-  if (kind == KIND_SCALAR) {
-    r.amp[0] = static_cast<float>(A);
-  } else if (kind == KIND_FOURIER) {
-    r.amp[0] = static_cast<float>(sf[m] * A);
-  } else {
-    // projector p_c = delta_c0 - k_c k_0 / |k|^2 (field.rs:138-151); k = 0
-    // gives 0/0 = NaN exactly as the reference does.
-    double kk = 0.0;
-    for (int i = 0; i < dim; ++i) kk += k[i] * k[i];
-    for (int c = 0; c < dim; ++c) {
-      const double p = (c == 0 ? 1.0 : 0.0) - (k[c] * k[0]) / kk;
-      r.amp[c] = static_cast<float>(p * A);
+  if (m < M) {  // modes past M pad the last MFMA tile with zeros
+    double k[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < dim; ++i) k[i] = cov[i * M + m];
+    const double a = z1[m], b = z2[m];
+    const double A = hypot(a, b);
+    for (int i = 0; i < 3; ++i) r.kap[i] = k[i] * kInv2Pi;
+    r.th = atan2(a, b) * kInv2Pi;
+    if (kind == KIND_SCALAR) {
+      r.amp[0] = static_cast<float>(A);
+    } else if (kind == KIND_FOURIER) {
+      r.amp[0] = static_cast<float>(sf[m] * A);
+    } else {
+      // projector p_c = delta_c0 - k_c k_0 / |k|^2 (field.rs:138-151); k = 0
+      // gives 0/0 = NaN exactly as the reference does.
+      double kk = 0.0;
+      for (int i = 0; i < dim; ++i) kk += k[i] * k[i];
+      for (int c = 0; c < dim; ++c) {
+        const double pc = (c == 0 ? 1.0 : 0.0) - (k[c] * k[0]) / kk;
+        r.amp[c] = static_cast<float>(pc * A);
+      }
     }
+    table[m] = r;
   }
-  table[m] = r;
+  // MFMA tile tables (tile t = m / 16, row i = m % 16):
+  //   a_tab[t][kk][i]: A operand of v_mfma_f64_16x16x4, lane l = 16 kk + i holds
+  //                    parameter kk of mode i (kap_0..kap_{dim-1}, then th, then 0)
+  //   amp_tab[t][c][4 g + it]: amplitude of component c of mode 4 it + g, the
+  //                    mode in D row 4 it + g seen by lanes 16 g .. 16 g + 15
+  const int t = static_cast<int>(m >> 4), i = static_cast<int>(m & 15);
+  for (int kk = 0; kk < 4; ++kk) {
+    double v = 0.0;
+    if (kk < dim) v = r.kap[kk];
+    else if (kk == dim) v = r.th;
+    a_tab[static_cast<int64_t>(t) * 64 + kk * 16 + i] = v;
+  }
+  const int it = i >> 2, g = i & 3;
+  for (int c = 0; c < 3; ++c) amp_tab[(static_cast<int64_t>(t) * 3 + c) * 16 + 4 * g + it] = r.amp[c];
 }
 
 // dim > 3 (scalar and Fourier summators only): [th, amp, kap_0..kap_{d-1}].
@@ -265,6 +285,130 @@ k_direct_incompr(const ModeRec* __restrict__ modes, int M, const double* __restr
   }
 }
 
+// ---------------------------------------------------------------------------
+// K1m / K2m: direct summators with the FP64 phase on the matrix cores.
+//
+// One v_mfma_f64_16x16x4f64 per 16 modes x 16 points computes every phase
+//   t(i, j) = sum_kk A[i][kk] B[kk][j] = kap_i . x_j + th_i   (exact FP64 FMA
+// chains, K = 4 = three coordinates + the constant 1 for th).  Operand and
+// result layouts (verified on gfx950):
+//   A: lane l holds A[l % 16][l / 16]   (mode l % 16, parameter l / 16)
+//   B: lane l holds B[l / 16][l % 16]   (coordinate l / 16 of point l % 16)
+//   D: lane l, item it holds D[4 it + l / 16][l % 16]
+// so a lane sees 4 modes (4 it + g, g = l / 16) of one point (j = l % 16).
+// The VALU keeps only fract + cvt + v_sin_f32 + FMA per point-mode, and the
+// matrix core runs the phase work concurrently.  A wave owns P groups of 16
+// points; partial sums of the four lane groups are combined at the end.
+// ---------------------------------------------------------------------------
+typedef double f64x4_t __attribute__((ext_vector_type(4)));
+
+template <int D, int NC, int P, bool GRID, typename OutT>
+__global__ void __launch_bounds__(kBlock)
+k_direct_mfma(const double* __restrict__ a_tab, const float* __restrict__ amp_tab, int ntiles,
+              const double* __restrict__ pos, int64_t pos_ld, const double* __restrict__ ax1,
+              const double* __restrict__ ax2, int64_t s1, int64_t s2, int64_t N,
+              OutT* __restrict__ out, int64_t out_ld, double scale) {
+  const int lane = threadIdx.x & 63;
+  const int g = lane >> 4, j = lane & 15;
+  const int64_t base =
+      (static_cast<int64_t>(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6)) * (16 * P);
+  // B operands: coordinate g of point base + 16 p + j (1 for g == D, 0 above)
+  double b[P];
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    int64_t pt = base + 16 * p + j;
+    pt = pt < N ? pt : N - 1;
+    double v = g == D ? 1.0 : 0.0;
+    if (g < D) {
+      if constexpr (!GRID) {
+        v = pos[g * pos_ld + pt];
+      } else if constexpr (D == 1) {
+        v = pos[pt];
+      } else if constexpr (D == 2) {
+        v = g == 0 ? pos[pt / s1] : ax1[pt % s1];
+      } else {
+        const int64_t i0 = pt / (s1 * s2), r = pt - i0 * (s1 * s2);
+        v = g == 0 ? pos[i0] : (g == 1 ? ax1[r / s2] : ax2[r % s2]);
+      }
+    }
+    b[p] = v;
+  }
+  double tot[P][NC];
+#pragma unroll
+  for (int p = 0; p < P; ++p)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) tot[p][c] = 0.0;
+
+  const f64x4_t zero = {0.0, 0.0, 0.0, 0.0};
+  // software-pipelined tile loads
+  double a_nxt = 0.0;
+  float4 am_nxt[NC];
+  auto load_tile = [&](int t) {
+    a_nxt = a_tab[static_cast<int64_t>(t) * 64 + lane];
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      am_nxt[c] = *reinterpret_cast<const float4*>(amp_tab + (static_cast<int64_t>(t) * 3 + c) * 16 + 4 * g);
+  };
+  if (ntiles > 0) load_tile(0);
+  constexpr int kTileChunk = 8;  // 8 tiles = 32 modes per lane per FP32 partial sum
+  for (int t0 = 0; t0 < ntiles; t0 += kTileChunk) {
+    const int t1 = min(t0 + kTileChunk, ntiles);
+    float acc[P][NC];
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) acc[p][c] = 0.0f;
+    for (int t = t0; t < t1; ++t) {
+      const double a = a_nxt;
+      float4 am[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) am[c] = am_nxt[c];
+      load_tile(min(t + 1, ntiles - 1));
+      f64x4_t ph[P];
+#pragma unroll
+      for (int p = 0; p < P; ++p) ph[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[p], zero, 0, 0, 0);
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        const float s0 = sin_rev(ph[p][0]), s1v = sin_rev(ph[p][1]);
+        const float s2v = sin_rev(ph[p][2]), s3 = sin_rev(ph[p][3]);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          float v = acc[p][c];
+          v = __builtin_fmaf(am[c].x, s0, v);
+          v = __builtin_fmaf(am[c].y, s1v, v);
+          v = __builtin_fmaf(am[c].z, s2v, v);
+          v = __builtin_fmaf(am[c].w, s3, v);
+          acc[p][c] = v;
+        }
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) tot[p][c] += static_cast<double>(acc[p][c]);
+  }
+  // lanes j, j + 16, j + 32, j + 48 hold partial sums of the same point
+#pragma unroll
+  for (int p = 0; p < P; ++p)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      double v = tot[p][c];
+      v += __shfl_xor(v, 16, 64);
+      v += __shfl_xor(v, 32, 64);
+      tot[p][c] = v;
+    }
+  if (g == 0) {
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      const int64_t pt = base + 16 * p + j;
+      if (pt < N) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) store(out + c * out_ld + pt, tot[p][c] * scale);
+      }
+    }
+  }
+}
+
 inline unsigned grid_for(int64_t n, int per_block) {
   return static_cast<unsigned>((n + per_block - 1) / per_block);
 }
@@ -291,30 +435,70 @@ hipError_t incompr_t(int64_t M, int64_t N, const void* table, const double* pos,
 
 }  // namespace
 
+namespace {
+
+size_t align256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+int mfma_tiles(int64_t M) { return static_cast<int>((M + 15) / 16); }
+
+// Table layout for dim <= 3: [ModeRec x M][a_tab: tiles x 64 f64][amp_tab: tiles x 3 x 16 f32]
+size_t rec_bytes(int64_t M) { return align256(sizeof(ModeRec) * static_cast<size_t>(M > 0 ? M : 1)); }
+size_t atab_bytes(int64_t M) { return align256(sizeof(double) * 64 * static_cast<size_t>(mfma_tiles(M))); }
+size_t amptab_bytes(int64_t M) { return align256(sizeof(float) * 48 * static_cast<size_t>(mfma_tiles(M))); }
+const double* atab_of(const void* t, int64_t M) {
+  return reinterpret_cast<const double*>(static_cast<const char*>(t) + rec_bytes(M));
+}
+const float* amptab_of(const void* t, int64_t M) {
+  return reinterpret_cast<const float*>(static_cast<const char*>(t) + rec_bytes(M) + atab_bytes(M));
+}
+
+template <int D, int NC, bool GRID, typename OutT>
+hipError_t mfma_t(int64_t M, int64_t N, const void* table, const double* pos, int64_t pos_ld,
+                  const double* ax1, const double* ax2, int64_t s1, int64_t s2, void* out,
+                  int64_t out_ld, double scale, hipStream_t s) {
+  constexpr int P = NC == 1 ? 8 : 4;  // points per lane group: 16 P per wave
+  const unsigned g = grid_for(N, (kBlock / 64) * 16 * P);
+  k_direct_mfma<D, NC, P, GRID, OutT><<<g, kBlock, 0, s>>>(
+      atab_of(table, M), amptab_of(table, M), mfma_tiles(M), pos, pos_ld, ax1, ax2, s1, s2, N,
+      static_cast<OutT*>(out), out_ld, scale);
+  return hipGetLastError();
+}
+
+}  // namespace
+
 size_t mode_table_bytes(int dim, int64_t M) {
-  const size_t per = dim <= 3 ? sizeof(ModeRec) : static_cast<size_t>(dim + 2) * sizeof(double);
-  const size_t b = per * static_cast<size_t>(M > 0 ? M : 1);
-  return (b + 255) & ~static_cast<size_t>(255);
+  if (dim > 3) return align256(static_cast<size_t>(dim + 2) * sizeof(double) *
+                               static_cast<size_t>(M > 0 ? M : 1));
+  return rec_bytes(M) + atab_bytes(M) + amptab_bytes(M);
 }
 
 hipError_t launch_prep(int kind, int dim, int64_t M, const double* cov, const double* z1,
                        const double* z2, const double* sf, void* table, hipStream_t stream) {
   if (M <= 0) return hipSuccess;
-  const unsigned g = grid_for(M, 256);
-  if (dim <= 3)
-    k_prep_small<<<g, 256, 0, stream>>>(kind, dim, M, cov, z1, z2, sf,
-                                        static_cast<ModeRec*>(table));
-  else
-    k_prep_generic<<<g, 256, 0, stream>>>(kind, dim, M, cov, z1, z2, sf,
-                                          static_cast<double*>(table));
+  if (dim <= 3) {
+    const int nt = mfma_tiles(M);
+    k_prep_small<<<grid_for(static_cast<int64_t>(nt) * 16, 256), 256, 0, stream>>>(
+        kind, dim, M, nt, cov, z1, z2, sf, static_cast<ModeRec*>(table),
+        const_cast<double*>(atab_of(table, M)), const_cast<float*>(amptab_of(table, M)));
+  } else {
+    k_prep_generic<<<grid_for(M, 256), 256, 0, stream>>>(kind, dim, M, cov, z1, z2, sf,
+                                                          static_cast<double*>(table));
+  }
   return hipGetLastError();
 }
 
 hipError_t launch_direct_scalar(int dim, int64_t M, int64_t N, const void* table,
                                 const double* pos, int64_t pos_ld, void* out, int out_dtype,
-                                double scale, hipStream_t s) {
+                                double scale, int variant, hipStream_t s) {
   if (N <= 0) return hipSuccess;
   const bool f32 = out_dtype == 1;
+  if (dim <= 3 && variant == 0) {
+#define GSC_M(D) (f32 ? mfma_t<D, 1, false, float>(M, N, table, pos, pos_ld, nullptr, nullptr, 1, 1, out, 0, scale, s) \
+                      : mfma_t<D, 1, false, double>(M, N, table, pos, pos_ld, nullptr, nullptr, 1, 1, out, 0, scale, s))
+    if (dim == 1) return GSC_M(1);
+    if (dim == 2) return GSC_M(2);
+    return GSC_M(3);
+#undef GSC_M
+  }
   switch (dim) {
     case 1: return f32 ? scalar_t<1, float>(M, N, table, pos, pos_ld, out, scale, s)
                        : scalar_t<1, double>(M, N, table, pos, pos_ld, out, scale, s);
@@ -339,9 +523,16 @@ hipError_t launch_direct_scalar(int dim, int64_t M, int64_t N, const void* table
 
 hipError_t launch_direct_incompr(int dim, int64_t M, int64_t N, const void* table,
                                  const double* pos, int64_t pos_ld, void* out, int64_t out_ld,
-                                 int out_dtype, double scale, hipStream_t s) {
+                                 int out_dtype, double scale, int variant, hipStream_t s) {
   if (N <= 0) return hipSuccess;
   const bool f32 = out_dtype == 1;
+  if (variant == 0) {
+    if (dim == 2)
+      return f32 ? mfma_t<2, 2, false, float>(M, N, table, pos, pos_ld, nullptr, nullptr, 1, 1, out, out_ld, scale, s)
+                 : mfma_t<2, 2, false, double>(M, N, table, pos, pos_ld, nullptr, nullptr, 1, 1, out, out_ld, scale, s);
+    return f32 ? mfma_t<3, 3, false, float>(M, N, table, pos, pos_ld, nullptr, nullptr, 1, 1, out, out_ld, scale, s)
+               : mfma_t<3, 3, false, double>(M, N, table, pos, pos_ld, nullptr, nullptr, 1, 1, out, out_ld, scale, s);
+  }
   if (dim == 2)
     return f32 ? incompr_t<2, float>(M, N, table, pos, pos_ld, out, out_ld, scale, s)
                : incompr_t<2, double>(M, N, table, pos, pos_ld, out, out_ld, scale, s);
@@ -351,21 +542,29 @@ hipError_t launch_direct_incompr(int dim, int64_t M, int64_t N, const void* tabl
 
 hipError_t launch_direct_grid(int dim, int64_t M, const int64_t* shape, const double* const* axes,
                               const void* table, void* out, int out_dtype, double scale,
-                              hipStream_t s) {
+                              int variant, hipStream_t s) {
   int64_t N = 1;
   for (int i = 0; i < dim; ++i) N *= shape[i];
   if (N <= 0) return hipSuccess;
-  constexpr int P = 8;
-  const unsigned g = grid_for(N, kBlock * P);
   const ModeRec* md = static_cast<const ModeRec*>(table);
   const int Mi = static_cast<int>(M);
   const int64_t s1 = dim > 1 ? shape[1] : 1, s2 = dim > 2 ? shape[2] : 1;
   const double* a0 = axes[0];
   const double* a1 = dim > 1 ? axes[1] : nullptr;
   const double* a2 = dim > 2 ? axes[2] : nullptr;
+  const bool f32 = out_dtype == 1;
+  if (variant == 0) {
+#define GSC_MG(D) (f32 ? mfma_t<D, 1, true, float>(M, N, table, a0, 0, a1, a2, s1, s2, out, 0, scale, s) \
+                       : mfma_t<D, 1, true, double>(M, N, table, a0, 0, a1, a2, s1, s2, out, 0, scale, s))
+    if (dim == 1) return GSC_MG(1);
+    if (dim == 2) return GSC_MG(2);
+    return GSC_MG(3);
+#undef GSC_MG
+  }
+  constexpr int P = 8;
+  const unsigned g = grid_for(N, kBlock * P);
 #define GSC_GRID(D, T) \
   k_direct_grid<D, P, T><<<g, kBlock, 0, s>>>(md, Mi, a0, a1, a2, s1, s2, N, static_cast<T*>(out), scale)
-  const bool f32 = out_dtype == 1;
   if (dim == 1) { if (f32) GSC_GRID(1, float); else GSC_GRID(1, double); }
   else if (dim == 2) { if (f32) GSC_GRID(2, float); else GSC_GRID(2, double); }
   else { if (f32) GSC_GRID(3, float); else GSC_GRID(3, double); }

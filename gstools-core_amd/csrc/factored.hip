@@ -22,12 +22,20 @@
 //
 // Kernel shape: 256 threads (4 waves, 2 x 2), block tile 128 x 128, wave tile
 // 64 x 64 = 2 x 2 v_mfma_f32_32x32x16 blocks, K chunks of 32 (16 modes)
-// staged in LDS.  The A tile is generated from P and Q in the chunk prologue
-// (the Khatri-Rao operand is never written to HBM).
+// staged in LDS.  The A tile is generated from P and Q while the previous
+// chunk's MFMAs run (the Khatri-Rao operand is never written to HBM).
+//
+// Table layouts are chunk-major so that one chunk of one block is contiguous:
+//   P/Q/R: tab[(c * n + a) * 16 + mm]            mode m = 16 c + mm, a < n
+//   B:     bt[((s * nch + c) * ncp + col) * 32 + kk], kk = 2 mm (+1 for Re)
+// with the mode count padded to mp = 32 * ceil(M / 32) (nch = mp / 16 chunks,
+// an even number; padded modes are zero).
 #include "gsc_internal.h"
 #include "gsc_factored.h"
 
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 namespace gsc {
 namespace {
@@ -43,13 +51,13 @@ constexpr int kPad = 8;    // LDS row padding (elements)
 constexpr int kLd = kKC + kPad;
 // Accumulation (ACC template argument).  The MFMA accumulates in FP32; over
 // K = 2M terms its rounding grows with the partial sum (~8e-6 sigma at
-// M = 4000).  Every kFlush chunks (64 modes) the FP32 accumulators are added
-// into a second set and cleared, so FP32 rounding only ever sees a 64-mode
-// partial sum:
+// M = 4000).  Every kFlush chunks (256 modes) the FP32 accumulators are added
+// into a second set and cleared, so FP32 rounding only ever sees a 256-mode
+// partial sum (256 modes):
 //   ACC_F64   second set in FP64  (most accurate, +128 VGPRs)
 //   ACC_F32X2 second set in FP32  (~1e-6 sigma at M = 4000, +64 VGPRs)
 //   ACC_F32   no second set       (fastest, error grows with M)
-constexpr int kFlush = 4;
+constexpr int kFlush = 16;  // even: the main loop is unrolled by two
 enum { ACC_F64 = 0, ACC_F32X2 = 1, ACC_F32 = 2 };
 template <int ACC> struct AccT { using type = double; };
 template <> struct AccT<ACC_F32X2> { using type = float; };
@@ -78,69 +86,91 @@ __device__ __forceinline__ void split(float x, T (&o)[NS]) {
 }
 
 // ---------------------------------------------------------------------------
-// Per-axis complex table: tab[a * mp + m] = amp_m e^{2 pi i (kap_{axis,m} x_a + th_m)}
-// (amp = A, th = theta/2pi when with_amp, else amp = 1, th = 0), FP64 -> FP32.
-// Modes m >= M are zero.  One thread per (a, m), 1-D grid over n * mp.
+// Per-axis complex tables, tab[m / 16][a][m % 16] = amp_m e^{2 pi i (kap_{axis,m} x_a + th_m)}
+// (amp = A, th = theta/2pi for axis 0; amp = 1, th = 0 otherwise), FP64 -> FP32.
+// Modes m >= M are zero.
 // ---------------------------------------------------------------------------
-__global__ void k_axis_table(const ModeRec* __restrict__ modes, int M, int mp, int axis,
-                             const double* __restrict__ x, int64_t n, int with_amp,
-                             float2* __restrict__ tab) {
-  const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (idx >= n * mp) return;
-  const int64_t a = idx / mp;
-  const int m = static_cast<int>(idx - a * mp);
-  float2 v = make_float2(0.0f, 0.0f);
-  if (m < M) {
+// All tables of one call in one launch.  blockIdx.z selects the table:
+//   0: P (axis 0, amplitude and theta folded in), chunk-major float2
+//   1: Q (axis 1, 3-D only), chunk-major float2
+//   2: the B operand, split straight from the FP64 value of R (last axis),
+//      columns ncols <= col < ncp zero-filled.
+// block = 16 rows x 16 modes of one chunk: grid (ceil(max(na, nb, ncp) / 16), mp / 16, 3).
+template <typename T, int NS>
+__global__ void __launch_bounds__(256)
+k_tables(const ModeRec* __restrict__ modes, int M, int dim, const double* __restrict__ x0,
+         const double* __restrict__ x1, const double* __restrict__ xl, int64_t na, int64_t nb,
+         int64_t ncols, int64_t ncp, float2* __restrict__ P, float2* __restrict__ Q,
+         T* __restrict__ bt) {
+  const int which = blockIdx.z;
+  if (which == 1 && dim != 3) return;
+  const int64_t a = static_cast<int64_t>(blockIdx.x) * 16 + (threadIdx.x >> 4);
+  const int c = blockIdx.y;
+  const int nch = gridDim.y;
+  const int mm = threadIdx.x & 15;
+  const int m = c * 16 + mm;
+  const int64_t n = which == 0 ? na : (which == 1 ? nb : ncp);
+  if (a >= n) return;
+  const double* x = which == 0 ? x0 : (which == 1 ? x1 : xl);
+  const int axis = which == 2 ? dim - 1 : which;
+  double re = 0.0, im = 0.0;
+  if (m < M && (which != 2 || a < ncols)) {
     const ModeRec md = modes[m];
     double t = md.kap[axis] * x[a];
     double amp = 1.0;
-    if (with_amp) {
+    if (which == 0) {
       t += md.th;
       amp = static_cast<double>(md.amp[0]);
     }
     const double r = t - rint(t);  // exact: |r| <= 1/2 revolution
-    double s, c;
-    sincospi(2.0 * r, &s, &c);
-    v = make_float2(static_cast<float>(amp * c), static_cast<float>(amp * s));
+    double sn, cs;
+    sincospi(2.0 * r, &sn, &cs);
+    re = amp * cs;
+    im = amp * sn;
   }
-  tab[a * mp + m] = v;
-}
-
-// B operand: bt[s][col][k] for col < ncp (padded), k < kp = 2 mp:
-// k = 2m -> Im T(col, m), k = 2m + 1 -> Re T(col, m); zero for col >= ncols.
-template <typename T, int NS>
-__global__ void k_split_b(const float2* __restrict__ tab, int64_t ncols, int64_t ncp, int mp,
-                          T* __restrict__ bt) {
-  const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (idx >= ncp * mp) return;
-  const int64_t col = idx / mp;
-  const int m = static_cast<int>(idx - col * mp);
-  float2 v = make_float2(0.0f, 0.0f);
-  if (col < ncols) v = tab[col * mp + m];
-  T im[NS], re[NS];
-  split<T, NS>(v.y, im);
-  split<T, NS>(v.x, re);
-  const int64_t kp = 2 * static_cast<int64_t>(mp);
+  if (which < 2) {
+    float2* tab = which == 0 ? P : Q;
+    tab[(static_cast<int64_t>(c) * n + a) * 16 + mm] =
+        make_float2(static_cast<float>(re), static_cast<float>(im));
+    return;
+  }
+  double ri = im, rr = re;
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
-    T* row = bt + (s * ncp + col) * kp;
-    row[2 * m] = im[s];
-    row[2 * m + 1] = re[s];
+    const T hi = static_cast<T>(static_cast<float>(ri));
+    const T hr = static_cast<T>(static_cast<float>(rr));
+    ri -= static_cast<double>(static_cast<float>(hi));
+    rr -= static_cast<double>(static_cast<float>(hr));
+    T* row = bt + ((static_cast<int64_t>(s) * nch + c) * ncp + a) * 32;
+    row[2 * mm] = hi;
+    row[2 * mm + 1] = hr;
   }
 }
 
 // ---------------------------------------------------------------------------
 // The GEMM.  rows = n_rows (KR: n_rows = na * nb, row = a * nb + b), cols.
 // out[row * ldo + col] = scale * sum_k A(row, k) B(k, col).
+//
+// Software pipeline, one barrier per K chunk, two register sets for the
+// global loads and two LDS buffers: in the step for chunk c the wave
+//   1. issues the global loads of chunk c + 2 (P, Q rows, B tile) into the
+//      register set freed by chunk c,
+//   2. reads chunk c's A/B fragments from LDS buffer c % 2 and issues its
+//      MFMAs,
+//   3. generates chunk c + 1's A tile from the register set loaded one step
+//      earlier (complex products, splits) and writes it with the B tile to
+//      the other LDS buffer, interleaved with the MFMAs of step 2.
+// The loop is unrolled by two so buffer and register-set indices are
+// compile-time constants.
 // ---------------------------------------------------------------------------
 template <typename T, int NS, bool KR, typename OutT, int ACC>
 __global__ void __launch_bounds__(256)
-k_factored_gemm(const float2* __restrict__ P, const float2* __restrict__ Q, int64_t nb,
-                int64_t n_rows, int mp, const T* __restrict__ bt, int64_t ncp, int64_t ncols,
-                OutT* __restrict__ out, int64_t ldo, double scale) {
+k_factored_gemm(const float2* __restrict__ P, const float2* __restrict__ Q, int64_t na,
+                int64_t nb, int64_t n_rows, int nch, const T* __restrict__ bt, int64_t ncp,
+                int64_t ncols, OutT* __restrict__ out, int64_t ldo, double scale) {
   using V8 = typename Vec8<T>::type;
-  __shared__ __attribute__((aligned(16))) T As[NS][kTM][kLd];
-  __shared__ __attribute__((aligned(16))) T Bs[NS][kTN][kLd];
+  __shared__ __attribute__((aligned(16))) T As[2][NS][kTM][kLd];
+  __shared__ __attribute__((aligned(16))) T Bs[2][NS][kTN][kLd];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -148,7 +178,6 @@ k_factored_gemm(const float2* __restrict__ P, const float2* __restrict__ Q, int6
   const int wr = wave >> 1, wc = wave & 1;
   const int64_t row0 = static_cast<int64_t>(blockIdx.x) * kTM;
   const int64_t col0 = static_cast<int64_t>(blockIdx.y) * kTN;
-  const int64_t kp = 2 * static_cast<int64_t>(mp);
 
   // generator assignment: thread -> (row gr, 8 modes at half gh of the chunk)
   const int gr = tid >> 1, gh = tid & 1;
@@ -156,10 +185,69 @@ k_factored_gemm(const float2* __restrict__ P, const float2* __restrict__ Q, int6
   const bool grow_ok = grow < n_rows;
   const int64_t ga = KR ? (grow_ok ? grow / nb : 0) : (grow_ok ? grow : 0);
   const int64_t gb = KR ? (grow_ok ? grow - ga * nb : 0) : 0;
-  const float2* prow = P + ga * mp;
-  const float2* qrow = KR ? Q + gb * mp : nullptr;
-  // B loader assignment: thread -> (col bc, k half bh)
+  // float4 = two modes; chunk c of row a starts at float2 index (c n + a) 16
+  const float4* prow = reinterpret_cast<const float4*>(P + ga * 16 + 8 * gh);
+  const float4* qrow = KR ? reinterpret_cast<const float4*>(Q + gb * 16 + 8 * gh) : nullptr;
+  const int64_t pstep = na * 8, qstep = nb * 8;  // float4 per chunk
+  // B loader assignment: thread -> (col bc, k half bh); chunk stride ncp * 32
   const int bc = tid >> 1, bh = tid & 1;
+  const T* brow = bt + (col0 + bc) * 32 + 16 * bh;
+  const int64_t bstep = ncp * 32, bsplit = static_cast<int64_t>(nch) * ncp * 32;
+
+  float4 pr[2][4], qr[2][4];
+  V8 br[2][NS][2];
+  auto gload = [&](const int set, int c) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      pr[set][i] = prow[c * pstep + i];
+      if constexpr (KR) qr[set][i] = qrow[c * qstep + i];
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const V8* src = reinterpret_cast<const V8*>(brow + s * bsplit + c * bstep);
+      br[set][s][0] = src[0];
+      br[set][s][1] = src[1];
+    }
+  };
+  auto gstore = [&](const int set, const int buf) {
+    T ws[NS][16];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      float wr0 = pr[set][i].x, wi0 = pr[set][i].y, wr1 = pr[set][i].z, wi1 = pr[set][i].w;
+      if constexpr (KR) {
+        const float4 q = qr[set][i];
+        const float r0 = wr0 * q.x - wi0 * q.y, i0 = wr0 * q.y + wi0 * q.x;
+        const float r1 = wr1 * q.z - wi1 * q.w, i1 = wr1 * q.w + wi1 * q.z;
+        wr0 = r0; wi0 = i0; wr1 = r1; wi1 = i1;
+      }
+      if (!grow_ok) wr0 = wi0 = wr1 = wi1 = 0.0f;
+      T t[NS];
+      split<T, NS>(wr0, t);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) ws[s][4 * i + 0] = t[s];
+      split<T, NS>(wi0, t);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) ws[s][4 * i + 1] = t[s];
+      split<T, NS>(wr1, t);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) ws[s][4 * i + 2] = t[s];
+      split<T, NS>(wi1, t);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) ws[s][4 * i + 3] = t[s];
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      V8* dst = reinterpret_cast<V8*>(&As[buf][s][gr][16 * gh]);
+      V8 lo, hi;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) { lo[i] = ws[s][i]; hi[i] = ws[s][8 + i]; }
+      dst[0] = lo;
+      dst[1] = hi;
+      V8* bd = reinterpret_cast<V8*>(&Bs[buf][s][bc][16 * bh]);
+      bd[0] = br[set][s][0];
+      bd[1] = br[set][s][1];
+    }
+  };
 
   using A2 = typename AccT<ACC>::type;
   f32x16_t acc[2][2];
@@ -173,7 +261,63 @@ k_factored_gemm(const float2* __restrict__ P, const float2* __restrict__ Q, int6
 #pragma unroll
       for (int v = 0; v < (ACC == ACC_F32 ? 1 : 16); ++v) acc2[i][j][v] = 0;
     }
-  auto flush = [&]() {
+
+  auto mfma_chunk = [&](const int buf) {
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      V8 af[NS][2], bf[NS][2];
+      const int kk = ks * 16 + 8 * (lane >> 5);
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          af[s][i] = *reinterpret_cast<const V8*>(&As[buf][s][wr * 64 + i * 32 + (lane & 31)][kk]);
+          bf[s][i] = *reinterpret_cast<const V8*>(&Bs[buf][s][wc * 64 + i * 32 + (lane & 31)][kk]);
+        }
+#pragma unroll
+      for (int sa = 0; sa < NS; ++sa)
+#pragma unroll
+        for (int sb = 0; sb < NS - sa; ++sb)
+#pragma unroll
+          for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = mfma(af[sa][i], bf[sb][j], acc[i][j]);
+    }
+  };
+  // Interleave hints for one step: the loads first, the first k-step's LDS
+  // reads, then every MFMA paired with LDS reads, generator VALU work and
+  // LDS writes of the next chunk.
+  constexpr int kMfma = 2 * 4 * (NS * (NS + 1) / 2);
+  auto sched = [&]() {
+    __builtin_amdgcn_sched_group_barrier(0x020, 4 * (KR ? 2 : 1) + 2 * NS, 0);  // VMEM read
+    __builtin_amdgcn_sched_group_barrier(0x100, 4 * NS, 0);                    // DS read
+#pragma unroll
+    for (int i = 0; i < kMfma; ++i) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+      if (i < 4 * NS) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x002, 6, 0);  // VALU
+      if (i >= kMfma - 4 * NS) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // DS write
+    }
+  };
+
+  gload(0, 0);
+  gload(1, nch > 1 ? 1 : 0);
+  gstore(0, 0);
+  __syncthreads();
+  for (int c0 = 0; c0 < nch; c0 += kFlush) {
+    const int c1 = min(c0 + kFlush, nch);
+    for (int c = c0; c < c1; c += 2) {
+      gload(0, min(c + 2, nch - 1));
+      mfma_chunk(0);
+      gstore(1, 1);
+      sched();
+      __syncthreads();
+      gload(1, min(c + 3, nch - 1));
+      mfma_chunk(1);
+      gstore(0, 0);
+      sched();
+      __syncthreads();
+    }
     if constexpr (ACC != ACC_F32) {
 #pragma unroll
       for (int i = 0; i < 2; ++i)
@@ -185,96 +329,7 @@ k_factored_gemm(const float2* __restrict__ P, const float2* __restrict__ Q, int6
             acc[i][j][v] = 0.0f;
           }
     }
-  };
-  int since_flush = 0;
-
-  for (int m0 = 0; m0 < mp; m0 += kKC / 2) {
-    // ---- A tile: W(row, m0 + 8 gh + 0..7) -> k = 16 gh + 0..15
-    {
-      T ws[NS][16];
-      if (grow_ok) {
-        const float4* p4 = reinterpret_cast<const float4*>(prow + m0 + 8 * gh);
-        const float4* q4 = KR ? reinterpret_cast<const float4*>(qrow + m0 + 8 * gh) : nullptr;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const float4 p = p4[i];  // two modes: (re, im, re, im)
-          float wr0 = p.x, wi0 = p.y, wr1 = p.z, wi1 = p.w;
-          if constexpr (KR) {
-            const float4 q = q4[i];
-            const float r0 = wr0 * q.x - wi0 * q.y, i0 = wr0 * q.y + wi0 * q.x;
-            const float r1 = wr1 * q.z - wi1 * q.w, i1 = wr1 * q.w + wi1 * q.z;
-            wr0 = r0; wi0 = i0; wr1 = r1; wi1 = i1;
-          }
-          T t[NS];
-          split<T, NS>(wr0, t);
-#pragma unroll
-          for (int s = 0; s < NS; ++s) ws[s][4 * i + 0] = t[s];
-          split<T, NS>(wi0, t);
-#pragma unroll
-          for (int s = 0; s < NS; ++s) ws[s][4 * i + 1] = t[s];
-          split<T, NS>(wr1, t);
-#pragma unroll
-          for (int s = 0; s < NS; ++s) ws[s][4 * i + 2] = t[s];
-          split<T, NS>(wi1, t);
-#pragma unroll
-          for (int s = 0; s < NS; ++s) ws[s][4 * i + 3] = t[s];
-        }
-      } else {
-#pragma unroll
-        for (int s = 0; s < NS; ++s)
-#pragma unroll
-          for (int i = 0; i < 16; ++i) ws[s][i] = static_cast<T>(0.0f);
-      }
-#pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        V8* dst = reinterpret_cast<V8*>(&As[s][gr][16 * gh]);
-        V8 lo, hi;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) { lo[i] = ws[s][i]; hi[i] = ws[s][8 + i]; }
-        dst[0] = lo;
-        dst[1] = hi;
-      }
-    }
-    // ---- B tile: bt[s][col0 + bc][2 m0 + 16 bh + 0..15]
-    {
-      const int64_t k0 = 2 * static_cast<int64_t>(m0) + 16 * bh;
-#pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        const V8* src = reinterpret_cast<const V8*>(bt + (s * ncp + col0 + bc) * kp + k0);
-        V8* dst = reinterpret_cast<V8*>(&Bs[s][bc][16 * bh]);
-        dst[0] = src[0];
-        dst[1] = src[1];
-      }
-    }
-    __syncthreads();
-    // ---- MFMAs: two k-steps of 16
-#pragma unroll
-    for (int ks = 0; ks < 2; ++ks) {
-      V8 af[NS][2], bf[NS][2];
-      const int kk = ks * 16 + 8 * (lane >> 5);
-#pragma unroll
-      for (int s = 0; s < NS; ++s)
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-          af[s][i] = *reinterpret_cast<const V8*>(&As[s][wr * 64 + i * 32 + (lane & 31)][kk]);
-          bf[s][i] = *reinterpret_cast<const V8*>(&Bs[s][wc * 64 + i * 32 + (lane & 31)][kk]);
-        }
-#pragma unroll
-      for (int sa = 0; sa < NS; ++sa)
-#pragma unroll
-        for (int sb = 0; sb < NS - sa; ++sb)
-#pragma unroll
-          for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j) acc[i][j] = mfma(af[sa][i], bf[sb][j], acc[i][j]);
-    }
-    __syncthreads();
-    if (++since_flush == kFlush) {
-      flush();
-      since_flush = 0;
-    }
   }
-  flush();
 
   // ---- epilogue: D[8 (v/4) + 4 (lane/32) + v%4][lane%32] of each 32 x 32 block
 #pragma unroll
@@ -295,38 +350,33 @@ k_factored_gemm(const float2* __restrict__ P, const float2* __restrict__ Q, int6
 }
 
 template <typename T, int NS, typename OutT, int ACC>
-void gemm_acc(bool kr, dim3 grid, const float2* P, const float2* Q, int64_t nb, int64_t rows,
-              int mp, const void* bt, int64_t ncp, int64_t ncols, void* out, int64_t ldo,
-              double scale, hipStream_t s) {
+void gemm_acc(bool kr, dim3 grid, const float2* P, const float2* Q, int64_t na, int64_t nb,
+              int64_t rows, int nch, const void* bt, int64_t ncp, int64_t ncols, void* out,
+              int64_t ldo, double scale, hipStream_t s) {
   if (kr)
     k_factored_gemm<T, NS, true, OutT, ACC><<<grid, 256, 0, s>>>(
-        P, Q, nb, rows, mp, static_cast<const T*>(bt), ncp, ncols, static_cast<OutT*>(out), ldo,
-        scale);
+        P, Q, na, nb, rows, nch, static_cast<const T*>(bt), ncp, ncols, static_cast<OutT*>(out),
+        ldo, scale);
   else
     k_factored_gemm<T, NS, false, OutT, ACC><<<grid, 256, 0, s>>>(
-        P, Q, nb, rows, mp, static_cast<const T*>(bt), ncp, ncols, static_cast<OutT*>(out), ldo,
-        scale);
+        P, Q, na, nb, rows, nch, static_cast<const T*>(bt), ncp, ncols, static_cast<OutT*>(out),
+        ldo, scale);
 }
 
 template <typename T, int NS, typename OutT>
-hipError_t gemm_t(bool kr, int accum, const float2* P, const float2* Q, int64_t nb, int64_t rows,
-                  int mp, const void* bt, int64_t ncp, int64_t ncols, void* out, int64_t ldo,
-                  double scale, hipStream_t s) {
+hipError_t gemm_t(bool kr, int accum, const float2* P, const float2* Q, int64_t na, int64_t nb,
+                  int64_t rows, int nch, const void* bt, int64_t ncp, int64_t ncols, void* out,
+                  int64_t ldo, double scale, hipStream_t s) {
   dim3 grid(static_cast<unsigned>((rows + kTM - 1) / kTM), static_cast<unsigned>(ncp / kTN));
   if (accum == ACC_F32X2)
-    gemm_acc<T, NS, OutT, ACC_F32X2>(kr, grid, P, Q, nb, rows, mp, bt, ncp, ncols, out, ldo, scale, s);
+    gemm_acc<T, NS, OutT, ACC_F32X2>(kr, grid, P, Q, na, nb, rows, nch, bt, ncp, ncols, out, ldo,
+                                     scale, s);
   else if (accum == ACC_F32)
-    gemm_acc<T, NS, OutT, ACC_F32>(kr, grid, P, Q, nb, rows, mp, bt, ncp, ncols, out, ldo, scale, s);
+    gemm_acc<T, NS, OutT, ACC_F32>(kr, grid, P, Q, na, nb, rows, nch, bt, ncp, ncols, out, ldo,
+                                   scale, s);
   else
-    gemm_acc<T, NS, OutT, ACC_F64>(kr, grid, P, Q, nb, rows, mp, bt, ncp, ncols, out, ldo, scale, s);
-  return hipGetLastError();
-}
-
-template <typename T, int NS>
-hipError_t split_t(const float2* tab, int64_t ncols, int64_t ncp, int mp, void* bt,
-                   hipStream_t s) {
-  const unsigned grid = static_cast<unsigned>((ncp * mp + 255) / 256);
-  k_split_b<T, NS><<<grid, 256, 0, s>>>(tab, ncols, ncp, mp, static_cast<T*>(bt));
+    gemm_acc<T, NS, OutT, ACC_F64>(kr, grid, P, Q, na, nb, rows, nch, bt, ncp, ncols, out, ldo,
+                                   scale, s);
   return hipGetLastError();
 }
 
@@ -337,7 +387,7 @@ FactoredPlan plan_factored(int dim, int64_t M, const int64_t* shape, int precisi
   p.dim = dim;
   p.ns = precision_splits(precision);
   p.bf16 = precision_is_bf16(precision);
-  p.mp = static_cast<int>(((M > 0 ? M : 1) + 15) / 16 * 16);
+  p.mp = static_cast<int>(((M > 0 ? M : 1) + 31) / 32 * 32);
   p.kp = 2 * static_cast<int64_t>(p.mp);
   if (dim == 2) {
     p.na = shape[0];
@@ -357,8 +407,6 @@ FactoredPlan plan_factored(int dim, int64_t M, const int64_t* shape, int precisi
   off += al(sizeof(float2) * static_cast<size_t>(p.na) * p.mp);
   p.off_q = off;
   if (dim == 3) off += al(sizeof(float2) * static_cast<size_t>(p.nb) * p.mp);
-  p.off_r = off;
-  off += al(sizeof(float2) * static_cast<size_t>(p.ncols) * p.mp);
   p.off_bt = off;
   off += al(2 * static_cast<size_t>(p.ns) * p.ncp * p.kp);
   p.bytes = off;
@@ -372,37 +420,36 @@ hipError_t launch_factored(const FactoredPlan& p, int64_t M, const double* const
   const ModeRec* modes = reinterpret_cast<const ModeRec*>(w);
   float2* P = reinterpret_cast<float2*>(w + p.off_p);
   float2* Q = reinterpret_cast<float2*>(w + p.off_q);
-  float2* R = reinterpret_cast<float2*>(w + p.off_r);
   void* bt = w + p.off_bt;
   const int Mi = static_cast<int>(M);
-  // axis tables (the mode table was written by launch_prep)
-  auto tgrid = [&](int64_t n) { return static_cast<unsigned>((n * p.mp + 255) / 256); };
-  k_axis_table<<<tgrid(p.na), 256, 0, s>>>(modes, Mi, p.mp, 0, axes[0], p.na, 1, P);
-  if (p.dim == 3)
-    k_axis_table<<<tgrid(p.nb), 256, 0, s>>>(modes, Mi, p.mp, 1, axes[1], p.nb, 0, Q);
-  const int last = p.dim - 1;
-  k_axis_table<<<tgrid(p.ncols), 256, 0, s>>>(modes, Mi, p.mp, last, axes[last], p.ncols, 0, R);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  // B operand from the last axis
+  // tables (the mode table was written by launch_prep): one launch
+  const int64_t nmax = std::max(std::max(p.na, p.dim == 3 ? p.nb : 0), p.ncp);
+  const dim3 tg(static_cast<unsigned>((nmax + 15) / 16), static_cast<unsigned>(p.mp / 16), 3);
+  const double* xl = axes[p.dim - 1];
+  const double* x1 = p.dim == 3 ? axes[1] : nullptr;
+#define GSC_TABLES(T, NS)                                                                     \
+  k_tables<T, NS><<<tg, 256, 0, s>>>(modes, Mi, p.dim, axes[0], x1, xl, p.na, p.nb, p.ncols, \
+                                     p.ncp, P, Q, static_cast<T*>(bt))
   if (p.bf16) {
-    e = p.ns == 1 ? split_t<__bf16, 1>(R, p.ncols, p.ncp, p.mp, bt, s)
-      : p.ns == 2 ? split_t<__bf16, 2>(R, p.ncols, p.ncp, p.mp, bt, s)
-                  : split_t<__bf16, 3>(R, p.ncols, p.ncp, p.mp, bt, s);
+    if (p.ns == 1) GSC_TABLES(__bf16, 1);
+    else if (p.ns == 2) GSC_TABLES(__bf16, 2);
+    else GSC_TABLES(__bf16, 3);
   } else {
-    e = p.ns == 1 ? split_t<_Float16, 1>(R, p.ncols, p.ncp, p.mp, bt, s)
-      : p.ns == 2 ? split_t<_Float16, 2>(R, p.ncols, p.ncp, p.mp, bt, s)
-                  : split_t<_Float16, 3>(R, p.ncols, p.ncp, p.mp, bt, s);
+    if (p.ns == 1) GSC_TABLES(_Float16, 1);
+    else if (p.ns == 2) GSC_TABLES(_Float16, 2);
+    else GSC_TABLES(_Float16, 3);
   }
+#undef GSC_TABLES
+  hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const bool kr = p.dim == 3;
   const double sc = scale;
   const bool f32 = out_dtype == 1;
 #define GSC_GEMM(T, NS)                                                                        \
-  (f32 ? gemm_t<T, NS, float>(kr, accum, P, Q, p.nb, p.rows, p.mp, bt, p.ncp, p.ncols, out,    \
-                              p.ncols, sc, s)                                                  \
-       : gemm_t<T, NS, double>(kr, accum, P, Q, p.nb, p.rows, p.mp, bt, p.ncp, p.ncols, out,   \
-                               p.ncols, sc, s))
+  (f32 ? gemm_t<T, NS, float>(kr, accum, P, Q, p.na, p.nb, p.rows, p.mp / 16, bt, p.ncp,       \
+                              p.ncols, out, p.ncols, sc, s)                                    \
+       : gemm_t<T, NS, double>(kr, accum, P, Q, p.na, p.nb, p.rows, p.mp / 16, bt, p.ncp,      \
+                               p.ncols, out, p.ncols, sc, s))
   if (p.bf16) {
     if (p.ns == 1) return GSC_GEMM(__bf16, 1);
     if (p.ns == 2) return GSC_GEMM(__bf16, 2);

@@ -22,19 +22,20 @@ inline bool precision_is_bf16(int prec) { return prec == 2 || prec == 3 || prec 
 //   [0, off_p)     ModeRec table (M records)
 //   [off_p, ...)   P: na x mp float2   (axis 0, amplitude and theta folded in)
 //   [off_q, ...)   Q: nb x mp float2   (axis 1, 3-D only)
-//   [off_r, ...)   R: ncols x mp float2 (last axis)
-//   [off_bt, ...)  B operand: ns x ncp x kp elements of fp16/bf16
+//   [off_bt, ...)  B operand (last axis, split): ns x ncp x kp elements of fp16/bf16
+// (all chunk-major: 16 modes of one row are contiguous, rows of one chunk
+// follow each other).
 struct FactoredPlan {
   int dim;
   int ns;
   bool bf16;
-  int mp;          // modes padded to a multiple of 16
+  int mp;          // modes padded to a multiple of 32 (an even number of 16-mode chunks)
   int64_t kp;      // 2 * mp
   int64_t na, nb;  // rows = na * nb (nb = 1 in 2-D)
   int64_t rows;
   int64_t ncols;   // last-axis length
   int64_t ncp;     // ncols padded to the column tile
-  size_t off_p, off_q, off_r, off_bt, bytes;
+  size_t off_p, off_q, off_bt, bytes;
 };
 
 FactoredPlan plan_factored(int dim, int64_t M, const int64_t* shape, int precision);
@@ -50,6 +51,6 @@ hipError_t launch_factored(const FactoredPlan& p, int64_t M, const double* const
 // pointers) by the C-order flat index; nothing is materialised.
 hipError_t launch_direct_grid(int dim, int64_t M, const int64_t* shape, const double* const* axes,
                               const void* table, void* out, int out_dtype, double scale,
-                              hipStream_t stream);
+                              int variant, hipStream_t stream);
 
 }  // namespace gsc

@@ -24,8 +24,9 @@ struct alignas(16) ModeRec {
 };
 static_assert(sizeof(ModeRec) == 48, "ModeRec layout");
 
-// For 4 <= dim <= kMaxDim the table is (dim + 2) doubles per mode:
-// [th, amp, kap_0 .. kap_{dim-1}].
+// For dim <= 3 the table holds the ModeRec records followed by the tiles of
+// the matrix-core kernels (see k_prep_small).  For 4 <= dim <= kMaxDim it is
+// (dim + 2) doubles per mode: [th, amp, kap_0 .. kap_{dim-1}].
 constexpr int kMaxDim = 16;
 
 enum Kind { KIND_SCALAR = 0, KIND_INCOMPR = 1, KIND_FOURIER = 2 };
@@ -38,13 +39,14 @@ hipError_t launch_prep(int kind, int dim, int64_t M, const double* cov, const do
                        const double* z2, const double* sf, void* table, hipStream_t stream);
 
 // out[j] (N values of out_dtype) = scale * sum_m amp0_m sin(2 pi (kap_m . x_j + th_m)).
+// variant 0: phases on the matrix cores (dim <= 3); 1: phases on the VALU.
 hipError_t launch_direct_scalar(int dim, int64_t M, int64_t N, const void* table,
                                 const double* pos, int64_t pos_ld, void* out, int out_dtype,
-                                double scale, hipStream_t stream);
+                                double scale, int variant, hipStream_t stream);
 
 // out[c * out_ld + j] = scale * sum_m amp_{c,m} sin(2 pi (kap_m . x_j + th_m)), c < dim.
 hipError_t launch_direct_incompr(int dim, int64_t M, int64_t N, const void* table,
                                  const double* pos, int64_t pos_ld, void* out, int64_t out_ld,
-                                 int out_dtype, double scale, hipStream_t stream);
+                                 int out_dtype, double scale, int variant, hipStream_t stream);
 
 }  // namespace gsc

@@ -14,7 +14,9 @@ no CPU fallback.  Deliberate differences from the reference:
     panic = "abort");
   * the GIL is released while the device works (ctypes drops it);
   * keyword-only extensions: out_dtype (np.float64 / np.float32), scale
-    (multiplies the result, e.g. sqrt(var / M)), device (HIP ordinal).
+    (multiplies the result, e.g. sqrt(var / M)), device (HIP ordinal),
+    variant ("valu": FP64 phases on the vector ALUs, the default;
+    "mfma": FP64 phases in matrix-core tiles, d <= 3).
 num_threads is accepted with the reference's meaning (worker threads); here
 it bounds the host threads that stage strided inputs.
 
@@ -60,6 +62,12 @@ def _threads_arg(num_threads) -> int:
     return min(int(num_threads), 1 << 30)
 
 
+def _variant(variant) -> int:
+    if variant not in _native.DIRECT_VARIANTS:
+        raise ValueError(f"variant must be one of {sorted(_native.DIRECT_VARIANTS)}")
+    return _native.DIRECT_VARIANTS[variant]
+
+
 def _out_dtype(out_dtype):
     dt = np.dtype(out_dtype)
     if dt == np.float64:
@@ -87,7 +95,7 @@ def _check_modes(cov, z1, z2, pos):
 
 
 def summate(cov_samples, z1, z2, pos, num_threads=None, *, out_dtype=np.float64, scale=1.0,
-            device=-1):
+            device=-1, variant="valu"):
     """Scalar randomization-method field (field.rs:37-65, lib.rs:33-48).
 
     u(x_j) = sum_m z1[m] cos(k_m . x_j) + z2[m] sin(k_m . x_j), k_m = cov_samples[:, m].
@@ -103,7 +111,8 @@ def summate(cov_samples, z1, z2, pos, num_threads=None, *, out_dtype=np.float64,
     out = np.empty(n, dtype=dt)
     if n == 0:
         return out
-    opts = _native.make_opts(out_dtype=code, scale=scale, device=device, num_threads=nt)
+    opts = _native.make_opts(out_dtype=code, scale=scale, device=device, num_threads=nt,
+                             direct_variant=_variant(variant))
     (csd, csm), (s1,), (s2,), (psd, psp) = _es(cov), _es(z1), _es(z2), _es(pos)
     rc = _native.lib().gsc_summate(d, cov.shape[1], n, _ptr(cov), csd, csm, _ptr(z1), s1,
                                    _ptr(z2), s2, _ptr(pos), psd, psp, _ptr(out), 1,
@@ -113,7 +122,7 @@ def summate(cov_samples, z1, z2, pos, num_threads=None, *, out_dtype=np.float64,
 
 
 def summate_incompr(cov_samples, z1, z2, pos, num_threads=None, *, out_dtype=np.float64,
-                    scale=1.0, device=-1):
+                    scale=1.0, device=-1, variant="valu"):
     """Incompressible vector field (field.rs:97-182, lib.rs:50-65); d must be 2 or 3.
 
     Returns a (d, N) array in Fortran order, like the reference (field.rs:166-174).
@@ -131,7 +140,8 @@ def summate_incompr(cov_samples, z1, z2, pos, num_threads=None, *, out_dtype=np.
     out = np.empty((d, n), dtype=dt, order="F")
     if n == 0:
         return out
-    opts = _native.make_opts(out_dtype=code, scale=scale, device=device, num_threads=nt)
+    opts = _native.make_opts(out_dtype=code, scale=scale, device=device, num_threads=nt,
+                             direct_variant=_variant(variant))
     (csd, csm), (s1,), (s2,), (psd, psp) = _es(cov), _es(z1), _es(z2), _es(pos)
     osd, osp = _es(out)
     rc = _native.lib().gsc_summate_incompr(d, cov.shape[1], n, _ptr(cov), csd, csm, _ptr(z1), s1,
@@ -142,7 +152,7 @@ def summate_incompr(cov_samples, z1, z2, pos, num_threads=None, *, out_dtype=np.
 
 
 def summate_fourier(spectrum_factor, modes, z1, z2, pos, num_threads=None, *,
-                    out_dtype=np.float64, scale=1.0, device=-1):
+                    out_dtype=np.float64, scale=1.0, device=-1, variant="valu"):
     """Fourier-method field (field.rs:219-249, lib.rs:67-84).
 
     u(x_j) = sum_m sf[m] (z1[m] cos(k_m . x_j) + z2[m] sin(k_m . x_j)), k_m = modes[:, m].
@@ -161,7 +171,8 @@ def summate_fourier(spectrum_factor, modes, z1, z2, pos, num_threads=None, *,
     out = np.empty(n, dtype=dt)
     if n == 0:
         return out
-    opts = _native.make_opts(out_dtype=code, scale=scale, device=device, num_threads=nt)
+    opts = _native.make_opts(out_dtype=code, scale=scale, device=device, num_threads=nt,
+                             direct_variant=_variant(variant))
     (ssf,), (msd, msm), (s1,), (s2,), (psd, psp) = _es(sf), _es(md), _es(z1), _es(z2), _es(pos)
     rc = _native.lib().gsc_summate_fourier(d, md.shape[1], n, _ptr(sf), ssf, _ptr(md), msd, msm,
                                            _ptr(z1), s1, _ptr(z2), s2, _ptr(pos), psd, psp,

@@ -40,6 +40,8 @@ class GscOpts(ctypes.Structure):
         ("device", ctypes.c_int32),
         ("num_threads", ctypes.c_int32),
         ("accum", ctypes.c_int32),
+        ("direct_variant", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
         ("scale", ctypes.c_double),
     ]
 
@@ -126,10 +128,11 @@ def lib():
 
 
 ACCUM = {"f64": 0, "f32x2": 1, "f32": 2}
+DIRECT_VARIANTS = {"mfma": 0, "valu": 1}
 
 
 def make_opts(out_dtype=GSC_F64, scale=1.0, device=-1, num_threads=0, path=GSC_PATH_AUTO,
-              precision=0, accum=0) -> GscOpts:
+              precision=0, accum=0, direct_variant=1) -> GscOpts:
     o = GscOpts()
     lib().gsc_opts_init(ctypes.byref(o))
     o.out_dtype = out_dtype
@@ -139,6 +142,7 @@ def make_opts(out_dtype=GSC_F64, scale=1.0, device=-1, num_threads=0, path=GSC_P
     o.path = int(path)
     o.precision = int(precision)
     o.accum = int(accum)
+    o.direct_variant = int(direct_variant)
     return o
 
 

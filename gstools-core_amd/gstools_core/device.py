@@ -18,22 +18,22 @@ def workspace_bytes(dim: int, m: int) -> int:
     return int(_native.lib().gsc_workspace_bytes(int(dim), int(m)))
 
 
-def _opts(out_f32, scale):
+def _opts(out_f32, scale, variant="valu"):
     return _native.make_opts(out_dtype=_native.GSC_F32 if out_f32 else _native.GSC_F64,
-                             scale=scale)
+                             scale=scale, direct_variant=_native.DIRECT_VARIANTS[variant])
 
 
 def summate_dev(dim, m, n, cov, z1, z2, pos, pos_ld, out, work, work_bytes, *, out_f32=False,
-                scale=1.0, stream=0):
-    o = _opts(out_f32, scale)
+                scale=1.0, stream=0, variant="valu"):
+    o = _opts(out_f32, scale, variant)
     rc = _native.lib().gsc_summate_dev(dim, m, n, cov, z1, z2, pos, pos_ld, out, work, work_bytes,
                                        ctypes.byref(o), ctypes.c_void_p(stream or None))
     _native.check(rc)
 
 
 def summate_incompr_dev(dim, m, n, cov, z1, z2, pos, pos_ld, out, out_ld, work, work_bytes, *,
-                        out_f32=False, scale=1.0, stream=0):
-    o = _opts(out_f32, scale)
+                        out_f32=False, scale=1.0, stream=0, variant="valu"):
+    o = _opts(out_f32, scale, variant)
     rc = _native.lib().gsc_summate_incompr_dev(dim, m, n, cov, z1, z2, pos, pos_ld, out, out_ld,
                                                work, work_bytes, ctypes.byref(o),
                                                ctypes.c_void_p(stream or None))
@@ -41,8 +41,8 @@ def summate_incompr_dev(dim, m, n, cov, z1, z2, pos, pos_ld, out, out_ld, work, 
 
 
 def summate_fourier_dev(dim, m, n, sf, modes, z1, z2, pos, pos_ld, out, work, work_bytes, *,
-                        out_f32=False, scale=1.0, stream=0):
-    o = _opts(out_f32, scale)
+                        out_f32=False, scale=1.0, stream=0, variant="valu"):
+    o = _opts(out_f32, scale, variant)
     rc = _native.lib().gsc_summate_fourier_dev(dim, m, n, sf, modes, z1, z2, pos, pos_ld, out,
                                                work, work_bytes, ctypes.byref(o),
                                                ctypes.c_void_p(stream or None))

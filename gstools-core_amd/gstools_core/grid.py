@@ -30,7 +30,7 @@ def _axes_args(axes):
     return axes
 
 
-def _opts(path, precision, out_code, scale, device, nt, accum="f64"):
+def _opts(path, precision, out_code, scale, device, nt, accum="f64", variant="valu"):
     if path not in PATHS:
         raise ValueError(f"path must be one of {sorted(PATHS)}")
     if precision not in _native.PRECISIONS:
@@ -39,7 +39,8 @@ def _opts(path, precision, out_code, scale, device, nt, accum="f64"):
         raise ValueError(f"accum must be one of {sorted(_native.ACCUM)}")
     return _native.make_opts(out_dtype=out_code, scale=scale, device=device, num_threads=nt,
                              path=PATHS[path], precision=_native.PRECISIONS[precision],
-                             accum=_native.ACCUM[accum])
+                             accum=_native.ACCUM[accum],
+                             direct_variant=_native.DIRECT_VARIANTS[variant])
 
 
 def grid_path(cov_samples, axes, path="auto", precision="auto") -> str:
@@ -55,7 +56,7 @@ def grid_path(cov_samples, axes, path="auto", precision="auto") -> str:
 
 
 def summate_grid(cov_samples, z1, z2, axes, num_threads=None, *, path="auto", precision="auto",
-                 out_dtype=np.float64, scale=1.0, device=-1, accum="f64"):
+                 out_dtype=np.float64, scale=1.0, device=-1, accum="f64", variant="valu"):
     """Field on the grid axes[0] x ... x axes[d-1] (d in 1..3), C order, shape (len(ax) ...).
 
     Equal to ``summate(cov_samples, z1, z2, pos)`` with pos the expanded grid
@@ -79,7 +80,7 @@ def summate_grid(cov_samples, z1, z2, axes, num_threads=None, *, path="auto", pr
     out = np.empty(shape, dtype=dt)
     if out.size == 0:
         return out
-    o = _opts(path, precision, code, scale, device, nt, accum)
+    o = _opts(path, precision, code, scale, device, nt, accum, variant)
     c_axes = (ctypes.c_void_p * d)(*[a.ctypes.data for a in axes])
     c_as = (ctypes.c_int64 * d)(*[a.strides[0] // 8 for a in axes])
     c_shape = (ctypes.c_int64 * d)(*shape)
@@ -99,10 +100,10 @@ def grid_workspace_bytes(dim, m, shape, path="auto", precision="auto") -> int:
 
 def summate_grid_dev(dim, m, cov, z1, z2, axes_ptrs, shape, out, work, work_bytes, *,
                      path="auto", precision="auto", out_f32=False, scale=1.0, stream=0,
-                     accum="f64"):
+                     accum="f64", variant="valu"):
     """Device form: every pointer is a device address (int); see gsc_summate_grid_dev."""
     o = _opts(path, precision, _native.GSC_F32 if out_f32 else _native.GSC_F64, scale, -1, 0,
-              accum)
+              accum, variant)
     c_axes = (ctypes.c_void_p * dim)(*axes_ptrs)
     c_shape = (ctypes.c_int64 * dim)(*shape)
     rc = _native.lib().gsc_summate_grid_dev(dim, m, cov, z1, z2, c_axes, c_shape, out, work,

@@ -74,14 +74,24 @@ typedef enum gsc_accum {
   GSC_ACCUM_F32 = 2     /* one FP32 set over all modes (error grows with M) */
 } gsc_accum;
 
+/* Where the direct (per-point) kernels evaluate the FP64 phases k . x.  On
+ * gfx950 the FP64 matrix tiles do not overlap with the VALU sine work (the
+ * measured time is the sum of both parts), so the VALU form is the default. */
+typedef enum gsc_direct_variant {
+  GSC_DIRECT_MFMA = 0,  /* v_mfma_f64_16x16x4 tiles of 16 modes x 16 points (dim <= 3) */
+  GSC_DIRECT_VALU = 1   /* FP64 FMA chains on the vector ALUs (default) */
+} gsc_direct_variant;
+
 typedef struct gsc_opts {
-  int32_t out_dtype;    /* gsc_dtype of `out` (default GSC_F64, as the reference) */
-  int32_t path;         /* gsc_path, structured entry points only */
-  int32_t precision;    /* gsc_precision, factored path only */
-  int32_t device;       /* HIP device ordinal, -1 = current device */
-  int32_t num_threads;  /* host threads for staging strided arrays, 0 = all */
-  int32_t accum;        /* gsc_accum, factored path only */
-  double scale;         /* multiplies every output value (1.0 = reference) */
+  int32_t out_dtype;       /* gsc_dtype of `out` (default GSC_F64, as the reference) */
+  int32_t path;            /* gsc_path, structured entry points only */
+  int32_t precision;       /* gsc_precision, factored path only */
+  int32_t device;          /* HIP device ordinal, -1 = current device */
+  int32_t num_threads;     /* host threads for staging strided arrays, 0 = all */
+  int32_t accum;           /* gsc_accum, factored path only */
+  int32_t direct_variant;  /* gsc_direct_variant, direct kernels only */
+  int32_t reserved;        /* zero */
+  double scale;            /* multiplies every output value (1.0 = reference) */
 } gsc_opts;
 
 /* Fills *opts with the defaults: f64 output, auto path/precision, device -1, scale 1. */

@@ -214,3 +214,51 @@ def test_device_entry_point_matches_host_entry_point():
     torch.cuda.synchronize()
     hosti = gstools_core.summate_incompr(md.cov_samples, md.z1, md.z2, w.pos)
     np.testing.assert_array_equal(outi.cpu().numpy(), hosti)
+
+
+def test_committed_oracle_vectors():
+    import os
+    from conftest import GOLDEN
+    with np.load(os.path.join(GOLDEN, "oracle_vectors.npz")) as f:
+        v = {k: f[k] for k in f.files}
+    for p in ("c1", "c2", "c5"):
+        g = gstools_core.summate(v[f"{p}_cov"], v[f"{p}_z1"], v[f"{p}_z2"], v[f"{p}_pos"])
+        m = v[f"{p}_cov"].shape[1]
+        err = _err(g, v[f"{p}_out"])
+        print(f"{p}: max|err|/sigma = {err / np.sqrt(m):.3e}")
+        assert err <= sigma_tol(m)
+    g = gstools_core.summate_incompr(v["c2_cov"], v["c2_z1"], v["c2_z2"], v["c2_pos"])
+    assert _err(g, v["c2i_out"]) <= sigma_tol(1000)
+    g = gstools_core.summate_fourier(v["f_sf"], v["f_modes"], v["f_z1"], v["f_z2"], v["f_pos"])
+    assert _err(g, v["f_out"]) <= sigma_tol(400) * v["f_sf"].max()
+
+
+@pytest.mark.parametrize("variant", ["mfma", "valu"])
+def test_direct_variants_against_oracle(variant):
+    # both phase engines (matrix-core FP64 tiles and VALU FP64 FMA chains)
+    for d, m, n, seed in ((1, 300, 3001, 1), (2, 1000, 20_000, 2), (3, 1000, 20_000, 3),
+                          (3, 17, 4099, 4), (2, 1, 77, 5)):
+        rng = np.random.default_rng(500 + seed)
+        cov = rng.normal(0, 0.05, size=(d, m))
+        z1, z2 = rng.normal(size=m), rng.normal(size=m)
+        pos = rng.uniform(0, 400, size=(d, n))
+        g = gstools_core.summate(cov, z1, z2, pos, variant=variant)
+        ref = oracle.summate(cov, z1, z2, pos)
+        err = _err(g, ref) / np.sqrt(m)
+        print(f"{variant} d={d} m={m} n={n}: {err:.3e}")
+        assert err <= 1e-5
+        if d in (2, 3):
+            gi = gstools_core.summate_incompr(cov, z1, z2, pos, variant=variant)
+            refi = oracle.summate_incompr(cov, z1, z2, pos)
+            assert _err(gi, refi) / np.sqrt(m) <= 1e-5
+    sf = np.abs(rng.normal(size=m))
+    g = gstools_core.summate_fourier(sf, cov, z1, z2, pos, variant=variant)
+    assert _err(g, oracle.summate_fourier(sf, cov, z1, z2, pos)) <= sigma_tol(m) * sf.max()
+
+
+def test_direct_variants_agree_at_scale():
+    w = workloads.c2(n=2_000_000)
+    md = w.modes
+    a = gstools_core.summate(md.cov_samples, md.z1, md.z2, w.pos, variant="mfma")
+    b = gstools_core.summate(md.cov_samples, md.z1, md.z2, w.pos, variant="valu")
+    assert _err(a, b) * md.scale <= 2e-6

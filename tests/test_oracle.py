@@ -246,3 +246,38 @@ def test_against_numpy_matrix_form():
     np.testing.assert_allclose(oracle.summate_incompr(k, z1, z2, pos), p @ z12, atol=1e-10)
     sf = rng.uniform(0.1, 2.0, size=300)
     np.testing.assert_allclose(oracle.summate_fourier(sf, k, z1, z2, pos), sf @ z12, atol=1e-10)
+
+
+# -- 6. committed oracle vectors (tests/golden/make_vectors.py) -----------------
+
+@pytest.fixture(scope="module")
+def vectors():
+    import os
+    from conftest import GOLDEN
+    with np.load(os.path.join(GOLDEN, "oracle_vectors.npz")) as f:
+        return {k: f[k] for k in f.files}
+
+
+def test_fixture_vectors_reproduce_bit_exact(vectors):
+    v = vectors
+    np.testing.assert_array_equal(oracle.summate(v["c1_cov"], v["c1_z1"], v["c1_z2"], v["c1_pos"]),
+                                  v["c1_out"])
+    np.testing.assert_array_equal(oracle.summate(v["c2_cov"], v["c2_z1"], v["c2_z2"], v["c2_pos"]),
+                                  v["c2_out"])
+    np.testing.assert_array_equal(
+        oracle.summate_incompr(v["c2_cov"], v["c2_z1"], v["c2_z2"], v["c2_pos"]), v["c2i_out"])
+    np.testing.assert_array_equal(oracle.summate(v["c5_cov"], v["c5_z1"], v["c5_z2"], v["c5_pos"]),
+                                  v["c5_out"])
+    np.testing.assert_array_equal(
+        oracle.summate_fourier(v["f_sf"], v["f_modes"], v["f_z1"], v["f_z2"], v["f_pos"]),
+        v["f_out"])
+
+
+def test_fixture_inputs_match_workload_generators(vectors):
+    from gstools_core import workloads
+    w = workloads.c1(n=1500)
+    np.testing.assert_array_equal(w.modes.cov_samples, vectors["c1_cov"])
+    np.testing.assert_array_equal(w.pos, vectors["c1_pos"])
+    w = workloads.c2(n=1000)
+    np.testing.assert_array_equal(w.modes.z2, vectors["c2_z2"])
+    np.testing.assert_array_equal(w.pos, vectors["c2_pos"])
