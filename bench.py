@@ -118,6 +118,68 @@ def grid_leg(steps: int, warmup: int) -> dict:
     return res
 
 
+def ensemble_leg(steps: int, warmup: int) -> dict:
+    """512 seeded 2-D realisations (workload c4) as one graph-captured plan."""
+    import numpy as np
+    import torch
+
+    from gstools_core import grid, workloads
+    from gstools_core.ensemble import Ensemble
+    from gstools_core.tuner import Tuner
+
+    ens_w = workloads.c4()
+    reals = [(w.modes.cov_samples, w.modes.z1, w.modes.z2, w.axes) for w in ens_w]
+    scales = [w.modes.scale for w in ens_w]
+    t0 = time.perf_counter()
+    tuner = Tuner(path=os.path.join("/tmp", "gsc_tuning.json"))
+    paths = [tuner.choose(2, tuple(len(a) for a in w.axes), w.modes.m, "fp16x3", "f64")
+             for w in ens_w]
+    t_tune = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ens = Ensemble(reals, paths=paths, scales=scales, out_dtype=np.float32, precision="fp16x3")
+    t_plan = time.perf_counter() - t0
+    stream = torch.cuda.current_stream()
+    per, total = time_steps(lambda: ens.run(stream.cuda_stream), steps, warmup, stream)
+    ms = total / steps
+    info = ens.info
+    res = {"workload": "512 2-D grids, sides uniform in {64..1024}, M=1000 Gaussian modes "
+                       "(l uniform in [5,50]) per realisation, FP32 output, fp16x3",
+           "point_modes": info["point_modes"], "factored": info["factored"],
+           "direct": info["direct"], "workspace_gb": info["workspace_bytes"] / 1e9,
+           "tune_s": t_tune, "plan_s": t_plan, "ms_per_step": ms,
+           "point_modes_per_s": info["point_modes"] / (ms * 1e-3)}
+    ens.close()
+    del ens
+    # the same realisations one grid call at a time (the caller-side loop)
+    dev = torch.device("cuda:0")
+    sub = ens_w[:64]
+    bufs = []
+    for w in sub:
+        md = w.modes
+        shape = [len(a) for a in w.axes]
+        nb = grid.grid_workspace_bytes(2, md.m, shape, path="factored", precision="fp16x3")
+        bufs.append((md.m, shape,
+                     torch.from_numpy(np.ascontiguousarray(md.cov_samples)).to(dev),
+                     torch.from_numpy(md.z1).to(dev), torch.from_numpy(md.z2).to(dev),
+                     [torch.from_numpy(a).to(dev) for a in w.axes],
+                     torch.empty(w.n_points, dtype=torch.float32, device=dev),
+                     torch.empty(nb, dtype=torch.uint8, device=dev), nb, md.scale))
+
+    def loop():
+        for m, shape, cov, z1, z2, axes, out, work, nb, sc in bufs:
+            grid.summate_grid_dev(2, m, cov.data_ptr(), z1.data_ptr(), z2.data_ptr(),
+                                  [a.data_ptr() for a in axes], shape, out.data_ptr(),
+                                  work.data_ptr(), nb, path="factored", precision="fp16x3",
+                                  out_f32=True, scale=sc, stream=stream.cuda_stream)
+
+    _, tl = time_steps(loop, max(2, steps // 4), 1, stream)
+    loop_ms = tl / max(2, steps // 4)
+    pm_sub = float(sum(w.n_points * w.modes.m for w in sub))
+    res["per_realisation_loop_64"] = {"ms_per_step": loop_ms,
+                                      "point_modes_per_s": pm_sub / (loop_ms * 1e-3)}
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,7 +190,16 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=10_000_000)
     ap.add_argument("--m", type=int, default=1000)
     ap.add_argument("--no-grid", action="store_true", help="skip the structured-grid leg")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble leg")
+    ap.add_argument("--legs", default="direct,grid,ensemble",
+                    help="comma list of legs: direct, grid, ensemble, copy (a 1 GiB device "
+                         "copy, the reference for HBM byte counters)")
     args = ap.parse_args()
+    legs = set(args.legs.split(","))
+    if args.no_grid:
+        legs.discard("grid")
+    if args.no_ensemble:
+        legs.discard("ensemble")
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the summation shards by points with no exchange; see DESIGN.md)", file=sys.stderr)
@@ -138,6 +209,25 @@ def main() -> int:
     import torch
 
     from gstools_core import device, workloads
+
+    if "copy" in legs:
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda:0")  # 1 GiB
+        a.fill_(1.0)
+        b = torch.empty_like(a)
+        per, tot = time_steps(lambda: b.copy_(a), args.steps, args.warmup,
+                              torch.cuda.current_stream())
+        print(json.dumps({"copy_bytes": a.numel() * 8, "ms_per_copy": tot / args.steps}),
+              flush=True)
+        if legs == {"copy"}:
+            return 0
+    if "direct" not in legs:
+        res = {}
+        if "grid" in legs:
+            res["structured_grid"] = grid_leg(args.steps, args.warmup)
+        if "ensemble" in legs:
+            res["ensemble"] = ensemble_leg(max(3, args.steps // 2), 1)
+        print(json.dumps(res), flush=True)
+        return 0
 
     log(f"generating workload c2: n={args.n} m={args.m}")
     w = workloads.c2(n=args.n, m=args.m)
@@ -195,9 +285,14 @@ def main() -> int:
     }
 
     grid_res = None
-    if not args.no_grid:
+    if "grid" in legs:
         log("grid leg (256^3, factored vs direct)")
         grid_res = grid_leg(args.steps, args.warmup)
+
+    ens_res = None
+    if "ensemble" in legs:
+        log("ensemble leg (512 realisations)")
+        ens_res = ensemble_leg(max(3, args.steps // 2), 1)
 
     # the matrix-core-phase variant of the same kernel, for comparison
     def step_mfma():
@@ -235,6 +330,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "direct_variants": direct_variants,
         "structured_grid": grid_res,
+        "ensemble": ens_res,
     }
     print(json.dumps(line), flush=True)
     return 0

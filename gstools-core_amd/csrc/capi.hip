@@ -108,6 +108,8 @@ int validate_opts(const gsc_opts* o) {
     return fail(GSC_ERR_INVALID, "out_dtype must be GSC_F64 or GSC_F32");
   if (o->direct_variant != GSC_DIRECT_MFMA && o->direct_variant != GSC_DIRECT_VALU)
     return fail(GSC_ERR_INVALID, "unknown direct_variant %d", o->direct_variant);
+  if (o->blocks_per_cu < 0 || o->blocks_per_cu > 64)
+    return fail(GSC_ERR_INVALID, "blocks_per_cu must be in 0..64");
   return GSC_OK;
 }
 
@@ -543,6 +545,336 @@ int gsc_summate_grid_dev(int dim, int64_t M, const double* cov, const double* z1
     if (!axes[i]) return fail(GSC_ERR_INVALID, "null axis %d", i);
   return grid_launch(dim, M, cov, z1, z2, axes, shape, out, work, o, path,
                      static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"
+
+// ---- ensembles ----------------------------------------------------------------
+
+struct gsc_ensemble {
+  int device = 0;
+  void* mem = nullptr;
+  size_t mem_bytes = 0;
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int64_t n_fact = 0, n_dir = 0;
+  double point_modes = 0.0;
+};
+
+namespace {
+
+void ensemble_release(gsc_ensemble* p) {
+  if (!p) return;
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->cap) (void)hipStreamDestroy(p->cap);
+  if (p->mem) (void)hipFree(p->mem);
+  delete p;
+}
+
+int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes,
+                   const int64_t* m_counts, const int32_t* paths, const double* scales,
+                   const double* cov, const double* z1, const double* z2, const double* axes,
+                   void* out, const gsc_opts* o) {
+  const size_t elt = o->out_dtype == GSC_F32 ? 4 : 8;
+  std::vector<gsc::EnsReal> rs((size_t)count);
+  std::vector<int> fact, dir, group_len;
+  std::vector<size_t> tab_off((size_t)count), p_off((size_t)count), q_off((size_t)count),
+      b_off((size_t)count);
+  // pass 1: geometry, offsets into the caller's arrays, path, workspace sizes
+  int64_t m_off = 0, a_off = 0, o_off = 0;
+  size_t ws = 0;
+  int max_tiles = 0;
+  for (int r = 0; r < count; ++r) {
+    gsc::EnsReal& e = rs[(size_t)r];
+    std::memset(&e, 0, sizeof(e));
+    const int64_t* sh = shapes + (int64_t)r * dim;
+    const int64_t M = m_counts[r];
+    e.cov = cov + dim * m_off;
+    e.z1 = z1 + m_off;
+    e.z2 = z2 + m_off;
+    e.ax0 = axes + a_off;
+    e.ax1 = axes + a_off + sh[0];
+    e.ax2 = dim == 3 ? axes + a_off + sh[0] + sh[1] : nullptr;
+    e.out = static_cast<char*>(out) + (size_t)o_off * elt;
+    e.M = M;
+    e.ntiles = (int)((M + 15) / 16);
+    e.dim = dim;
+    e.s0 = sh[0];
+    e.s1 = sh[1];
+    e.s2 = dim == 3 ? sh[2] : 1;
+    e.npts = grid_points(dim, sh);
+    e.scale = scales ? scales[r] : o->scale;
+    gsc_opts po = *o;
+    po.path = paths ? paths[r] : GSC_PATH_AUTO;
+    e.path = choose_grid_path(dim, sh, &po);
+    m_off += M;
+    for (int i = 0; i < dim; ++i) a_off += sh[i];
+    o_off += e.npts;
+    max_tiles = std::max(max_tiles, e.ntiles);
+    tab_off[(size_t)r] = ws;
+    ws += align256(gsc::mode_table_bytes(dim, M));
+    if (e.npts == 0) continue;
+    if (e.path == GSC_PATH_FACTORED) {
+      const gsc::FactoredPlan fp = gsc::plan_factored(dim, M, sh, o->precision);
+      e.mp = fp.mp;
+      e.na = fp.na;
+      e.nb = fp.nb;
+      e.rows = fp.rows;
+      e.ncols = fp.ncols;
+      e.ncp = fp.ncp;
+      e.col_tiles = fp.ncp / 128;
+      p_off[(size_t)r] = ws;
+      ws += align256(sizeof(float2) * (size_t)fp.na * fp.mp);
+      q_off[(size_t)r] = ws;
+      if (dim == 3) ws += align256(sizeof(float2) * (size_t)fp.nb * fp.mp);
+      b_off[(size_t)r] = ws;
+      ws += align256(2 * (size_t)fp.ns * fp.ncp * fp.kp);
+      fact.push_back(r);
+    } else {
+      dir.push_back(r);
+    }
+    plan->point_modes += (double)e.npts * (double)M;
+  }
+  // longest first: most modes, then most tiles; then a longest-processing-time
+  // split into kXcdGroups groups of whole realisations (one per XCD), each
+  // group keeping that order
+  std::stable_sort(fact.begin(), fact.end(), [&](int a, int b) {
+    const gsc::EnsReal &ea = rs[(size_t)a], &eb = rs[(size_t)b];
+    if (ea.mp != eb.mp) return ea.mp > eb.mp;
+    return ea.rows * ea.ncp > eb.rows * eb.ncp;
+  });
+  {
+    std::vector<std::vector<int>> groups(gsc::kXcdGroups);
+    std::vector<double> load(gsc::kXcdGroups, 0.0);
+    for (int r : fact) {
+      const gsc::EnsReal& e = rs[(size_t)r];
+      const double w = (double)((e.rows + 127) / 128 * e.col_tiles) * e.mp;
+      const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      groups[(size_t)g].push_back(r);
+      load[(size_t)g] += w;
+    }
+    fact.clear();
+    for (auto& g : groups) fact.insert(fact.end(), g.begin(), g.end());
+    group_len.assign(gsc::kXcdGroups, 0);
+    for (int g = 0; g < gsc::kXcdGroups; ++g) group_len[(size_t)g] = (int)groups[(size_t)g].size();
+  }
+  std::stable_sort(dir.begin(), dir.end(), [&](int a, int b) {
+    return rs[(size_t)a].M * rs[(size_t)a].npts > rs[(size_t)b].M * rs[(size_t)b].npts;
+  });
+  std::vector<int64_t> tile_start(fact.size() + 1, 0), blk_start(dir.size() + 1, 0);
+  std::vector<int64_t> task_start(3 * fact.size() + 1, 0);
+  for (size_t i = 0; i < fact.size(); ++i) {
+    const gsc::EnsReal& e = rs[(size_t)fact[i]];
+    tile_start[i + 1] = tile_start[i] + (e.rows + 127) / 128 * e.col_tiles;
+    // table tasks: groups of 16 rows of P (na), Q (nb, 3-D only) and the B operand (ncp)
+    task_start[3 * i + 1] = task_start[3 * i] + (e.na + 15) / 16;
+    task_start[3 * i + 2] = task_start[3 * i + 1] + (dim == 3 ? (e.nb + 15) / 16 : 0);
+    task_start[3 * i + 3] = task_start[3 * i + 2] + e.ncp / 16;
+  }
+  for (size_t i = 0; i < dir.size(); ++i) {
+    const gsc::EnsReal& e = rs[(size_t)dir[i]];
+    blk_start[i + 1] = blk_start[i] + (e.npts + gsc::kEnsPoints - 1) / gsc::kEnsPoints;
+  }
+  // tile ranges of the groups
+  std::vector<int64_t> group_start(gsc::kXcdGroups + 1, 0);
+  {
+    size_t i = 0;
+    for (int g = 0; g < gsc::kXcdGroups; ++g) {
+      i += (size_t)group_len[(size_t)g];
+      group_start[(size_t)g + 1] = tile_start[i];
+    }
+  }
+  plan->n_fact = (int64_t)fact.size();
+  plan->n_dir = (int64_t)dir.size();
+  // control block: descriptors, lists, prefix sums, counters
+  const size_t c_desc = align256(sizeof(gsc::EnsReal) * (size_t)count);
+  const size_t c_fact = align256(sizeof(int) * (fact.size() + 1));
+  const size_t c_ts = align256(sizeof(int64_t) * tile_start.size());
+  const size_t c_dir = align256(sizeof(int) * (dir.size() + 1));
+  const size_t c_bs = align256(sizeof(int64_t) * blk_start.size());
+  const size_t c_tk = align256(sizeof(int64_t) * task_start.size());
+  const size_t c_gs = align256(sizeof(int64_t) * group_start.size());
+  const size_t c_cnt = 256;
+  const size_t ctrl = c_desc + c_fact + c_ts + c_dir + c_bs + c_tk + c_gs + c_cnt;
+  plan->mem_bytes = ctrl + ws;
+  if (hipMalloc(&plan->mem, plan->mem_bytes) != hipSuccess) {
+    plan->mem = nullptr;
+    return fail(GSC_ERR_ALLOC, "hipMalloc of %zu bytes for the ensemble failed",
+                plan->mem_bytes);
+  }
+  char* base = static_cast<char*>(plan->mem);
+  char* work = base + ctrl;
+  for (int r = 0; r < count; ++r) {
+    gsc::EnsReal& e = rs[(size_t)r];
+    const size_t rb = sizeof(gsc::ModeRec) * (size_t)(e.M > 0 ? e.M : 1);
+    e.table = reinterpret_cast<gsc::ModeRec*>(work + tab_off[(size_t)r]);
+    e.a_tab = reinterpret_cast<double*>(work + tab_off[(size_t)r] + align256(rb));
+    e.amp_tab = reinterpret_cast<float*>(work + tab_off[(size_t)r] + align256(rb) +
+                                         align256(sizeof(double) * 64 * (size_t)e.ntiles));
+    if (e.path == GSC_PATH_FACTORED && e.npts > 0) {
+      e.P = reinterpret_cast<float2*>(work + p_off[(size_t)r]);
+      e.Q = reinterpret_cast<float2*>(work + q_off[(size_t)r]);
+      e.bt = work + b_off[(size_t)r];
+    }
+  }
+  std::vector<char> hctrl(ctrl, 0);
+  std::memcpy(hctrl.data(), rs.data(), sizeof(gsc::EnsReal) * rs.size());
+  std::memcpy(hctrl.data() + c_desc, fact.data(), sizeof(int) * fact.size());
+  std::memcpy(hctrl.data() + c_desc + c_fact, tile_start.data(), sizeof(int64_t) * tile_start.size());
+  std::memcpy(hctrl.data() + c_desc + c_fact + c_ts, dir.data(), sizeof(int) * dir.size());
+  std::memcpy(hctrl.data() + c_desc + c_fact + c_ts + c_dir, blk_start.data(),
+              sizeof(int64_t) * blk_start.size());
+  std::memcpy(hctrl.data() + c_desc + c_fact + c_ts + c_dir + c_bs, task_start.data(),
+              sizeof(int64_t) * task_start.size());
+  std::memcpy(hctrl.data() + c_desc + c_fact + c_ts + c_dir + c_bs + c_tk, group_start.data(),
+              sizeof(int64_t) * group_start.size());
+  GSC_HIP(hipMemcpy(base, hctrl.data(), ctrl, hipMemcpyHostToDevice));
+
+  int cus = 256;
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, plan->device);
+  gsc::EnsLaunch el{};
+  el.d = reinterpret_cast<const gsc::EnsReal*>(base);
+  el.count = count;
+  el.fact = reinterpret_cast<const int*>(base + c_desc);
+  el.nfact = (int)fact.size();
+  el.tile_start = reinterpret_cast<const int64_t*>(base + c_desc + c_fact);
+  el.dir = reinterpret_cast<const int*>(base + c_desc + c_fact + c_ts);
+  el.ndir = (int)dir.size();
+  el.blk_start = reinterpret_cast<const int64_t*>(base + c_desc + c_fact + c_ts + c_dir);
+  el.task_start = reinterpret_cast<const int64_t*>(base + c_desc + c_fact + c_ts + c_dir + c_bs);
+  el.n_table_tasks = task_start.back();
+  el.group_start =
+      reinterpret_cast<const int64_t*>(base + c_desc + c_fact + c_ts + c_dir + c_bs + c_tk);
+  el.counters = reinterpret_cast<int*>(base + c_desc + c_fact + c_ts + c_dir + c_bs + c_tk + c_gs);
+  el.dim = dim;
+  el.max_tiles = max_tiles;
+  const int64_t bpc = o->blocks_per_cu > 0 ? o->blocks_per_cu : 2;
+  el.gemm_blocks = (int)std::max<int64_t>(1, std::min<int64_t>(tile_start.back(), bpc * cus));
+  el.direct_blocks = (int)std::max<int64_t>(1, std::min<int64_t>(blk_start.back(), 8 * (int64_t)cus));
+
+  GSC_HIP(hipStreamCreateWithFlags(&plan->cap, hipStreamNonBlocking));
+  GSC_HIP(hipStreamBeginCapture(plan->cap, hipStreamCaptureModeThreadLocal));
+  const hipError_t le = gsc::launch_ensemble(el, o->precision, o->out_dtype, o->accum, plan->cap);
+  hipGraph_t g = nullptr;
+  const hipError_t ce = hipStreamEndCapture(plan->cap, &g);
+  plan->graph = g;
+  if (le != hipSuccess) return fail(GSC_ERR_HIP, "ensemble launch capture: %s", hipGetErrorString(le));
+  if (ce != hipSuccess) return fail(GSC_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ce));
+  GSC_HIP(hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0));
+  return GSC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gsc_ensemble_create(gsc_ensemble** plan, int count, int dim, const int64_t* shapes,
+                        const int64_t* m_counts, const int32_t* paths, const double* scales,
+                        const double* cov, const double* z1, const double* z2,
+                        const double* axes, void* out, const gsc_opts* opts) {
+  if (!plan) return fail(GSC_ERR_INVALID, "null plan pointer");
+  *plan = nullptr;
+  gsc_opts tmp;
+  const gsc_opts* o = opts_or_default(opts, &tmp);
+  int rc;
+  if ((rc = validate_opts(o)) || (rc = validate_precision(o))) return rc;
+  if (dim != 2 && dim != 3) return fail(GSC_ERR_DIM, "ensembles support dim 2 or 3 (got %d)", dim);
+  if (count < 1 || count > 20000)
+    return fail(GSC_ERR_INVALID, "count must be in 1..20000 (got %d)", count);
+  if (!shapes || !m_counts) return fail(GSC_ERR_INVALID, "null shapes or m_counts");
+  int64_t m_total = 0, pts = 0;
+  for (int r = 0; r < count; ++r) {
+    if (m_counts[r] < 0 || m_counts[r] > (int64_t)0x7fffffff)
+      return fail(GSC_ERR_INVALID, "realisation %d: bad mode count", r);
+    for (int i = 0; i < dim; ++i)
+      if (shapes[(int64_t)r * dim + i] < 0)
+        return fail(GSC_ERR_INVALID, "realisation %d: negative axis length", r);
+    if (paths && (paths[r] < GSC_PATH_AUTO || paths[r] > GSC_PATH_FACTORED))
+      return fail(GSC_ERR_INVALID, "realisation %d: unknown path %d", r, paths[r]);
+    m_total += m_counts[r];
+    pts += grid_points(dim, shapes + (int64_t)r * dim);
+  }
+  if (m_total > 0 && (!cov || !z1 || !z2)) return fail(GSC_ERR_INVALID, "null mode arrays");
+  if (pts > 0 && (!axes || !out)) return fail(GSC_ERR_INVALID, "null axes or output");
+  gsc_ensemble* p = nullptr;
+  try {
+    p = new gsc_ensemble();
+    if ((rc = resolve_device(o, &p->device))) {
+      delete p;
+      return rc;
+    }
+    if (hipSetDevice(p->device) != hipSuccess) {
+      delete p;
+      return fail(GSC_ERR_HIP, "hipSetDevice failed");
+    }
+    if ((rc = check_arch(p->device))) {
+      delete p;
+      return rc;
+    }
+    rc = ensemble_build(p, count, dim, shapes, m_counts, paths, scales, cov, z1, z2, axes, out, o);
+  } catch (const std::bad_alloc&) {
+    rc = fail(GSC_ERR_ALLOC, "host allocation failed");
+  }
+  if (rc != GSC_OK) {
+    ensemble_release(p);
+    return rc;
+  }
+  *plan = p;
+  return GSC_OK;
+}
+
+int gsc_ensemble_run(gsc_ensemble* plan, void* stream) {
+  if (!plan || !plan->exec) return fail(GSC_ERR_INVALID, "invalid ensemble plan");
+  GSC_HIP(hipGraphLaunch(plan->exec, static_cast<hipStream_t>(stream)));
+  return GSC_OK;
+}
+
+int gsc_ensemble_info(const gsc_ensemble* plan, int64_t* n_factored, int64_t* n_direct,
+                      size_t* workspace_bytes, double* point_modes) {
+  if (!plan) return fail(GSC_ERR_INVALID, "invalid ensemble plan");
+  if (n_factored) *n_factored = plan->n_fact;
+  if (n_direct) *n_direct = plan->n_dir;
+  if (workspace_bytes) *workspace_bytes = plan->mem_bytes;
+  if (point_modes) *point_modes = plan->point_modes;
+  return GSC_OK;
+}
+
+void gsc_ensemble_destroy(gsc_ensemble* plan) { ensemble_release(plan); }
+
+void* gsc_device_alloc(size_t bytes, int device) {
+  if (device >= 0 && hipSetDevice(device) != hipSuccess) {
+    fail(GSC_ERR_HIP, "hipSetDevice(%d) failed", device);
+    return nullptr;
+  }
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes > 0 ? bytes : 1) != hipSuccess) {
+    fail(GSC_ERR_ALLOC, "hipMalloc of %zu bytes failed", bytes);
+    return nullptr;
+  }
+  return p;
+}
+
+void gsc_device_free(void* ptr) {
+  if (ptr) (void)hipFree(ptr);
+}
+
+int gsc_memcpy_to_device(void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return GSC_OK;
+  GSC_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return GSC_OK;
+}
+
+int gsc_memcpy_to_host(void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return GSC_OK;
+  GSC_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return GSC_OK;
+}
+
+int gsc_synchronize(void) {
+  GSC_HIP(hipDeviceSynchronize());
+  return GSC_OK;
 }
 
 }  // extern "C"

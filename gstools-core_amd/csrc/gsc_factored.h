@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "gsc_internal.h"
+
 #include <cstddef>
 #include <cstdint>
 
@@ -40,6 +42,88 @@ struct FactoredPlan {
 
 FactoredPlan plan_factored(int dim, int64_t M, const int64_t* shape, int precision);
 
+// One realisation of an ensemble (device-resident descriptor).
+struct EnsReal {
+  const double* cov;  // (dim, M) row-major
+  const double* z1;
+  const double* z2;
+  const double* ax0;
+  const double* ax1;
+  const double* ax2;
+  void* out;          // dense C-order grid
+  ModeRec* table;     // mode records, then the matrix-core tile tables
+  double* a_tab;
+  float* amp_tab;
+  float2* P;          // factored tables (chunk-major)
+  float2* Q;
+  void* bt;
+  int64_t M;
+  int ntiles;         // 16-mode tiles of the prep kernel
+  int dim;
+  int path;           // GSC_PATH_DIRECT or GSC_PATH_FACTORED
+  int mp;             // factored: modes padded to 32
+  int64_t na, nb, rows, ncols, ncp, col_tiles;  // factored geometry
+  int64_t s0, s1, s2, npts;                     // grid shape
+  double scale;
+};
+
+// Launch sequence of an ensemble (all pointers device-resident).  fact / dir
+// list the realisations on each path, tile_start / blk_start their prefix sums
+// of work items (n + 1 entries), counters kEnsCounters ints.  max_tiles bounds the
+// batched prep grid; the *_blocks are the persistent grid sizes.
+struct EnsLaunch {
+  const EnsReal* d;
+  int count;
+  const int* fact;
+  int nfact;
+  const int64_t* tile_start;
+  const int64_t* task_start;   // table tasks: 3 nfact + 1 prefix sums
+  const int64_t* group_start;  // tile ranges of the kXcdGroups groups (kXcdGroups + 1)
+  int64_t n_table_tasks;
+  const int* dir;
+  int ndir;
+  const int64_t* blk_start;
+  int* counters;
+  int dim;
+  int max_tiles;
+  int gemm_blocks;
+  int direct_blocks;
+};
+constexpr int kEnsPoints = 2048;  // points per direct work item
+constexpr int kXcdGroups = 8;     // ensemble GEMM work groups (one per XCD of an MI355X)
+// counters: kXcdGroups GEMM group counters, then the direct-kernel counter
+constexpr int kEnsCounters = kXcdGroups + 1;
+hipError_t launch_ensemble(const EnsLaunch& e, int precision, int out_dtype, int accum,
+                           hipStream_t stream);
+
+
+// Arguments of the per-operand-format launchers (gsc_gemm.h, gemm_*.hip).
+struct TableArgs {
+  const ModeRec* modes;
+  int M, dim;
+  const double* x0;
+  const double* x1;
+  const double* xl;
+  int64_t na, nb, ncols, ncp;
+  float2* P;
+  float2* Q;
+  void* bt;
+};
+struct GemmArgs {
+  const float2* P;
+  const float2* Q;
+  int64_t na, nb, rows;
+  int nch;
+  const void* bt;
+  int64_t ncp, ncols;
+  void* out;
+  bool out_f64;
+  int64_t ldo;
+  double scale;
+  bool kr;
+  int accum;
+};
+
 // Expects the mode table at the start of `work` (launch_prep, KIND_SCALAR).
 // axes: device pointers, one per dimension.  out: dense C-order grid.
 // accum: 0 = FP64 flush (default), 1 = two-level FP32, 2 = plain FP32.
@@ -52,5 +136,13 @@ hipError_t launch_factored(const FactoredPlan& p, int64_t M, const double* const
 hipError_t launch_direct_grid(int dim, int64_t M, const int64_t* shape, const double* const* axes,
                               const void* table, void* out, int out_dtype, double scale,
                               int variant, hipStream_t stream);
+
+// Per operand format (T = _Float16 or __bf16, NS splits), instantiated in gemm_*.hip.
+template <typename T, int NS>
+hipError_t tables_launch(const TableArgs& a, dim3 grid, hipStream_t s);
+template <typename T, int NS>
+hipError_t factored_gemm_launch(const GemmArgs& a, dim3 grid, hipStream_t s);
+template <typename T, int NS>
+hipError_t ens_launch(const EnsLaunch& e, int accum, bool out_f64, hipStream_t s);
 
 }  // namespace gsc

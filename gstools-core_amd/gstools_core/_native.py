@@ -41,7 +41,7 @@ class GscOpts(ctypes.Structure):
         ("num_threads", ctypes.c_int32),
         ("accum", ctypes.c_int32),
         ("direct_variant", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("blocks_per_cu", ctypes.c_int32),
         ("scale", ctypes.c_double),
     ]
 
@@ -79,6 +79,18 @@ SIGNATURES = {
                                 _i64p, _vp, _optp]),
     "gsc_summate_grid_dev": (_int, [_int, _i64, _vp, _vp, _vp, _vpp, _i64p, _vp, _vp, _sz, _optp,
                                     _vp]),
+    "gsc_ensemble_create": (_int, [ctypes.POINTER(_vp), _int, _int, _i64p, _i64p,
+                                   ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double),
+                                   _vp, _vp, _vp, _vp, _vp, _optp]),
+    "gsc_ensemble_run": (_int, [_vp, _vp]),
+    "gsc_ensemble_info": (_int, [_vp, _i64p, _i64p, ctypes.POINTER(_sz),
+                                 ctypes.POINTER(ctypes.c_double)]),
+    "gsc_ensemble_destroy": (None, [_vp]),
+    "gsc_device_alloc": (_vp, [_sz, _int]),
+    "gsc_device_free": (None, [_vp]),
+    "gsc_memcpy_to_device": (_int, [_vp, _vp, _sz]),
+    "gsc_memcpy_to_host": (_int, [_vp, _vp, _sz]),
+    "gsc_synchronize": (_int, []),
 }
 
 _lib = None
@@ -144,6 +156,45 @@ def make_opts(out_dtype=GSC_F64, scale=1.0, device=-1, num_threads=0, path=GSC_P
     o.accum = int(accum)
     o.direct_variant = int(direct_variant)
     return o
+
+
+class DeviceBuffer:
+    """A device allocation owned by Python (gsc_device_alloc / gsc_device_free)."""
+
+    def __init__(self, nbytes: int, device: int = -1):
+        self.nbytes = int(nbytes)
+        self.ptr = lib().gsc_device_alloc(self.nbytes, int(device))
+        if not self.ptr:
+            raise MemoryError(last_error() or "device allocation failed")
+
+    @classmethod
+    def from_array(cls, a, device: int = -1) -> "DeviceBuffer":
+        import numpy as np
+        a = np.ascontiguousarray(a)
+        buf = cls(a.nbytes, device)
+        check(lib().gsc_memcpy_to_device(buf.ptr, a.ctypes.data, a.nbytes))
+        return buf
+
+    def to_array(self, dtype, count: int, offset_bytes: int = 0):
+        import numpy as np
+        out = np.empty(count, dtype=dtype)
+        check(lib().gsc_memcpy_to_host(out.ctypes.data, self.ptr + offset_bytes, out.nbytes))
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().gsc_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize() -> None:
+    check(lib().gsc_synchronize())
 
 
 def last_error() -> str:

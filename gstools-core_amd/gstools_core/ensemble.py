@@ -1,0 +1,138 @@
+"""Ensembles of realisations on structured grids in one graph-captured launch set.
+
+GSTools generates an ensemble by calling the summator once per realisation
+(one seed, one set of modes each).  ``Ensemble`` evaluates many realisations,
+each with its own modes and its own grid, through one plan of the C-ABI
+(gsc_ensemble_create): the inputs are uploaded once, the library builds flat
+work lists over all realisations (GEMM tiles for factored ones, point blocks
+for direct ones) and captures the launches in a HIP graph; ``run`` replays it.
+
+The per-realisation choice between the direct kernel and the factored MFMA
+contraction comes from the library rule, an explicit ``paths`` list, or the
+autotuner (``tune=True``, see gstools_core.tuner).
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _native
+from .grid import PATHS
+
+
+class Ensemble:
+    """Device-resident ensemble plan.
+
+    realisations: sequence of (cov_samples (d, M_r), z1 (M_r), z2 (M_r), axes)
+    with axes a list of d 1-D coordinate arrays; all realisations share d (2 or 3).
+    """
+
+    def __init__(self, realisations: Sequence, *, precision: str = "auto",
+                 out_dtype=np.float32, accum: str = "f64", paths: Optional[Sequence[str]] = None,
+                 scales: Optional[Sequence[float]] = None, tune: bool = False, tuner=None,
+                 device: int = -1, blocks_per_cu: int = 0):
+        if len(realisations) == 0:
+            raise ValueError("an ensemble needs at least one realisation")
+        dims = {len(r[3]) for r in realisations}
+        if len(dims) != 1:
+            raise ValueError("all realisations must have the same number of axes")
+        self.dim = dims.pop()
+        if self.dim not in (2, 3):
+            raise ValueError("ensembles support 2-D and 3-D grids")
+        self.count = len(realisations)
+        self.out_dtype = np.dtype(out_dtype)
+        if self.out_dtype not in (np.float32, np.float64):
+            raise ValueError("out_dtype must be float32 or float64")
+        covs, z1s, z2s, axs, shapes, ms = [], [], [], [], [], []
+        for cov, z1, z2, axes in realisations:
+            cov = np.ascontiguousarray(cov, dtype=np.float64)
+            if cov.shape[0] != self.dim or cov.shape[1] != len(z1) or cov.shape[1] != len(z2):
+                raise ValueError("cov_samples must be (d, M) with len(z1) == len(z2) == M")
+            covs.append(cov.ravel())
+            z1s.append(np.asarray(z1, dtype=np.float64))
+            z2s.append(np.asarray(z2, dtype=np.float64))
+            axs.extend(np.asarray(a, dtype=np.float64) for a in axes)
+            shapes.append([len(a) for a in axes])
+            ms.append(cov.shape[1])
+        self.shapes = [tuple(s) for s in shapes]
+        self.m_counts = ms
+        if tune:
+            from .tuner import Tuner
+            tuner = tuner or Tuner()
+            paths = [tuner.choose(self.dim, s, m, precision, accum) for s, m in zip(shapes, ms)]
+        path_codes = None
+        if paths is not None:
+            if len(paths) != self.count:
+                raise ValueError("one path per realisation")
+            path_codes = (ctypes.c_int32 * self.count)(*[PATHS[p] for p in paths])
+        scale_arr = None
+        if scales is not None:
+            if len(scales) != self.count:
+                raise ValueError("one scale per realisation")
+            scale_arr = (ctypes.c_double * self.count)(*[float(s) for s in scales])
+        cat = lambda xs: np.concatenate(xs) if xs else np.zeros(0)  # noqa: E731
+        self._cov = _native.DeviceBuffer.from_array(cat(covs), device)
+        self._z1 = _native.DeviceBuffer.from_array(cat(z1s), device)
+        self._z2 = _native.DeviceBuffer.from_array(cat(z2s), device)
+        self._ax = _native.DeviceBuffer.from_array(cat(axs), device)
+        self.n_points = [int(np.prod(s)) for s in shapes]
+        self._out = _native.DeviceBuffer(max(1, sum(self.n_points)) * self.out_dtype.itemsize,
+                                         device)
+        opts = _native.make_opts(
+            out_dtype=_native.GSC_F32 if self.out_dtype == np.float32 else _native.GSC_F64,
+            precision=_native.PRECISIONS[precision], accum=_native.ACCUM[accum], device=device)
+        opts.blocks_per_cu = int(blocks_per_cu)
+        c_shapes = (ctypes.c_int64 * (self.count * self.dim))(*[v for s in shapes for v in s])
+        c_ms = (ctypes.c_int64 * self.count)(*ms)
+        plan = ctypes.c_void_p()
+        rc = _native.lib().gsc_ensemble_create(
+            ctypes.byref(plan), self.count, self.dim, c_shapes, c_ms, path_codes, scale_arr,
+            self._cov.ptr, self._z1.ptr, self._z2.ptr, self._ax.ptr, self._out.ptr,
+            ctypes.byref(opts))
+        _native.check(rc)
+        self._plan = plan
+
+    @property
+    def info(self) -> dict:
+        nf, nd = ctypes.c_int64(), ctypes.c_int64()
+        wb, pm = ctypes.c_size_t(), ctypes.c_double()
+        _native.check(_native.lib().gsc_ensemble_info(self._plan, ctypes.byref(nf),
+                                                      ctypes.byref(nd), ctypes.byref(wb),
+                                                      ctypes.byref(pm)))
+        return {"factored": nf.value, "direct": nd.value, "workspace_bytes": wb.value,
+                "point_modes": pm.value}
+
+    def run(self, stream: int = 0) -> None:
+        """Replays the captured graph on `stream` (0: null stream); does not synchronise."""
+        _native.check(_native.lib().gsc_ensemble_run(self._plan, ctypes.c_void_p(stream or None)))
+
+    @property
+    def out_ptr(self) -> int:
+        return self._out.ptr
+
+    def results(self) -> List[np.ndarray]:
+        """Synchronises and copies every realisation's grid to the host."""
+        _native.synchronize()
+        flat = self._out.to_array(self.out_dtype, max(1, sum(self.n_points)))
+        res, off = [], 0
+        for s, n in zip(self.shapes, self.n_points):
+            res.append(flat[off:off + n].reshape(s))
+            off += n
+        return res
+
+    def close(self) -> None:
+        if getattr(self, "_plan", None):
+            _native.lib().gsc_ensemble_destroy(self._plan)
+            self._plan = None
+        for b in ("_cov", "_z1", "_z2", "_ax", "_out"):
+            buf = getattr(self, b, None)
+            if buf is not None:
+                buf.free()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

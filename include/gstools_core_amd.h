@@ -51,7 +51,7 @@ typedef enum gsc_dtype { GSC_F64 = 0, GSC_F32 = 1 } gsc_dtype;
 
 /* Evaluation path for structured grids (gsc_summate_grid*). */
 typedef enum gsc_path {
-  GSC_PATH_AUTO = 0,      /* tuner's choice */
+  GSC_PATH_AUTO = 0,      /* library rule (factored when both grid extents are >= 16) */
   GSC_PATH_DIRECT = 1,    /* per-point kernel (FP64 phase, FP32 sin) */
   GSC_PATH_FACTORED = 2   /* per-axis tables contracted on the MFMA cores */
 } gsc_path;
@@ -67,7 +67,7 @@ typedef enum gsc_precision {
 } gsc_precision;
 
 /* Accumulation of the factored path's FP32 MFMA partial sums, which are
- * flushed every 64 modes into a second accumulator set. */
+ * flushed every 256 modes into a second accumulator set. */
 typedef enum gsc_accum {
   GSC_ACCUM_F64 = 0,    /* second set in FP64 (default, most accurate) */
   GSC_ACCUM_F32X2 = 1,  /* second set in FP32 (fewer registers) */
@@ -90,7 +90,7 @@ typedef struct gsc_opts {
   int32_t num_threads;     /* host threads for staging strided arrays, 0 = all */
   int32_t accum;           /* gsc_accum, factored path only */
   int32_t direct_variant;  /* gsc_direct_variant, direct kernels only */
-  int32_t reserved;        /* zero */
+  int32_t blocks_per_cu;   /* persistent factored-GEMM blocks per CU in ensembles, 0 = 2 */
   double scale;            /* multiplies every output value (1.0 = reference) */
 } gsc_opts;
 
@@ -207,6 +207,43 @@ int gsc_summate_grid_dev(int dim, int64_t M, const double* cov, const double* z1
                          const double* z2, const double* const* axes, const int64_t* shape,
                          void* out, void* work, size_t work_bytes, const gsc_opts* opts,
                          void* stream);
+
+/*
+ * Ensembles of realisations on structured grids (no counterpart in the
+ * reference, whose callers loop over realisations in Python and call summate
+ * once per field).  `count` realisations of one dimension dim in {2, 3};
+ * realisation r has m_counts[r] modes and the grid shapes[r * dim + i].
+ * Device arrays hold the realisations back to back:
+ *   cov  (dim, M_r) row-major blocks;  z1, z2  M_r values each;
+ *   axes for each r its dim axes back to back;
+ *   out  for each r its dense C-order grid (opts->out_dtype).
+ * paths[r] is GSC_PATH_DIRECT, GSC_PATH_FACTORED or GSC_PATH_AUTO (NULL: all
+ * AUTO, the rule of gsc_grid_path); scales[r] multiplies realisation r
+ * (NULL: opts->scale for all).  The plan owns its device workspace and a HIP
+ * graph captured at creation: a counter reset, one mode-preparation launch,
+ * one table launch, one persistent factored-GEMM kernel and one persistent
+ * direct kernel that take 128x128 tiles / 2048-point blocks of all
+ * realisations from flat work lists (longest modes first).  gsc_ensemble_run
+ * replays the graph on `stream`; runs of one plan must not overlap.
+ */
+typedef struct gsc_ensemble gsc_ensemble;
+int gsc_ensemble_create(gsc_ensemble** plan, int count, int dim, const int64_t* shapes,
+                        const int64_t* m_counts, const int32_t* paths, const double* scales,
+                        const double* cov, const double* z1, const double* z2,
+                        const double* axes, void* out, const gsc_opts* opts);
+int gsc_ensemble_run(gsc_ensemble* plan, void* stream);
+/* n_factored / n_direct realisations, workspace bytes, total point-modes. */
+int gsc_ensemble_info(const gsc_ensemble* plan, int64_t* n_factored, int64_t* n_direct,
+                      size_t* workspace_bytes, double* point_modes);
+void gsc_ensemble_destroy(gsc_ensemble* plan);
+
+/* Device memory plumbing for hosts without their own allocator (e.g. plain
+ * numpy callers of the device-resident entry points). */
+void* gsc_device_alloc(size_t bytes, int device);
+void gsc_device_free(void* ptr);
+int gsc_memcpy_to_device(void* dst, const void* src, size_t bytes);
+int gsc_memcpy_to_host(void* dst, const void* src, size_t bytes);
+int gsc_synchronize(void);
 
 #ifdef __cplusplus
 }

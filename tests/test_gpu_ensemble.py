@@ -1,0 +1,137 @@
+"""GPU tier: ensembles (one graph-captured plan over many realisations) and the tuner.
+
+Each realisation of an ensemble must equal the single-grid evaluation of the
+same path (same kernels, same arithmetic) and the FP64 oracle within the
+path's tolerance (1e-5 sigma for the direct path and fp16x3).  The graph is
+replayed several times with eager library work in between, and every replay
+is checked (graph replays must not depend on state left by earlier ones).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+import gstools_core
+from gstools_core import _native, grid, workloads
+from gstools_core.ensemble import Ensemble
+
+pytestmark = pytest.mark.gpu
+
+
+def _reals(dim, shapes, ms, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for sh, m in zip(shapes, ms):
+        ls = rng.uniform(3.0, 12.0)
+        cov = rng.normal(0, 1.0 / ls, size=(dim, m))
+        z1, z2 = rng.normal(size=m), rng.normal(size=m)
+        axes = [np.arange(n) * rng.uniform(0.5, 1.5) + rng.uniform(-5, 5) for n in sh]
+        out.append((cov, z1, z2, axes))
+    return out
+
+
+SHAPES_2D = [(8, 9), (64, 100), (130, 257), (1, 40), (200, 3), (77, 129), (256, 256)]
+MS_2D = [50, 300, 1000, 17, 64, 513, 1000]
+
+
+@pytest.mark.parametrize("mode", ["auto", "direct", "factored", "mixed"])
+def test_ensemble_2d_matches_single_grid_calls(mode):
+    reals = _reals(2, SHAPES_2D, MS_2D, 10)
+    if mode == "auto":
+        paths = None
+    elif mode == "mixed":
+        paths = ["direct" if i % 2 else "factored" for i in range(len(reals))]
+    else:
+        paths = [mode] * len(reals)
+    ens = Ensemble(reals, paths=paths, out_dtype=np.float64, precision="fp16x3")
+    info = ens.info
+    assert info["factored"] + info["direct"] == len(reals)
+    ens.run()
+    res = ens.results()
+    for i, (cov, z1, z2, axes) in enumerate(reals):
+        p = paths[i] if paths else grid.grid_path(cov, axes)
+        if mode == "factored" and min(len(a) for a in axes) < 1:
+            p = "direct"
+        single = grid.summate_grid(cov, z1, z2, axes, path=p, precision="fp16x3")
+        np.testing.assert_allclose(res[i], single, rtol=0, atol=1e-9 * np.sqrt(cov.shape[1]))
+        ref = oracle.summate(cov, z1, z2, oracle.grid_points(axes)).reshape(single.shape)
+        assert np.abs(res[i] - ref).max() / np.sqrt(cov.shape[1]) <= 1e-5
+    ens.close()
+
+
+def test_ensemble_3d_and_scales():
+    shapes = [(16, 20, 33), (40, 8, 129), (5, 5, 5), (64, 64, 64)]
+    ms = [100, 257, 30, 1000]
+    reals = _reals(3, shapes, ms, 11)
+    scales = [1.0 / np.sqrt(m) for m in ms]
+    ens = Ensemble(reals, scales=scales, out_dtype=np.float32)
+    ens.run()
+    res = ens.results()
+    for r, (cov, z1, z2, axes), s in zip(res, reals, scales):
+        ref = oracle.summate(cov, z1, z2, oracle.grid_points(axes)).reshape(r.shape) * s
+        assert r.dtype == np.float32
+        assert np.abs(r - ref).max() <= 1e-5 + 2e-7 * np.abs(ref).max()
+    ens.close()
+
+
+def test_graph_replays_with_eager_work_between():
+    reals = _reals(2, SHAPES_2D[:5], MS_2D[:5], 12)
+    ens = Ensemble(reals, out_dtype=np.float64)
+    ens.run()
+    first = [r.copy() for r in ens.results()]
+    rng = np.random.default_rng(0)
+    for k in range(4):
+        # eager library work and synchronisation between replays
+        cov = rng.normal(size=(3, 40))
+        gstools_core.summate(cov, rng.normal(size=40), rng.normal(size=40),
+                             rng.normal(size=(3, 1000)))
+        grid.summate_grid(reals[2][0], reals[2][1], reals[2][2], reals[2][3])
+        ens.run()
+        again = ens.results()
+        for a, b in zip(first, again):
+            np.testing.assert_array_equal(a, b)
+    ens.close()
+
+
+def test_c4_realisations_in_one_plan():
+    ens_w = workloads.c4(count=512)[:24]
+    reals = [(w.modes.cov_samples, w.modes.z1, w.modes.z2, w.axes) for w in ens_w]
+    scales = [w.modes.scale for w in ens_w]
+    ens = Ensemble(reals, scales=scales, out_dtype=np.float32, precision="fp16x3")
+    ens.run()
+    res = ens.results()
+    worst = 0.0
+    for w, r in zip(ens_w, res):
+        idx = workloads.subset_indices(w.n_points, 2000, seed=41)
+        ref = oracle.summate(w.modes.cov_samples, w.modes.z1, w.modes.z2,
+                             workloads.grid_points(w.axes, idx)) * w.modes.scale
+        worst = max(worst, float(np.abs(r.ravel()[idx] - ref).max()))
+    print(f"c4 subset (24 realisations) max|err|/sigma = {worst:.3e}")
+    assert worst <= 1e-5
+    ens.close()
+
+
+def test_tuner_chooses_and_caches(tmp_path):
+    from gstools_core.tuner import Tuner
+    t = Tuner(path=str(tmp_path / "tune.json"), repeats=2)
+    p1 = t.choose(2, (300, 400), 1000, "fp16x3", "f64")
+    p2 = t.choose(2, (8, 9), 50, "fp16x3", "f64")
+    assert p1 in ("direct", "factored") and p2 in ("direct", "factored")
+    assert p1 == "factored"  # a 512 x 512 class at M = 1024 favours the MFMA path
+    t2 = Tuner(path=str(tmp_path / "tune.json"))
+    assert t2.choose(2, (300, 400), 1000, "fp16x3", "f64") == p1  # from the cache
+    reals = _reals(2, [(300, 400), (8, 9)], [1000, 50], 13)
+    ens = Ensemble(reals, tune=True, tuner=t2, out_dtype=np.float64)
+    ens.run()
+    for r, (cov, z1, z2, axes) in zip(ens.results(), reals):
+        ref = oracle.summate(cov, z1, z2, oracle.grid_points(axes)).reshape(r.shape)
+        assert np.abs(r - ref).max() / np.sqrt(cov.shape[1]) <= 1e-5
+    ens.close()
+
+
+def test_device_buffers_roundtrip():
+    a = np.arange(1000, dtype=np.float64)
+    b = _native.DeviceBuffer.from_array(a)
+    np.testing.assert_array_equal(b.to_array(np.float64, 1000), a)
+    b.free()
