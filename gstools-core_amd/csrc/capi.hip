@@ -308,7 +308,7 @@ int64_t grid_points(int dim, const int64_t* shape) {
 }
 
 // Path choice: the factored contraction needs a 2-D or 3-D grid; below a
-// 16-point last axis or 16 rows its 128 x 128 tiles are mostly padding.
+// 16-point last axis or 16 rows its GEMM block tiles are mostly padding.
 int choose_grid_path(int dim, const int64_t* shape, const gsc_opts* o) {
   if (dim == 1) return GSC_PATH_DIRECT;
   if (o->path == GSC_PATH_DIRECT || o->path == GSC_PATH_FACTORED) return o->path;
@@ -358,7 +358,7 @@ void gsc_opts_init(gsc_opts* o) {
   o->precision = GSC_PREC_AUTO;
   o->device = -1;
   o->num_threads = 0;
-  o->accum = GSC_ACCUM_F64;
+  o->accum = GSC_ACCUM_F32X2;
   o->direct_variant = GSC_DIRECT_VALU;
   o->scale = 1.0;
 }
@@ -580,8 +580,9 @@ int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes
   const size_t elt = o->out_dtype == GSC_F32 ? 4 : 8;
   std::vector<gsc::EnsReal> rs((size_t)count);
   std::vector<int> fact, dir, group_len;
-  std::vector<size_t> tab_off((size_t)count), p_off((size_t)count), q_off((size_t)count),
-      b_off((size_t)count);
+  std::vector<size_t> tab_off((size_t)count), f_off((size_t)count);
+  std::vector<gsc::FactoredPlan> plans((size_t)count);
+  const int tile_cols = gsc::gemm_tile_cols(gsc::precision_splits(o->precision), o->accum);
   // pass 1: geometry, offsets into the caller's arrays, path, workspace sizes
   int64_t m_off = 0, a_off = 0, o_off = 0;
   size_t ws = 0;
@@ -618,19 +619,18 @@ int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes
     if (e.npts == 0) continue;
     if (e.path == GSC_PATH_FACTORED) {
       const gsc::FactoredPlan fp = gsc::plan_factored(dim, M, sh, o->precision);
+      plans[(size_t)r] = fp;
       e.mp = fp.mp;
       e.na = fp.na;
-      e.nb = fp.nb;
-      e.rows = fp.rows;
+      e.nrow = fp.nrow;
+      e.arows = fp.arows;
+      e.nb_out = fp.nb_out;
       e.ncols = fp.ncols;
       e.ncp = fp.ncp;
-      e.col_tiles = fp.ncp / 128;
-      p_off[(size_t)r] = ws;
-      ws += align256(sizeof(float2) * (size_t)fp.na * fp.mp);
-      q_off[(size_t)r] = ws;
-      if (dim == 3) ws += align256(sizeof(float2) * (size_t)fp.nb * fp.mp);
-      b_off[(size_t)r] = ws;
-      ws += align256(2 * (size_t)fp.ns * fp.ncp * fp.kp);
+      e.row_tiles = fp.row_tiles;
+      e.col_tiles = (fp.ncols + tile_cols - 1) / tile_cols;
+      f_off[(size_t)r] = ws;  // S, A, C as laid out after the plan's own mode table
+      ws += align256(fp.bytes - fp.off_s);
       fact.push_back(r);
     } else {
       dir.push_back(r);
@@ -643,14 +643,14 @@ int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes
   std::stable_sort(fact.begin(), fact.end(), [&](int a, int b) {
     const gsc::EnsReal &ea = rs[(size_t)a], &eb = rs[(size_t)b];
     if (ea.mp != eb.mp) return ea.mp > eb.mp;
-    return ea.rows * ea.ncp > eb.rows * eb.ncp;
+    return ea.row_tiles * ea.col_tiles > eb.row_tiles * eb.col_tiles;
   });
   {
     std::vector<std::vector<int>> groups(gsc::kXcdGroups);
     std::vector<double> load(gsc::kXcdGroups, 0.0);
     for (int r : fact) {
       const gsc::EnsReal& e = rs[(size_t)r];
-      const double w = (double)((e.rows + 127) / 128 * e.col_tiles) * e.mp;
+      const double w = (double)(e.row_tiles * e.col_tiles) * e.mp;
       const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
       groups[(size_t)g].push_back(r);
       load[(size_t)g] += w;
@@ -667,11 +667,11 @@ int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes
   std::vector<int64_t> task_start(3 * fact.size() + 1, 0);
   for (size_t i = 0; i < fact.size(); ++i) {
     const gsc::EnsReal& e = rs[(size_t)fact[i]];
-    tile_start[i + 1] = tile_start[i] + (e.rows + 127) / 128 * e.col_tiles;
-    // table tasks: groups of 16 rows of P (na), Q (nb, 3-D only) and the B operand (ncp)
-    task_start[3 * i + 1] = task_start[3 * i] + (e.na + 15) / 16;
-    task_start[3 * i + 2] = task_start[3 * i + 1] + (dim == 3 ? (e.nb + 15) / 16 : 0);
-    task_start[3 * i + 3] = task_start[3 * i + 2] + e.ncp / 16;
+    tile_start[i + 1] = tile_start[i] + e.row_tiles * e.col_tiles;
+    // table tasks: groups of 64 rows of S (na, 3-D only), A (arows) and C (ncp)
+    task_start[3 * i + 1] = task_start[3 * i] + (dim == 3 ? (e.na + 63) / 64 : 0);
+    task_start[3 * i + 2] = task_start[3 * i + 1] + e.arows / 64;
+    task_start[3 * i + 3] = task_start[3 * i + 2] + e.ncp / 64;
   }
   for (size_t i = 0; i < dir.size(); ++i) {
     const gsc::EnsReal& e = rs[(size_t)dir[i]];
@@ -714,9 +714,11 @@ int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes
     e.amp_tab = reinterpret_cast<float*>(work + tab_off[(size_t)r] + align256(rb) +
                                          align256(sizeof(double) * 64 * (size_t)e.ntiles));
     if (e.path == GSC_PATH_FACTORED && e.npts > 0) {
-      e.P = reinterpret_cast<float2*>(work + p_off[(size_t)r]);
-      e.Q = reinterpret_cast<float2*>(work + q_off[(size_t)r]);
-      e.bt = work + b_off[(size_t)r];
+      const gsc::FactoredPlan& fp = plans[(size_t)r];
+      char* f = work + f_off[(size_t)r];
+      e.S = reinterpret_cast<float2*>(f);
+      e.A = reinterpret_cast<uint32_t*>(f + (fp.off_a - fp.off_s));
+      e.C = reinterpret_cast<float2*>(f + (fp.off_c - fp.off_s));
     }
   }
   std::vector<char> hctrl(ctrl, 0);
@@ -750,7 +752,7 @@ int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes
   el.counters = reinterpret_cast<int*>(base + c_desc + c_fact + c_ts + c_dir + c_bs + c_tk + c_gs);
   el.dim = dim;
   el.max_tiles = max_tiles;
-  const int64_t bpc = o->blocks_per_cu > 0 ? o->blocks_per_cu : 2;
+  const int64_t bpc = o->blocks_per_cu > 0 ? o->blocks_per_cu : 1;
   el.gemm_blocks = (int)std::max<int64_t>(1, std::min<int64_t>(tile_start.back(), bpc * cus));
   el.direct_blocks = (int)std::max<int64_t>(1, std::min<int64_t>(blk_start.back(), 8 * (int64_t)cus));
 

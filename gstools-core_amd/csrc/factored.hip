@@ -16,7 +16,7 @@ namespace {
 // ---------------------------------------------------------------------------
 // Ensembles: many realisations (own modes, own grid) in one graph of launches.
 // Every realisation has a descriptor; factored realisations contribute
-// 128 x 128 GEMM tiles, direct ones blocks of kEnsPts points, to two flat work
+// GEMM block tiles, direct ones blocks of kEnsPts points, to two flat work
 // lists consumed by persistent kernels through atomic counters.
 // ---------------------------------------------------------------------------
 __global__ void k_ens_reset(int* __restrict__ counters) {
@@ -85,27 +85,30 @@ FactoredPlan plan_factored(int dim, int64_t M, const int64_t* shape, int precisi
   p.ns = precision_splits(precision);
   p.bf16 = precision_is_bf16(precision);
   p.mp = static_cast<int>(((M > 0 ? M : 1) + 31) / 32 * 32);
+  p.nch = p.mp / 16;
   p.kp = 2 * static_cast<int64_t>(p.mp);
   if (dim == 2) {
-    p.na = shape[0];
-    p.nb = 1;
-    p.rows = shape[0];
+    p.na = 1;
+    p.nrow = shape[0];
+    p.nb_out = 0;
     p.ncols = shape[1];
   } else {
     p.na = shape[0];
-    p.nb = shape[1];
-    p.rows = shape[0] * shape[1];
+    p.nrow = shape[1];
+    p.nb_out = shape[1];
     p.ncols = shape[2];
   }
-  p.ncp = (p.ncols + 127) / 128 * 128;  // the GEMM column tile is 128
+  p.arows = (p.nrow + kBlockRows - 1) / kBlockRows * kBlockRows;
+  p.ncp = (p.ncols + kMaxTileCols - 1) / kMaxTileCols * kMaxTileCols;
+  p.row_tiles = (dim == 3 ? p.na : 1) * (p.arows / kBlockRows);
   auto al = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
   size_t off = al(mode_table_bytes(3, M));
-  p.off_p = off;
-  off += al(sizeof(float2) * static_cast<size_t>(p.na) * p.mp);
-  p.off_q = off;
-  if (dim == 3) off += al(sizeof(float2) * static_cast<size_t>(p.nb) * p.mp);
-  p.off_bt = off;
-  off += al(2 * static_cast<size_t>(p.ns) * p.ncp * p.kp);
+  p.off_s = off;
+  if (dim == 3) off += al(sizeof(float2) * static_cast<size_t>(p.na) * p.mp);
+  p.off_a = off;
+  off += al(sizeof(uint32_t) * 16 * static_cast<size_t>(p.ns) * p.nch * p.arows);
+  p.off_c = off;
+  off += al(sizeof(float2) * static_cast<size_t>(p.ncp) * p.mp);
   p.bytes = off;
   return p;
 }
@@ -139,22 +142,27 @@ template <typename T, int NS> struct EnsF {
 hipError_t launch_factored(const FactoredPlan& p, int64_t M, const double* const* axes,
                            void* work, void* out, int out_dtype, double scale, int accum,
                            hipStream_t s) {
+  if (p.row_tiles == 0 || p.ncols == 0) return hipSuccess;  // an empty grid
   char* w = static_cast<char*>(work);
   const ModeRec* modes = reinterpret_cast<const ModeRec*>(w);
-  float2* P = reinterpret_cast<float2*>(w + p.off_p);
-  float2* Q = reinterpret_cast<float2*>(w + p.off_q);
-  void* bt = w + p.off_bt;
+  float2* S = reinterpret_cast<float2*>(w + p.off_s);
+  uint32_t* A = reinterpret_cast<uint32_t*>(w + p.off_a);
+  float2* C = reinterpret_cast<float2*>(w + p.off_c);
   const int Mi = static_cast<int>(M);
+  const bool kr = p.dim == 3;
   // tables (the mode table was written by launch_prep): one launch
-  const int64_t nmax = std::max(std::max(p.na, p.dim == 3 ? p.nb : 0), p.ncp);
-  const dim3 tg(static_cast<unsigned>((nmax + 15) / 16), static_cast<unsigned>(p.mp / 16), 3);
-  TableArgs ta{modes, Mi, p.dim, axes[0], p.dim == 3 ? axes[1] : nullptr, axes[p.dim - 1],
-               p.na, p.nb, p.ncols, p.ncp, P, Q, bt};
+  const int64_t nmax = std::max(std::max(kr ? p.na : 0, p.arows), p.ncp);
+  const dim3 tg(static_cast<unsigned>((nmax + 63) / 64), static_cast<unsigned>(p.nch), 3);
+  TableArgs ta{modes,     Mi,     p.dim,   p.nch,   axes[0], axes[kr ? 1 : 0], axes[p.dim - 1],
+               p.na,      p.nrow, p.arows, p.ncols, p.ncp,   S,               A,
+               C};
   hipError_t e = by_format<TablesF>(p.bf16, p.ns, ta, tg, s);
   if (e != hipSuccess) return e;
-  GemmArgs ga{P, Q, p.na, p.nb, p.rows, p.mp / 16, bt, p.ncp, p.ncols, out, out_dtype != 1,
-              p.ncols, scale, p.dim == 3, accum};
-  const dim3 grid(static_cast<unsigned>((p.rows + 127) / 128), static_cast<unsigned>(p.ncp / 128));
+  GemmArgs ga{A,       p.arows, p.nch, S,     p.na,  C,  p.ncp, p.nrow, p.nb_out, out,
+              out_dtype != 1, p.ncols, p.ncols, scale, kr, accum};
+  const int64_t tile_cols = gemm_tile_cols(p.ns, accum);
+  const dim3 grid(static_cast<unsigned>(p.row_tiles),
+                  static_cast<unsigned>((p.ncols + tile_cols - 1) / tile_cols));
   return by_format<GemmF>(p.bf16, p.ns, ga, grid, s);
 }
 

@@ -20,25 +20,48 @@ inline int precision_splits(int prec) {
 }
 inline bool precision_is_bf16(int prec) { return prec == 2 || prec == 3 || prec == 5; }
 
+// Operand roles.  The GEMM rows run over the A axis (axis 0 in 2-D, axis 1
+// in 3-D), its columns over the last axis; K = 2M interleaves the real and
+// imaginary parts of every mode.  In 3-D a block works on one index a of
+// axis 0 and multiplies that axis' factor into the B operand.
+//   S: axis 0 of a 3-D grid (amplitude and theta folded in): na x mp float2
+//   A: the A axis, split into ns parts: ns x nch x arows x 16 packed pairs
+//      (2-D: amplitude and theta folded in); rows >= the axis length are zero
+//   C: last axis, ncp x mp float2; columns >= ncols are zero
 // Workspace layout (bytes from the start of the caller's workspace):
-//   [0, off_p)     ModeRec table (M records)
-//   [off_p, ...)   P: na x mp float2   (axis 0, amplitude and theta folded in)
-//   [off_q, ...)   Q: nb x mp float2   (axis 1, 3-D only)
-//   [off_bt, ...)  B operand (last axis, split): ns x ncp x kp elements of fp16/bf16
-// (all chunk-major: 16 modes of one row are contiguous, rows of one chunk
-// follow each other).
+//   [0, off_s)       ModeRec table (M records)
+//   [off_s, off_a)   S (3-D only)
+//   [off_a, off_c)   A
+//   [off_c, bytes)   C
+// (all chunk-major: the 16 modes of one row are contiguous, rows of one
+// chunk follow each other).
+constexpr int kBlockRows = 256;  // GEMM block rows: 4 waves x 64
+constexpr int kMaxTileCols = 128;
 struct FactoredPlan {
   int dim;
   int ns;
   bool bf16;
-  int mp;          // modes padded to a multiple of 32 (an even number of 16-mode chunks)
-  int64_t kp;      // 2 * mp
-  int64_t na, nb;  // rows = na * nb (nb = 1 in 2-D)
-  int64_t rows;
-  int64_t ncols;   // last-axis length
-  int64_t ncp;     // ncols padded to the column tile
-  size_t off_p, off_q, off_bt, bytes;
+  int mp;            // modes padded to a multiple of 32 (an even number of 16-mode chunks)
+  int nch;           // mp / 16
+  int64_t kp;        // 2 * mp
+  int64_t na;        // 3-D: axis 0 (one block row band per index); 2-D: 1
+  int64_t nrow;      // length of the A axis
+  int64_t arows;     // nrow padded to kBlockRows
+  int64_t nb_out;    // 3-D: output rows per axis-0 index (= nrow); 2-D: 0
+  int64_t ncols;     // last-axis length
+  int64_t ncp;       // ncols padded to kMaxTileCols
+  int64_t row_tiles; // 3-D: na * arows / kBlockRows; 2-D: arows / kBlockRows
+  size_t off_s, off_a, off_c, bytes;
 };
+
+// Columns of a GEMM block tile for an operand format and accumulation mode:
+// the wave keeps 2 x tile/32 accumulator blocks, plus a second set for
+// the two-level modes, within its register file.
+constexpr int gemm_tile_cols(int ns, int accum) {
+  if (accum == 0) return 32;  // FP64 second set
+  if (ns == 3) return 64;
+  return 128;
+}
 
 FactoredPlan plan_factored(int dim, int64_t M, const int64_t* shape, int precision);
 
@@ -54,15 +77,15 @@ struct EnsReal {
   ModeRec* table;     // mode records, then the matrix-core tile tables
   double* a_tab;
   float* amp_tab;
-  float2* P;          // factored tables (chunk-major)
-  float2* Q;
-  void* bt;
+  float2* S;          // factored tables (roles as in FactoredPlan)
+  uint32_t* A;
+  float2* C;
   int64_t M;
   int ntiles;         // 16-mode tiles of the prep kernel
   int dim;
   int path;           // GSC_PATH_DIRECT or GSC_PATH_FACTORED
   int mp;             // factored: modes padded to 32
-  int64_t na, nb, rows, ncols, ncp, col_tiles;  // factored geometry
+  int64_t na, nrow, arows, nb_out, ncols, ncp, row_tiles, col_tiles;  // factored geometry
   int64_t s0, s1, s2, npts;                     // grid shape
   double scale;
 };
@@ -77,7 +100,7 @@ struct EnsLaunch {
   const int* fact;
   int nfact;
   const int64_t* tile_start;
-  const int64_t* task_start;   // table tasks: 3 nfact + 1 prefix sums
+  const int64_t* task_start;   // table tasks: 3 nfact + 1 prefix sums (roles S, A, C)
   const int64_t* group_start;  // tile ranges of the kXcdGroups groups (kXcdGroups + 1)
   int64_t n_table_tasks;
   const int* dir;
@@ -100,25 +123,28 @@ hipError_t launch_ensemble(const EnsLaunch& e, int precision, int out_dtype, int
 // Arguments of the per-operand-format launchers (gsc_gemm.h, gemm_*.hip).
 struct TableArgs {
   const ModeRec* modes;
-  int M, dim;
-  const double* x0;
-  const double* x1;
-  const double* xl;
-  int64_t na, nb, ncols, ncp;
-  float2* P;
-  float2* Q;
-  void* bt;
+  int M, dim, nch;
+  const double* xs;  // axis of S (3-D), A and C
+  const double* xa;
+  const double* xc;
+  int64_t na, nrow, arows, ncols, ncp;
+  float2* S;
+  uint32_t* A;
+  float2* C;
 };
 struct GemmArgs {
-  const float2* P;
-  const float2* Q;
-  int64_t na, nb, rows;
+  const uint32_t* A;
+  int64_t arows;
   int nch;
-  const void* bt;
-  int64_t ncp, ncols;
+  const float2* S;
+  int64_t na;
+  const float2* C;
+  int64_t ncp;
+  int64_t nrow;    // valid rows of the A axis
+  int64_t nb_out;  // 3-D: output row of (a, b) is a * nb_out + b
   void* out;
   bool out_f64;
-  int64_t ldo;
+  int64_t ldo, ncols;
   double scale;
   bool kr;
   int accum;
@@ -126,7 +152,7 @@ struct GemmArgs {
 
 // Expects the mode table at the start of `work` (launch_prep, KIND_SCALAR).
 // axes: device pointers, one per dimension.  out: dense C-order grid.
-// accum: 0 = FP64 flush (default), 1 = two-level FP32, 2 = plain FP32.
+// accum: 0 = FP64 second level, 1 = two-level FP32 (default), 2 = plain FP32.
 hipError_t launch_factored(const FactoredPlan& p, int64_t M, const double* const* axes,
                            void* work, void* out, int out_dtype, double scale, int accum,
                            hipStream_t stream);

@@ -1,35 +1,48 @@
 // Factored summation on structured grids, contracted on the MFMA cores:
-// device templates, instantiated per operand type in gemm_*.hip.
+// device templates, instantiated per operand format in gemm_*.hip.
 //
 // On a grid x = (x_a, y_b, z_c) the per-mode phase separates, and with the
 // amplitude-phase form used throughout this library
 //   z1 cos(phi) + z2 sin(phi) = A sin(phi + theta),   A = hypot(z1, z2),
 //   theta = atan2(z1, z2),
 // the field is the imaginary part of a product of per-axis tables:
-//   u(a, b, c) = Im sum_m P(a, m) Q(b, m) R(c, m)
-//   P(a, m) = A_m e^{i(theta_m + k_xm x_a)},  Q(b, m) = e^{i k_ym y_b},
-//   R(c, m) = e^{i k_zm z_c}.
-// With W = P Q (complex, per row (a, b)) and K = 2M interleaved as
-// k = 2m -> (Re W, Im R), k = 2m + 1 -> (Im W, Re R):
-//   u(row, c) = sum_k A(row, k) B(k, c),  A = [Re W | Im W],  B = [Im R ; Re R]
-// a real GEMM (rows = Nx Ny, cols = Nz).  In 2-D, W = P and B is built from Q.
+//   u(a, b, c) = Im sum_m S(a, m) Q(b, m) C(c, m)
+//   S(a, m) = A_m e^{i(theta_m + k_xm x_a)},  Q(b, m) = e^{i k_ym y_b},
+//   C(c, m) = e^{i k_zm z_c}.
+// For a fixed a this is a real GEMM over K = 2M, rows b, columns c:
+//   u(a, b, c) = sum_k A(b, k) B_a(k, c),  A = Q, B_a = S(a) C,
+// with K interleaved as k = 2m -> (Re A, Im B), k = 2m + 1 -> (Im A, Re B),
+// so that Re A Im B + Im A Re B = Im(A B).  In 2-D the A operand is the
+// axis-0 table (amplitude and theta folded in) and B is the axis-1 table.
 //
-// Tables are evaluated in FP64 (phase in revolutions, exact fract range
-// reduction, sincospi) and rounded to FP32.  The MFMA operands are split
-// into NS parts of T (fp16 or bf16): x = sum_s x_s, x_0 = T(x),
+// Tables are evaluated with the phase in FP64 revolutions, reduced exactly
+// (t - rint(t)), rounded to FP32 and passed to the FP32 sine and cosine
+// instructions (argument in revolutions).  MFMA operands are split into NS
+// parts of T (fp16 or bf16): x = sum_s x_s, x_0 = T(x),
 // x_{s+1} = T(x - sum_{t<=s} x_t) (each difference exact in FP32), and the
 // products with i + j < NS are accumulated in FP32 on the matrix cores:
 // NS = 1 -> 1 product, NS = 2 -> 3 products (x3), NS = 3 -> 6 products (x6).
 //
-// Kernel shape: 256 threads (4 waves, 2 x 2), block tile 128 x 128, wave tile
-// 64 x 64 = 2 x 2 v_mfma_f32_32x32x16 blocks, K chunks of 32 (16 modes)
-// staged in LDS.  The A tile is generated from P and Q while the previous
-// chunk's MFMAs run (the Khatri-Rao operand is never written to HBM).
+// Kernel shape: 256 threads = 4 waves stacked in rows, block tile
+// 256 rows x 32 JN columns; a wave owns 64 rows x 32 JN columns
+// (2 x JN v_mfma_f32_32x32x16 blocks).  The A operand is split once, when the
+// table is built, and every wave loads its fragments straight from the
+// table (global memory / L2) into registers in the MFMA layout: no LDS and
+// no arithmetic for A.  The B operand (S(a) C in 3-D, C in 2-D) is built per
+// block and 16-mode chunk by the four waves (complex product, split) into a
+// double-buffered LDS tile, one barrier per chunk.  Per chunk a wave issues
+// 2 JN NS(NS+1)/2 MFMAs per 16 k and 4 x (JN > 1 ? 32 JN / 64 : 1) complex
+// products for B.
 //
-// Table layouts are chunk-major so that one chunk of one block is contiguous:
-//   P/Q/R: tab[(c * n + a) * 16 + mm]            mode m = 16 c + mm, a < n
-//   B:     bt[((s * nch + c) * ncp + col) * 32 + kk], kk = 2 mm (+1 for Re)
-// with the mode count padded to mp = 32 * ceil(M / 32) (nch = mp / 16 chunks,
+// Table layouts, chosen so that every load instruction of the GEMM reads
+// whole cache lines (mode m = 16 c + mm, pair = (Re, Im)):
+//   S (float2):        S[(c * na + a) * 16 + mm]
+//   A (packed pairs):  A[((((s * nch + c) * 2 + ks) * 2 + h) * arows + row) * 4 + e],
+//                      mm = 8 ks + 4 h + e: the 4 pairs one MFMA lane holds
+//   C (float2):        C[((m / 2) * ncp + col) * 2 + m % 2]: mode pairs of one
+//                      column contiguous, columns of one mode pair contiguous
+// with the mode count padded to mp = 32 ceil(M / 32) (nch = mp / 16 chunks),
+// padded modes, rows and columns zero.
 #pragma once
 
 #include "gsc_device.h"
@@ -44,29 +57,44 @@ namespace gemm {
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kTM = 128;   // block rows
-constexpr int kTN = 128;   // block cols
-constexpr int kKC = 32;    // k per chunk (16 modes)
-constexpr int kPad = 8;    // LDS row padding (elements)
-constexpr int kLd = kKC + kPad;
+constexpr int kLdB = 40;    // LDS row of the B tile: 32 k + 8 pad (T elements, 80 bytes)
+constexpr int kFlush = 16;  // chunks (256 modes) per FP32 accumulation interval
 // Accumulation (ACC template argument).  The MFMA accumulates in FP32; over
 // K = 2M terms its rounding grows with the partial sum (~8e-6 sigma at
-// M = 4000).  Every kFlush chunks (256 modes) the FP32 accumulators are added
-// into a second set and cleared, so FP32 rounding only ever sees a 256-mode
-// partial sum (256 modes):
-//   ACC_F64   second set in FP64  (most accurate, +128 VGPRs)
-//   ACC_F32X2 second set in FP32  (~1e-6 sigma at M = 4000, +64 VGPRs)
-//   ACC_F32   no second set       (fastest, error grows with M)
-constexpr int kFlush = 16;  // even: the main loop is unrolled by two
+// M = 4000).  Every kFlush chunks the FP32 accumulators are added into a
+// second set and restarted, so FP32 rounding only ever sees 256 modes:
+//   ACC_F64   second set in FP64  (1 column block per wave)
+//   ACC_F32X2 second set in FP32  (~1e-6 sigma at M = 4000)
+//   ACC_F32   no second set       (error grows with M)
 enum { ACC_F64 = 0, ACC_F32X2 = 1, ACC_F32 = 2 };
 template <int ACC> struct AccT { using type = double; };
 template <> struct AccT<ACC_F32X2> { using type = float; };
 template <> struct AccT<ACC_F32> { using type = float; };
 
-template <typename T> struct Vec8;
-template <> struct Vec8<_Float16> { using type = half8_t; };
-template <> struct Vec8<__bf16> { using type = bf16x8_t; };
+// Column blocks (of 32) per wave: the accumulators, the second set and the
+// operand fragments must fit one wave's register file.
+template <int NS, int ACC>
+constexpr int cols_blocks() {
+  return ACC == ACC_F64 ? 1 : (NS == 3 ? 2 : 4);
+}
+
+static_assert(32 * cols_blocks<2, ACC_F32X2>() == gemm_tile_cols(2, ACC_F32X2) &&
+                  32 * cols_blocks<3, ACC_F32>() == gemm_tile_cols(3, ACC_F32) &&
+                  32 * cols_blocks<1, ACC_F64>() == gemm_tile_cols(1, ACC_F64),
+              "host and device tile widths agree");
+
+template <typename T> struct Fmt;
+template <> struct Fmt<_Float16> {
+  using v8 = half8_t;
+  using v2 = _Float16 __attribute__((ext_vector_type(2)));
+};
+template <> struct Fmt<__bf16> {
+  using v8 = bf16x8_t;
+  using v2 = __bf16 __attribute__((ext_vector_type(2)));
+};
 
 __device__ __forceinline__ f32x16_t mfma(half8_t a, half8_t b, f32x16_t c) {
   return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
@@ -75,329 +103,385 @@ __device__ __forceinline__ f32x16_t mfma(bf16x8_t a, bf16x8_t b, f32x16_t c) {
   return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
 }
 
+// (x, y) -> NS packed pairs of T with (x, y) = sum_s pair_s, round to nearest
+// even; each residual x - sum_{t<=s} x_t is exact in FP32.
 template <typename T, int NS>
-__device__ __forceinline__ void split(float x, T (&o)[NS]) {
-  float r = x;
+__device__ __forceinline__ void split_pair(f2v v, uint32_t (&o)[NS]) {
+  using V2 = typename Fmt<T>::v2;
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
-    const T h = static_cast<T>(r);
-    o[s] = h;
-    r = r - static_cast<float>(h);
+    const V2 h = __builtin_convertvector(v, V2);
+    o[s] = __builtin_bit_cast(uint32_t, h);
+    if (s + 1 < NS) v = v - __builtin_convertvector(h, f2v);
   }
 }
 
-// ---------------------------------------------------------------------------
-// Per-axis complex tables, tab[m / 16][a][m % 16] = amp_m e^{2 pi i (kap_{axis,m} x_a + th_m)}
-// (amp = A, th = theta/2pi for axis 0; amp = 1, th = 0 otherwise), FP64 -> FP32.
-// Modes m >= M are zero.
-// ---------------------------------------------------------------------------
-// All tables of one call in one launch.  blockIdx.z selects the table:
-//   0: P (axis 0, amplitude and theta folded in), chunk-major float2
-//   1: Q (axis 1, 3-D only), chunk-major float2
-//   2: the B operand, split straight from the FP64 value of R (last axis),
-//      columns ncols <= col < ncp zero-filled.
-// block = 16 rows x 16 modes of one chunk: grid (ceil(max(na, nb, ncp) / 16), mp / 16, 3).
-// sin and cos of 2 pi r for |r| <= 1/2, r exact in FP64: r = rh + rl with
-// rh = fl32(r), and sin(2 pi (rh + rl)) = sin(2 pi rh) + 2 pi rl cos(2 pi rh)
-// + O(rl^2) with |rl| <= 2^-26, so the FP32 result keeps the ~1 ulp of
-// sincospif instead of inheriting the rounding of r.
-__device__ __forceinline__ void sincos_2pi(double r, float& sn, float& cs) {
-  const float rh = static_cast<float>(r);
-  const float rl = static_cast<float>(r - static_cast<double>(rh));
-  float sh, ch;
-  sincospif(2.0f * rh, &sh, &ch);
-  const float dl = 6.283185307179586f * rl;
-  sn = __builtin_fmaf(dl, ch, sh);
-  cs = __builtin_fmaf(-dl, sh, ch);
+// amp e^{2 pi i (kap_axis x + th)} of one mode at one coordinate (amp = 1,
+// th = 0 unless with_amp): FP64 phase in revolutions, exact reduction to
+// [-1/2, 1/2], then the FP32 cosine and sine instructions.
+__device__ __forceinline__ f2v table_value(const ModeRec& md, int axis, double x, bool with_amp) {
+  double t = md.kap[axis] * x;
+  if (with_amp) t += md.th;
+  const float r = static_cast<float>(t - rint(t));
+  const float amp = with_amp ? md.amp[0] : 1.0f;
+  return f2v{amp * __builtin_amdgcn_cosf(r), amp * __builtin_amdgcn_sinf(r)};
 }
 
-template <typename T>
-struct alignas(4) Pair {
-  T im, re;
-};
-
-// One entry (row a, mode 16 c + mm) of a table: which = 0 -> P (axis 0,
-// amplitude and theta folded in), 1 -> Q (axis 1, 3-D), 2 -> B operand (last
-// axis, split into NS parts; columns ncols <= a < ncp are zero).
+// Four table entries: role 0 -> S (3-D axis 0), 1 -> A (split), 2 -> C; row
+// `row` of that table, modes 16 c + 4 g .. + 3 (g < 4).  Each role's store
+// is one 16-byte vector per part, consecutive rows contiguous for A and C.
 template <typename T, int NS>
-__device__ __forceinline__ void table_entry(const ModeRec* __restrict__ modes, int M, int dim,
-                                            const double* __restrict__ x0,
-                                            const double* __restrict__ x1,
-                                            const double* __restrict__ xl, int64_t na, int64_t nb,
-                                            int64_t ncols, int64_t ncp, float2* __restrict__ P,
-                                            float2* __restrict__ Q, T* __restrict__ bt, int which,
-                                            int64_t a, int c, int nch, int mm) {
-  if (which == 1 && dim != 3) return;
-  const int m = c * 16 + mm;
-  const int64_t n = which == 0 ? na : (which == 1 ? nb : ncp);
-  if (a >= n || c >= nch) return;
-  const double* x = which == 0 ? x0 : (which == 1 ? x1 : xl);
-  const int axis = which == 2 ? dim - 1 : which;
-  float re = 0.0f, im = 0.0f;
-  if (m < M && (which != 2 || a < ncols)) {
-    const ModeRec md = modes[m];
-    double t = md.kap[axis] * x[a];
-    if (which == 0) t += md.th;
-    float sn, cs;
-    sincos_2pi(t - rint(t), sn, cs);  // t - rint(t) is exact: |r| <= 1/2 revolution
-    const float amp = which == 0 ? md.amp[0] : 1.0f;
-    re = amp * cs;
-    im = amp * sn;
+__device__ __forceinline__ void table_quad(const ModeRec* __restrict__ modes, int M, int dim,
+                                           int nch, const double* __restrict__ xs,
+                                           const double* __restrict__ xa,
+                                           const double* __restrict__ xc, int64_t na,
+                                           int64_t nrow, int64_t arows, int64_t ncols,
+                                           int64_t ncp, float2* __restrict__ S,
+                                           uint32_t* __restrict__ A, float2* __restrict__ C,
+                                           int role, int64_t row, int c, int g) {
+  if (c >= nch) return;
+  const int m0 = c * 16 + 4 * g;
+  const int64_t cc = c;
+  int axis;
+  bool amp, valid;
+  double x = 0.0;
+  if (role == 0) {
+    if (dim != 3 || row >= na) return;
+    axis = 0;
+    amp = true;
+    valid = true;
+    x = xs[row];
+  } else if (role == 1) {
+    if (row >= arows) return;
+    axis = dim - 2;
+    amp = dim == 2;
+    valid = row < nrow;
+    if (valid) x = xa[row];
+  } else {
+    if (row >= ncp) return;
+    axis = dim - 1;
+    amp = false;
+    valid = row < ncols;
+    if (valid) x = xc[row];
   }
-  if (which < 2) {
-    float2* tab = which == 0 ? P : Q;
-    tab[(static_cast<int64_t>(c) * n + a) * 16 + mm] = make_float2(re, im);
-    return;
-  }
-  T hi[NS], hr[NS];
-  split<T, NS>(im, hi);
-  split<T, NS>(re, hr);
+  f2v v[4];
 #pragma unroll
-  for (int s = 0; s < NS; ++s) {
-    Pair<T>* row = reinterpret_cast<Pair<T>*>(bt + ((static_cast<int64_t>(s) * nch + c) * ncp + a) * 32);
-    row[mm] = Pair<T>{hi[s], hr[s]};
+  for (int e = 0; e < 4; ++e)
+    v[e] = (valid && m0 + e < M) ? table_value(modes[m0 + e], axis, x, amp) : f2v{0.0f, 0.0f};
+  if (role == 0) {
+    float4* d = reinterpret_cast<float4*>(S + (cc * na + row) * 16 + 4 * g);
+    d[0] = make_float4(v[0].x, v[0].y, v[1].x, v[1].y);
+    d[1] = make_float4(v[2].x, v[2].y, v[3].x, v[3].y);
+  } else if (role == 1) {
+    uint32_t o[4][NS];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) split_pair<T, NS>(v[e], o[e]);
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      *reinterpret_cast<u32x4*>(A + ((((s * nch + cc) * 4) + g) * arows + row) * 4) =
+          u32x4{o[0][s], o[1][s], o[2][s], o[3][s]};
+  } else {
+    float4* d = reinterpret_cast<float4*>(C);
+    d[(8 * cc + 2 * g) * ncp + row] = make_float4(v[0].x, v[0].y, v[1].x, v[1].y);
+    d[(8 * cc + 2 * g + 1) * ncp + row] = make_float4(v[2].x, v[2].y, v[3].x, v[3].y);
   }
 }
 
+// All tables of one call in one launch.  block = 64 rows x 16 modes of one
+// chunk (thread: row t % 64, modes 4 (t / 64) .. + 3):
+// grid (ceil(max rows / 64), nch, 3), blockIdx.z = role.
 template <typename T, int NS>
 __global__ void __launch_bounds__(256)
-k_tables(const ModeRec* __restrict__ modes, int M, int dim, const double* __restrict__ x0,
-         const double* __restrict__ x1, const double* __restrict__ xl, int64_t na, int64_t nb,
-         int64_t ncols, int64_t ncp, float2* __restrict__ P, float2* __restrict__ Q,
-         T* __restrict__ bt) {
-  table_entry<T, NS>(modes, M, dim, x0, x1, xl, na, nb, ncols, ncp, P, Q, bt, blockIdx.z,
-                     static_cast<int64_t>(blockIdx.x) * 16 + (threadIdx.x >> 4), blockIdx.y,
-                     gridDim.y, threadIdx.x & 15);
+k_tables(const ModeRec* __restrict__ modes, int M, int dim, int nch,
+         const double* __restrict__ xs, const double* __restrict__ xa,
+         const double* __restrict__ xc, int64_t na, int64_t nrow, int64_t arows, int64_t ncols,
+         int64_t ncp, float2* __restrict__ S, uint32_t* __restrict__ A, float2* __restrict__ C) {
+  table_quad<T, NS>(modes, M, dim, nch, xs, xa, xc, na, nrow, arows, ncols, ncp, S, A, C,
+                    blockIdx.z, static_cast<int64_t>(blockIdx.x) * 64 + (threadIdx.x & 63),
+                    blockIdx.y, threadIdx.x >> 6);
 }
 
 // ---------------------------------------------------------------------------
-// The GEMM.  rows = n_rows (KR: n_rows = na * nb, row = a * nb + b), cols.
-// out[row * ldo + col] = scale * sum_k A(row, k) B(k, col).
+// The GEMM.  One block tile: rows arow0 .. arow0 + 255 of the A table
+// (output rows out_row0 + r for r < nvalid), columns col0 .. col0 + 32 JN - 1
+// (stored for col < ncols):  out[(out_row0 + r) * ldo + col] = scale * u.
 //
-// Software pipeline, one barrier per K chunk, two register sets for the
-// global loads and two LDS buffers: in the step for chunk c the wave
-//   1. issues the global loads of chunk c + 2 (P, Q rows, B tile) into the
-//      register set freed by chunk c,
-//   2. reads chunk c's A/B fragments from LDS buffer c % 2 and issues its
-//      MFMAs,
-//   3. generates chunk c + 1's A tile from the register set loaded one step
-//      earlier (complex products, splits) and writes it with the B tile to
-//      the other LDS buffer, interleaved with the MFMAs of step 2.
-// The loop is unrolled by two so buffer and register-set indices are
-// compile-time constants.
+// Software pipeline, one barrier per chunk: in the step for chunk c a wave
+//   1. requests chunk c + 1's A fragments into the other register set and
+//      reads chunk c's ks = 1 B fragments from LDS buffer c % 2,
+//   2. issues chunk c's ks = 0 MFMAs (fragments read in the previous step)
+//      and, between them, builds chunk c + 1's B tile into the other buffer
+//      from C (and S) values loaded one step earlier,
+//   3. waits at the barrier, requests the C (and S) values of chunk c + 2
+//      and reads chunk c + 1's ks = 0 B fragments,
+//   4. issues chunk c's ks = 1 MFMAs.
 // ---------------------------------------------------------------------------
-template <typename T, int NS>
+template <typename T, int NS, int JN>
 struct GemmSmem {
-  T As[2][NS][kTM][kLd];
-  T Bs[2][NS][kTN][kLd];
+  T B[2][NS][32 * JN][kLdB];
 };
 
-// One 128 x 128 output tile (rows row0.., cols col0..) of one GEMM; used by
-// the grid kernel below and by the persistent ensemble kernel.  Ends with a
-// barrier after the last LDS read, so the caller may start another tile.
-template <typename T, int NS, bool KR, int ACC>
-__device__ __forceinline__ void gemm_tile(GemmSmem<T, NS>& sm, const float2* __restrict__ P,
-                                          const float2* __restrict__ Q, int64_t na, int64_t nb,
-                                          int64_t n_rows, int nch, const T* __restrict__ bt,
-                                          int64_t ncp, int64_t ncols, void* __restrict__ out,
-                                          bool out_f64, int64_t ldo, double scale, int64_t row0,
-                                          int64_t col0) {
-  using V8 = typename Vec8<T>::type;
-  auto& As = sm.As;
-  auto& Bs = sm.Bs;
+struct TileJob {
+  const uint32_t* A;
+  int64_t arows;
+  int nch;
+  const float2* S;  // 3-D: S table and the block's index a
+  int64_t na, a;
+  const float2* C;
+  int64_t ncp;
+  int64_t arow0, col0;
+  void* out;
+  bool out_f64;
+  int64_t ldo, out_row0, nvalid, ncols;
+  double scale;
+};
 
+template <typename T, int NS, bool KR, int ACC, int JN>
+__device__ __forceinline__ void gemm_tile(GemmSmem<T, NS, JN>& sm, const TileJob& jb) {
+  using V8 = typename Fmt<T>::v8;
+  using A2 = typename AccT<ACC>::type;
+  constexpr int GC = JN == 4 ? 2 : 1;  // B columns built per lane
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int wave = tid >> 6;
-  const int wr = wave >> 1, wc = wave & 1;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int hl = lane >> 5;  // k half of an MFMA fragment
+  const int lr = lane & 31;  // row / column within a 32 x 32 block
+  const int nch = jb.nch;
 
-  // generator assignment: thread -> (row gr, 8 modes at half gh of the chunk)
-  const int gr = tid >> 1, gh = tid & 1;
-  const int64_t grow = row0 + gr;
-  const bool grow_ok = grow < n_rows;
-  const int64_t ga = KR ? (grow_ok ? grow / nb : 0) : (grow_ok ? grow : 0);
-  const int64_t gb = KR ? (grow_ok ? grow - ga * nb : 0) : 0;
-  // float4 = two modes; chunk c of row a starts at float2 index (c n + a) 16
-  const float4* prow = reinterpret_cast<const float4*>(P + ga * 16 + 8 * gh);
-  const float4* qrow = KR ? reinterpret_cast<const float4*>(Q + gb * 16 + 8 * gh) : nullptr;
-  const int64_t pstep = na * 8, qstep = nb * 8;  // float4 per chunk
-  // B loader assignment: thread -> (col bc, k half bh); chunk stride ncp * 32
-  const int bc = tid >> 1, bh = tid & 1;
-  const T* brow = bt + (col0 + bc) * 32 + 16 * bh;
-  const int64_t bstep = ncp * 32, bsplit = static_cast<int64_t>(nch) * ncp * 32;
+  // A fragments: row arow0 + 64 wave + 32 i + lr, packed pairs 8 ks + 4 hl .. + 3
+  const uint32_t* abase = jb.A + (hl * jb.arows + jb.arow0 + 64 * wave + lr) * 4;
+  const int64_t a_ks = 2 * jb.arows * 4;
+  const int64_t a_chunk = 2 * a_ks;
+  const int64_t a_part = a_chunk * nch;
+  // B builder: this wave builds modes 4 wave .. + 3 (k = 8 wave .. + 7) of
+  // each chunk for columns lane + 64 g (JN = 4), lane (JN = 2) or lr (JN = 1)
+  const bool gen_on = JN > 1 || lane < 32;
+  const int gcol = JN == 1 ? lr : lane;
+  // mode pair 8 c + 2 wave + h of column col0 + gcol + 64 g: float4 index
+  // (8 c + 2 wave + h) ncp + col0 + gcol + 64 g
+  const float4* cbase = reinterpret_cast<const float4*>(jb.C) + 2 * wave * jb.ncp + jb.col0 + gcol;
+  const int64_t c_pair = jb.ncp;
+  const int64_t c_chunk = 8 * jb.ncp;
+  // S(a, m) of one chunk: lane l < 16 loads mode l with a vector load (a
+  // scalar load would make every LDS wait in the loop wait for it too); the
+  // wave takes its four modes with readlane
+  const float2* sbase = KR ? jb.S + jb.a * 16 + (lane & 15) : nullptr;
+  const int64_t s_chunk = jb.na * 16;
 
-  float4 pr[2][4], qr[2][4];
-  V8 br[2][NS][2];
-  auto gload = [&](const int set, int c) {
+  // C values (and, in 3-D, the wave's four S values) of one chunk, loaded a
+  // step before they are used
+  float4 cr[GC][2];
+  float2 sl;
+  auto gload = [&](int c) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      pr[set][i] = prow[c * pstep + i];
-      if constexpr (KR) qr[set][i] = qrow[c * qstep + i];
+    for (int g = 0; g < GC; ++g) {
+      const float4* p = cbase + c * c_chunk + 64 * g;
+      cr[g][0] = p[0];
+      cr[g][1] = p[c_pair];
     }
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      const V8* src = reinterpret_cast<const V8*>(brow + s * bsplit + c * bstep);
-      br[set][s][0] = src[0];
-      br[set][s][1] = src[1];
-    }
+    if constexpr (KR) sl = sbase[c * s_chunk];
   };
-  auto gstore = [&](const int set, const int buf) {
-    T ws[NS][16];
+  auto gstore = [&](const int buf) {
+    float2 sv[4];
+    if constexpr (KR) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      float wr0 = pr[set][i].x, wi0 = pr[set][i].y, wr1 = pr[set][i].z, wi1 = pr[set][i].w;
-      if constexpr (KR) {
-        const float4 q = qr[set][i];
-        const float r0 = wr0 * q.x - wi0 * q.y, i0 = wr0 * q.y + wi0 * q.x;
-        const float r1 = wr1 * q.z - wi1 * q.w, i1 = wr1 * q.w + wi1 * q.z;
-        wr0 = r0; wi0 = i0; wr1 = r1; wi1 = i1;
+      for (int q = 0; q < 4; ++q) {
+        sv[q].x = __builtin_bit_cast(float, __builtin_amdgcn_readlane(
+                                                __builtin_bit_cast(int, sl.x), 4 * wave + q));
+        sv[q].y = __builtin_bit_cast(float, __builtin_amdgcn_readlane(
+                                                __builtin_bit_cast(int, sl.y), 4 * wave + q));
       }
-      if (!grow_ok) wr0 = wi0 = wr1 = wi1 = 0.0f;
-      T t[NS];
-      split<T, NS>(wr0, t);
-#pragma unroll
-      for (int s = 0; s < NS; ++s) ws[s][4 * i + 0] = t[s];
-      split<T, NS>(wi0, t);
-#pragma unroll
-      for (int s = 0; s < NS; ++s) ws[s][4 * i + 1] = t[s];
-      split<T, NS>(wr1, t);
-#pragma unroll
-      for (int s = 0; s < NS; ++s) ws[s][4 * i + 2] = t[s];
-      split<T, NS>(wi1, t);
-#pragma unroll
-      for (int s = 0; s < NS; ++s) ws[s][4 * i + 3] = t[s];
     }
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      V8* dst = reinterpret_cast<V8*>(&As[buf][s][gr][16 * gh]);
-      V8 lo, hi;
+    for (int g = 0; g < GC; ++g) {
+      uint32_t w[NS][4];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) { lo[i] = ws[s][i]; hi[i] = ws[s][8 + i]; }
-      dst[0] = lo;
-      dst[1] = hi;
-      V8* bd = reinterpret_cast<V8*>(&Bs[buf][s][bc][16 * bh]);
-      bd[0] = br[set][s][0];
-      bd[1] = br[set][s][1];
+      for (int q = 0; q < 4; ++q) {
+        const float4 v = cr[g][q >> 1];
+        float re = (q & 1) ? v.z : v.x, im = (q & 1) ? v.w : v.y;
+        if constexpr (KR) {
+          const float r2 = sv[q].x * re - sv[q].y * im;
+          im = sv[q].x * im + sv[q].y * re;
+          re = r2;
+        }
+        uint32_t o[NS];
+        split_pair<T, NS>(f2v{im, re}, o);  // B pair (Im, Re)
+#pragma unroll
+        for (int s = 0; s < NS; ++s) w[s][q] = o[s];
+      }
+      if (gen_on) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+          *reinterpret_cast<u32x4*>(&sm.B[buf][s][gcol + 64 * g][8 * wave]) =
+              u32x4{w[s][0], w[s][1], w[s][2], w[s][3]};
+      }
     }
   };
 
-  using A2 = typename AccT<ACC>::type;
-  f32x16_t acc[2][2];
-  A2 acc2[2][2][ACC == ACC_F32 ? 1 : 16];
+  V8 af[2][2][NS][2];  // [chunk parity][ks][part][row block]
+  auto aload = [&](const int set, int c) {
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+          af[set][ks][s][i] = *reinterpret_cast<const V8*>(abase + s * a_part + c * a_chunk +
+                                                           ks * a_ks + 128 * i);
+  };
+  V8 bf[2][NS][JN];  // [ks][part][column block]
+  auto bread = [&](const int ks, const int buf) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+#pragma unroll
+      for (int j = 0; j < JN; ++j)
+        bf[ks][s][j] = *reinterpret_cast<const V8*>(&sm.B[buf][s][32 * j + lr][16 * ks + 8 * hl]);
+  };
+
+  f32x16_t acc[2][JN];
+  A2 acc2[2][JN][ACC == ACC_F32 ? 1 : 16];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < JN; ++j) {
 #pragma unroll
       for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
 #pragma unroll
       for (int v = 0; v < (ACC == ACC_F32 ? 1 : 16); ++v) acc2[i][j][v] = 0;
     }
 
-  auto mfma_chunk = [&](const int buf) {
+  auto mfma_ks = [&](const int set, const int ks) {
 #pragma unroll
-    for (int ks = 0; ks < 2; ++ks) {
-      V8 af[NS][2], bf[NS][2];
-      const int kk = ks * 16 + 8 * (lane >> 5);
+    for (int sa = 0; sa < NS; ++sa)
 #pragma unroll
-      for (int s = 0; s < NS; ++s)
+      for (int sb = 0; sb < NS - sa; ++sb)
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
-          af[s][i] = *reinterpret_cast<const V8*>(&As[buf][s][wr * 64 + i * 32 + (lane & 31)][kk]);
-          bf[s][i] = *reinterpret_cast<const V8*>(&Bs[buf][s][wc * 64 + i * 32 + (lane & 31)][kk]);
-        }
+        for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int sa = 0; sa < NS; ++sa)
-#pragma unroll
-        for (int sb = 0; sb < NS - sa; ++sb)
-#pragma unroll
-          for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j) acc[i][j] = mfma(af[sa][i], bf[sb][j], acc[i][j]);
-    }
-  };
-  // Interleave hints for one step: the loads first, the first k-step's LDS
-  // reads, then every MFMA paired with LDS reads, generator VALU work and
-  // LDS writes of the next chunk.
-  constexpr int kMfma = 2 * 4 * (NS * (NS + 1) / 2);
-  auto sched = [&]() {
-    __builtin_amdgcn_sched_group_barrier(0x020, 4 * (KR ? 2 : 1) + 2 * NS, 0);  // VMEM read
-    __builtin_amdgcn_sched_group_barrier(0x100, 4 * NS, 0);                    // DS read
-#pragma unroll
-    for (int i = 0; i < kMfma; ++i) {
-      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
-      if (i < 4 * NS) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-      __builtin_amdgcn_sched_group_barrier(0x002, 6, 0);  // VALU
-      if (i >= kMfma - 4 * NS) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // DS write
-    }
+          for (int j = 0; j < JN; ++j)
+            acc[i][j] = mfma(af[set][ks][sa][i], bf[ks][sb][j], acc[i][j]);
   };
 
-  gload(0, 0);
-  gload(1, nch > 1 ? 1 : 0);
-  gstore(0, 0);
-  __syncthreads();
-  for (int c0 = 0; c0 < nch; c0 += kFlush) {
-    const int c1 = min(c0 + kFlush, nch);
-    for (int c = c0; c < c1; c += 2) {
-      gload(0, min(c + 2, nch - 1));
-      mfma_chunk(0);
-      gstore(1, 1);
-      sched();
-      __syncthreads();
-      gload(1, min(c + 3, nch - 1));
-      mfma_chunk(1);
-      gstore(0, 0);
-      sched();
-      __syncthreads();
+  // One chunk c (register set and LDS buffer c % 2).  The barrier sits
+  // between building chunk c + 1's B tile and reading its first fragments;
+  // chunk c's fragments were all read before it, so the next build into
+  // this buffer (after the following barrier) cannot overtake a read.
+  // A wave issues in order, so the memory and ALU work of each half is
+  // spread between its MFMAs (scheduling groups): a burst of loads would
+  // stall the MFMAs behind it.
+  constexpr int kM = 2 * JN * NS * (NS + 1) / 2;  // MFMAs per half
+  constexpr int kA = 4 * NS;                       // A fragment loads per chunk
+  constexpr int kB = NS * JN;                      // B fragment reads per half
+  constexpr int kW = NS * GC;                      // B tile writes per chunk
+  constexpr int kG = 2 * GC + (KR ? 1 : 0);        // C (and S) loads per chunk
+  auto step = [&](int c, const int set) {
+    const int cn = c + 1 < nch ? c + 1 : c;
+    aload(set ^ 1, cn);
+    bread(1, set);
+    mfma_ks(set, 0);
+    // at c = nch - 1 this rebuilds the last chunk into the buffer of chunk
+    // nch - 2, which no wave reads again
+    gstore(set ^ 1);
+#pragma unroll
+    for (int k = 0; k < kM; ++k) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+      if (k < kA) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);            // VMEM read
+      else if (k < kA + kB) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // DS read
+      __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);                        // VALU
+      if (k >= kM - kW) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);      // DS write
     }
+    __builtin_amdgcn_sched_barrier(0);
+    __syncthreads();
+    gload(c + 2 < nch ? c + 2 : cn);
+    bread(0, set ^ 1);
+    mfma_ks(set, 1);
+#pragma unroll
+    for (int k = 0; k < kM; ++k) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+      if (k < kG) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);            // VMEM read
+      else if (k < kG + kB) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // DS read
+      __builtin_amdgcn_sched_group_barrier(0x002, 1, 0);                        // VALU
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  };
+
+  gload(0);
+  gstore(0);
+  gload(nch > 1 ? 1 : 0);
+  aload(0, 0);
+  __syncthreads();
+  bread(0, 0);
+  // nch is even (modes padded to 32): the unrolled pair fixes set and buffer
+  for (int c = 0; c < nch; c += 2) {
+    step(c, 0);
+    step(c + 1, 1);
     if constexpr (ACC != ACC_F32) {
+      if ((c + 2) % kFlush == 0 || c + 2 == nch) {
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+          for (int j = 0; j < JN; ++j)
 #pragma unroll
-          for (int v = 0; v < 16; ++v) {
-            acc2[i][j][v] += static_cast<A2>(acc[i][j][v]);
-            acc[i][j][v] = 0.0f;
-          }
+            for (int v = 0; v < 16; ++v) {
+              acc2[i][j][v] += static_cast<A2>(acc[i][j][v]);
+              acc[i][j][v] = 0.0f;
+            }
+      }
     }
   }
+  __syncthreads();  // the caller may reuse the LDS tile for its next tile
 
   // ---- epilogue: D[8 (v/4) + 4 (lane/32) + v%4][lane%32] of each 32 x 32 block
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int64_t col = col0 + wc * 64 + j * 32 + (lane & 31);
-      if (col >= ncols) continue;
+    for (int j = 0; j < JN; ++j) {
+      const int64_t col = jb.col0 + 32 * j + lr;
+      if (col >= jb.ncols) continue;
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
-        const int64_t row = row0 + wr * 64 + i * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        const int r = 64 * wave + 32 * i + 8 * (v >> 2) + 4 * hl + (v & 3);
+        if (r >= jb.nvalid) continue;
         double val;
         if constexpr (ACC == ACC_F32) val = static_cast<double>(acc[i][j][v]);
         else val = static_cast<double>(acc2[i][j][v]);
-        if (row < n_rows) {
-          if (out_f64) static_cast<double*>(out)[row * ldo + col] = val * scale;
-          else static_cast<float*>(out)[row * ldo + col] = static_cast<float>(val * scale);
-        }
+        const int64_t o = (jb.out_row0 + r) * jb.ldo + col;
+        if (jb.out_f64) static_cast<double*>(jb.out)[o] = val * jb.scale;
+        else static_cast<float*>(jb.out)[o] = static_cast<float>(val * jb.scale);
       }
     }
 }
 
+// Tile job of row tile rt, column tile ct of one contraction.
+template <bool KR, int JN>
+__device__ __forceinline__ TileJob make_job(const uint32_t* A, int64_t arows, int nch,
+                                            const float2* S, int64_t na, const float2* C,
+                                            int64_t ncp, int64_t nrow, int64_t nb_out,
+                                            void* out, bool out_f64, int64_t ldo,
+                                            int64_t ncols, double scale, int64_t rt,
+                                            int64_t ct) {
+  const int64_t nbt = arows / kBlockRows;
+  const int64_t a = KR ? rt / nbt : 0;
+  const int64_t arow0 = (rt - a * nbt) * kBlockRows;
+  return TileJob{A,     arows, nch,  S,   na,      a,           C,
+                 ncp,   arow0, ct * 32 * JN, out, out_f64, ldo,
+                 KR ? a * nb_out + arow0 : arow0, nrow - arow0, ncols, scale};
+}
+
 template <typename T, int NS, bool KR, int ACC>
-__global__ void __launch_bounds__(256)
-k_factored_gemm(const float2* __restrict__ P, const float2* __restrict__ Q, int64_t na,
-                int64_t nb, int64_t n_rows, int nch, const T* __restrict__ bt, int64_t ncp,
-                int64_t ncols, void* __restrict__ out, bool out_f64, int64_t ldo, double scale) {
-  __shared__ __attribute__((aligned(16))) GemmSmem<T, NS> sm;
-  gemm_tile<T, NS, KR, ACC>(sm, P, Q, na, nb, n_rows, nch, bt, ncp, ncols, out, out_f64, ldo,
-                            scale, static_cast<int64_t>(blockIdx.x) * kTM,
-                            static_cast<int64_t>(blockIdx.y) * kTN);
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+k_factored_gemm(GemmArgs g) {
+  constexpr int JN = cols_blocks<NS, ACC>();
+  __shared__ __attribute__((aligned(16))) GemmSmem<T, NS, JN> sm;
+  const TileJob jb = make_job<KR, JN>(g.A, g.arows, g.nch, g.S, g.na, g.C, g.ncp, g.nrow,
+                                      g.nb_out, g.out, g.out_f64, g.ldo, g.ncols, g.scale,
+                                      blockIdx.x, blockIdx.y);
+  gemm_tile<T, NS, KR, ACC, JN>(sm, jb);
 }
 
 // Tables of all factored realisations: one block per task (realisation,
-// table, group of 16 rows); the block walks every 16-mode chunk of its rows.
-// task_start has 3 nfact + 1 entries: tasks of fact[i] / table w start at
+// role, group of 64 rows); the block walks every chunk of its rows.
+// task_start has 3 nfact + 1 entries: tasks of fact[i] / role w start at
 // task_start[3 i + w].
 template <typename T, int NS>
 __global__ void __launch_bounds__(256)
@@ -406,27 +490,29 @@ k_ens_tables(const EnsReal* __restrict__ d, const int* __restrict__ fact,
   const int64_t task = blockIdx.x;
   const int k = find_item(task_start, 3 * nfact, task);
   const EnsReal& r = d[fact[k / 3]];
-  const int which = k % 3;
-  const int64_t a = (task - task_start[k]) * 16 + (threadIdx.x >> 4);
+  const int role = k % 3;
+  const int64_t row = (task - task_start[k]) * 64 + (threadIdx.x & 63);
   const int nch = r.mp / 16;
+  const double* xa = r.dim == 3 ? r.ax1 : r.ax0;
+  const double* xc = r.dim == 3 ? r.ax2 : r.ax1;
   for (int c = 0; c < nch; ++c)
-    table_entry<T, NS>(r.table, static_cast<int>(r.M), r.dim, r.ax0, r.ax1,
-                       r.dim == 3 ? r.ax2 : r.ax1, r.na, r.nb, r.ncols, r.ncp, r.P, r.Q,
-                       static_cast<T*>(r.bt), which, a, c, nch, threadIdx.x & 15);
+    table_quad<T, NS>(r.table, static_cast<int>(r.M), r.dim, nch, r.ax0, xa, xc, r.na, r.nrow,
+                      r.arows, r.ncols, r.ncp, r.S, r.A, r.C, role, row, c, threadIdx.x >> 6);
 }
 
 // Persistent GEMM over the tiles of all factored realisations.  The tiles
 // are split into kXcdGroups groups of whole realisations (balanced on the
 // host); block b starts on group b % kXcdGroups -- workgroups are dispatched
-// to the XCDs round-robin, so a realisation's table bands stay in one XCD's
-// L2 -- and, once its group is exhausted, helps the others in turn.  Every
-// group counter only grows, so every block leaves every group and exits.
+// to the XCDs round-robin, so a realisation's tables stay in one XCD's L2 --
+// and, once its group is exhausted, helps the others in turn.  Every group
+// counter only grows, so every block leaves every group and exits.
 template <typename T, int NS, bool KR, int ACC>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_ens_gemm(const EnsReal* __restrict__ d, const int* __restrict__ fact,
            const int64_t* __restrict__ tile_start, int nfact,
            const int64_t* __restrict__ group_start, int* __restrict__ counters, bool out_f64) {
-  __shared__ __attribute__((aligned(16))) GemmSmem<T, NS> sm;
+  constexpr int JN = cols_blocks<NS, ACC>();
+  __shared__ __attribute__((aligned(16))) GemmSmem<T, NS, JN> sm;
   __shared__ int64_t tile_s;
   const int home = blockIdx.x % kXcdGroups;
   for (int k = 0; k < kXcdGroups; ++k) {
@@ -442,9 +528,10 @@ k_ens_gemm(const EnsReal* __restrict__ d, const int* __restrict__ fact,
       const EnsReal& r = d[fact[i]];
       const int64_t local = t - tile_start[i];
       const int64_t rt = local / r.col_tiles, ct = local - rt * r.col_tiles;
-      gemm_tile<T, NS, KR, ACC>(sm, r.P, r.Q, r.na, r.nb, r.rows, r.mp / 16,
-                                static_cast<const T*>(r.bt), r.ncp, r.ncols, r.out, out_f64,
-                                r.ncols, r.scale, rt * kTM, ct * kTN);
+      const TileJob jb = make_job<KR, JN>(r.A, r.arows, r.mp / 16, r.S, r.na, r.C, r.ncp, r.nrow,
+                                          r.nb_out, r.out, out_f64, r.ncols, r.ncols, r.scale,
+                                          rt, ct);
+      gemm_tile<T, NS, KR, ACC, JN>(sm, jb);
     }
   }
 }
@@ -453,19 +540,15 @@ k_ens_gemm(const EnsReal* __restrict__ d, const int* __restrict__ fact,
 
 template <typename T, int NS>
 hipError_t tables_launch(const TableArgs& a, dim3 grid, hipStream_t s) {
-  gemm::k_tables<T, NS><<<grid, 256, 0, s>>>(a.modes, a.M, a.dim, a.x0, a.x1, a.xl, a.na, a.nb,
-                                              a.ncols, a.ncp, a.P, a.Q, static_cast<T*>(a.bt));
+  gemm::k_tables<T, NS><<<grid, 256, 0, s>>>(a.modes, a.M, a.dim, a.nch, a.xs, a.xa, a.xc, a.na,
+                                              a.nrow, a.arows, a.ncols, a.ncp, a.S, a.A, a.C);
   return hipGetLastError();
 }
 
 template <typename T, int NS>
 hipError_t factored_gemm_launch(const GemmArgs& a, dim3 grid, hipStream_t s) {
   using namespace gemm;
-  const T* bt = static_cast<const T*>(a.bt);
-#define GSC_G(KR, ACC)                                                                          \
-  k_factored_gemm<T, NS, KR, ACC><<<grid, 256, 0, s>>>(a.P, a.Q, a.na, a.nb, a.rows, a.nch, bt, \
-                                                       a.ncp, a.ncols, a.out, a.out_f64, a.ldo,  \
-                                                       a.scale)
+#define GSC_G(KR, ACC) k_factored_gemm<T, NS, KR, ACC><<<grid, 256, 0, s>>>(a)
   if (a.kr) {
     if (a.accum == ACC_F32X2) GSC_G(true, ACC_F32X2);
     else if (a.accum == ACC_F32) GSC_G(true, ACC_F32);

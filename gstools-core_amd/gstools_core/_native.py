@@ -144,7 +144,7 @@ DIRECT_VARIANTS = {"mfma": 0, "valu": 1}
 
 
 def make_opts(out_dtype=GSC_F64, scale=1.0, device=-1, num_threads=0, path=GSC_PATH_AUTO,
-              precision=0, accum=0, direct_variant=1) -> GscOpts:
+              precision=0, accum=1, direct_variant=1) -> GscOpts:
     o = GscOpts()
     lib().gsc_opts_init(ctypes.byref(o))
     o.out_dtype = out_dtype

@@ -30,7 +30,7 @@ class Ensemble:
     """
 
     def __init__(self, realisations: Sequence, *, precision: str = "auto",
-                 out_dtype=np.float32, accum: str = "f64", paths: Optional[Sequence[str]] = None,
+                 out_dtype=np.float32, accum: str = "f32x2", paths: Optional[Sequence[str]] = None,
                  scales: Optional[Sequence[float]] = None, tune: bool = False, tuner=None,
                  device: int = -1, blocks_per_cu: int = 0):
         if len(realisations) == 0:

@@ -30,7 +30,7 @@ def _axes_args(axes):
     return axes
 
 
-def _opts(path, precision, out_code, scale, device, nt, accum="f64", variant="valu"):
+def _opts(path, precision, out_code, scale, device, nt, accum="f32x2", variant="valu"):
     if path not in PATHS:
         raise ValueError(f"path must be one of {sorted(PATHS)}")
     if precision not in _native.PRECISIONS:
@@ -56,7 +56,7 @@ def grid_path(cov_samples, axes, path="auto", precision="auto") -> str:
 
 
 def summate_grid(cov_samples, z1, z2, axes, num_threads=None, *, path="auto", precision="auto",
-                 out_dtype=np.float64, scale=1.0, device=-1, accum="f64", variant="valu"):
+                 out_dtype=np.float64, scale=1.0, device=-1, accum="f32x2", variant="valu"):
     """Field on the grid axes[0] x ... x axes[d-1] (d in 1..3), C order, shape (len(ax) ...).
 
     Equal to ``summate(cov_samples, z1, z2, pos)`` with pos the expanded grid
@@ -100,7 +100,7 @@ def grid_workspace_bytes(dim, m, shape, path="auto", precision="auto") -> int:
 
 def summate_grid_dev(dim, m, cov, z1, z2, axes_ptrs, shape, out, work, work_bytes, *,
                      path="auto", precision="auto", out_f32=False, scale=1.0, stream=0,
-                     accum="f64", variant="valu"):
+                     accum="f32x2", variant="valu"):
     """Device form: every pointer is a device address (int); see gsc_summate_grid_dev."""
     o = _opts(path, precision, _native.GSC_F32 if out_f32 else _native.GSC_F64, scale, -1, 0,
               accum, variant)

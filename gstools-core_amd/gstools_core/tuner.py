@@ -52,7 +52,7 @@ class Tuner:
         return f"{self.device_key}|{dim}|{'x'.join(map(str, cls))}|{mb}|{precision}|{accum}"
 
     def measure(self, dim: int, shape: Sequence[int], m: int, precision: str = "auto",
-                accum: str = "f64") -> dict:
+                accum: str = "f32x2") -> dict:
         """Times both paths on the shape class of (shape, m); returns ms per path."""
         cls, mb = shape_class(shape, m)
         rng = np.random.default_rng(12345)
@@ -88,7 +88,7 @@ class Tuner:
         return res
 
     def choose(self, dim: int, shape: Sequence[int], m: int, precision: str = "auto",
-               accum: str = "f64") -> str:
+               accum: str = "f32x2") -> str:
         """'direct' or 'factored' for this grid, measuring its class on first use."""
         if dim < 2 or min(shape) == 0:
             return "direct"

@@ -67,10 +67,12 @@ typedef enum gsc_precision {
 } gsc_precision;
 
 /* Accumulation of the factored path's FP32 MFMA partial sums, which are
- * flushed every 256 modes into a second accumulator set. */
+ * flushed every 256 modes into a second accumulator set.  The second set's
+ * registers limit the GEMM tile: 128 columns per block for F32X2 and F32
+ * (64 for bf16x6), 32 for F64. */
 typedef enum gsc_accum {
-  GSC_ACCUM_F64 = 0,    /* second set in FP64 (default, most accurate) */
-  GSC_ACCUM_F32X2 = 1,  /* second set in FP32 (fewer registers) */
+  GSC_ACCUM_F64 = 0,    /* second set in FP64 (narrow tiles, slowest) */
+  GSC_ACCUM_F32X2 = 1,  /* second set in FP32 (default; ~1e-6 sigma at M = 4000) */
   GSC_ACCUM_F32 = 2     /* one FP32 set over all modes (error grows with M) */
 } gsc_accum;
 
@@ -90,7 +92,7 @@ typedef struct gsc_opts {
   int32_t num_threads;     /* host threads for staging strided arrays, 0 = all */
   int32_t accum;           /* gsc_accum, factored path only */
   int32_t direct_variant;  /* gsc_direct_variant, direct kernels only */
-  int32_t blocks_per_cu;   /* persistent factored-GEMM blocks per CU in ensembles, 0 = 2 */
+  int32_t blocks_per_cu;   /* persistent factored-GEMM blocks per CU in ensembles, 0 = 1 */
   double scale;            /* multiplies every output value (1.0 = reference) */
 } gsc_opts;
 

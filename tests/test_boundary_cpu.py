@@ -43,7 +43,7 @@ def test_version_and_defaults():
     o = _native.GscOpts()
     lib.gsc_opts_init(ctypes.byref(o))
     assert (o.out_dtype, o.path, o.precision, o.device, o.scale) == (0, 0, 0, -1, 1.0)
-    assert (o.accum, o.direct_variant, o.blocks_per_cu) == (0, 1, 0)
+    assert (o.accum, o.direct_variant, o.blocks_per_cu) == (1, 1, 0)
     assert gstools_core.__version__ == "1.1.0"
 
 
