@@ -239,7 +239,7 @@ def main() -> int:
     z2 = torch.from_numpy(md.z2).to(dev)
     pos = torch.from_numpy(np.ascontiguousarray(w.pos)).to(dev)
     out = torch.empty(n, dtype=torch.float64, device=dev)
-    nb = device.workspace_bytes(d, m)
+    nb = max(device.workspace_bytes(d, m, n, v) for v in ("local", "valu", "mfma"))
     work = torch.empty(nb, dtype=torch.uint8, device=dev)
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream
@@ -268,8 +268,9 @@ def main() -> int:
     value = n * m / (ms_per_step * 1e-3)
     log(f"ms/step {ms_per_step:.3f} (min {min(per):.3f}, wall {wall * 1e3 / args.steps:.3f})")
 
-    # roofline of the dominant kernel: one hardware sine per point-mode
-    # (amplitude-phase form), i.e. N*M transcendentals per launch
+    # roofline: one hardware sine per point-mode (amplitude-phase form), i.e.
+    # N*M transcendentals per summation; the measured time is the whole
+    # device call (cell sort, mode split and k_direct_local, all on `stream`)
     avg_launch_s = ms_per_step * 1e-3
     achieved = n * m / avg_launch_s / 1e12
     roofline = {
@@ -279,9 +280,9 @@ def main() -> int:
         "unit": "Tsin/s",
         "frac": round(achieved / (TRANS_PEAK / 1e12), 4),
         "traffic": None,
-        "kernel": "k_direct_scalar<3,8,double>",
-        "work_per_launch": f"{n}*{m} point-modes, 1 v_sin_f32 + 3 v_fma_f64 + fract + cvt + "
-                           "1 v_fma_f32 each",
+        "kernel": "gsc_summate_dev: cell sort + k_direct_local<3,double>",
+        "work_per_launch": f"{n}*{m} point-modes, 1 v_sin_f32 + 1 v_fma_f32 each on the VALU; "
+                           "phases on v_mfma_f32_16x16x4 (256 point-modes per instruction)",
     }
 
     grid_res = None
@@ -294,16 +295,17 @@ def main() -> int:
         log("ensemble leg (512 realisations)")
         ens_res = ensemble_leg(max(3, args.steps // 2), 1)
 
-    # the matrix-core-phase variant of the same kernel, for comparison
-    def step_mfma():
-        device.summate_dev(d, m, n, cov.data_ptr(), z1.data_ptr(), z2.data_ptr(), pos.data_ptr(),
-                           n, out.data_ptr(), work.data_ptr(), nb, stream=sh, variant="mfma")
+    # the FP64-phase variants of the direct kernel, for comparison
+    direct_variants = {"local_phase": {"ms_per_step": ms_per_step, "point_modes_per_s": value}}
+    for vname, key in (("valu", "valu_phase"), ("mfma", "mfma_phase")):
+        def step_v(vname=vname):
+            device.summate_dev(d, m, n, cov.data_ptr(), z1.data_ptr(), z2.data_ptr(),
+                               pos.data_ptr(), n, out.data_ptr(), work.data_ptr(), nb, stream=sh,
+                               variant=vname)
 
-    _, tot_mfma = time_steps(step_mfma, max(3, args.steps // 2), 1, stream)
-    mfma_ms = tot_mfma / max(3, args.steps // 2)
-    direct_variants = {"valu_phase": {"ms_per_step": ms_per_step, "point_modes_per_s": value},
-                       "mfma_phase": {"ms_per_step": mfma_ms,
-                                      "point_modes_per_s": n * m / (mfma_ms * 1e-3)}}
+        _, tot_v = time_steps(step_v, max(3, args.steps // 2), 1, stream)
+        v_ms = tot_v / max(3, args.steps // 2)
+        direct_variants[key] = {"ms_per_step": v_ms, "point_modes_per_s": n * m / (v_ms * 1e-3)}
 
     cpu = None
     if args.cpu_sample > 0:
@@ -321,7 +323,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64-phase/f32-sin",
+        "dtype": "f64-cell-phase/f32-local-phase/f32-sin",
         "data": "synthetic",
         "config": {"workload": f"3-D unstructured summate, N={n} uniform in [0,500]^3, "
                                f"M={m} Gaussian modes (l=20), seed 19031977",

@@ -31,7 +31,8 @@ FLAGS = [
 # Per-file flags.  direct.hip: MFMA results go to arch VGPRs (its FP64 phase
 # tiles feed VALU work directly; AGPR results would cost two v_accvgpr_read per
 # phase).  The factored GEMM needs more than 256 registers and keeps AGPRs.
-FILE_FLAGS = {"direct.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
+FILE_FLAGS = {"direct.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
+              "direct_local.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
 
 
 def _compile(src: str) -> str:

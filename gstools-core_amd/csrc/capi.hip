@@ -106,7 +106,7 @@ int validate_common(int dim, int64_t M, int64_t N, const double* cov, const doub
 int validate_opts(const gsc_opts* o) {
   if (o->out_dtype != GSC_F64 && o->out_dtype != GSC_F32)
     return fail(GSC_ERR_INVALID, "out_dtype must be GSC_F64 or GSC_F32");
-  if (o->direct_variant != GSC_DIRECT_MFMA && o->direct_variant != GSC_DIRECT_VALU)
+  if (o->direct_variant < GSC_DIRECT_MFMA || o->direct_variant > GSC_DIRECT_LOCAL)
     return fail(GSC_ERR_INVALID, "unknown direct_variant %d", o->direct_variant);
   if (o->blocks_per_cu < 0 || o->blocks_per_cu > 64)
     return fail(GSC_ERR_INVALID, "blocks_per_cu must be in 0..64");
@@ -150,6 +150,39 @@ size_t align256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
 // Shared body of the three host-pointer summators.
 //   kind: gsc::KIND_*; for KIND_INCOMPR `out` is (dim, N) with strides
 //   (out_sd, out_sp), otherwise N values with stride out_s = out_sp.
+// The local-phase (binned) direct kernel serves the scalar and Fourier
+// summators in 2-D and 3-D; incompressible fields and other dimensions use the
+// FP64-phase VALU kernel for that variant.
+bool use_binned(int kind, int dim, int64_t N, const gsc_opts* o) {
+  return o->direct_variant == GSC_DIRECT_LOCAL && kind != gsc::KIND_INCOMPR &&
+         gsc::binned_supported(dim, N);
+}
+
+size_t direct_work_bytes(int kind, int dim, int64_t M, int64_t N, const gsc_opts* o) {
+  const size_t table = align256(gsc::mode_table_bytes(dim, M));
+  return use_binned(kind, dim, N, o) ? table + gsc::binned_work_bytes(dim, M, N) : table;
+}
+
+// Launches the direct kernel of `kind` on device buffers; `work` holds the
+// mode table (written by launch_prep) followed by the binned kernel's area.
+hipError_t launch_direct(int kind, int dim, int64_t M, int64_t N, void* work, const double* pos,
+                         int64_t pos_ld, void* out, int64_t out_ld, const gsc_opts* o,
+                         hipStream_t s) {
+  if (kind == gsc::KIND_INCOMPR) {
+    const int variant = o->direct_variant == GSC_DIRECT_MFMA ? GSC_DIRECT_MFMA : GSC_DIRECT_VALU;
+    return gsc::launch_direct_incompr(dim, M, N, work, pos, pos_ld, out, out_ld, o->out_dtype,
+                                      o->scale, variant, s);
+  }
+  if (use_binned(kind, dim, N, o)) {
+    void* bin = static_cast<char*>(work) + align256(gsc::mode_table_bytes(dim, M));
+    return gsc::launch_direct_binned(dim, M, N, work, pos, pos_ld, out, o->out_dtype, o->scale,
+                                     bin, s);
+  }
+  const int variant = o->direct_variant == GSC_DIRECT_MFMA ? GSC_DIRECT_MFMA : GSC_DIRECT_VALU;
+  return gsc::launch_direct_scalar(dim, M, N, work, pos, pos_ld, out, o->out_dtype, o->scale,
+                                   variant, s);
+}
+
 int host_summate(int kind, int dim, int64_t M, int64_t N, const double* sf, int64_t sf_s,
                  const double* cov, int64_t cov_sd, int64_t cov_sm, const double* z1,
                  int64_t z1_s, const double* z2, int64_t z2_s, const double* pos, int64_t pos_sd,
@@ -195,7 +228,7 @@ int host_summate(int kind, int dim, int64_t M, int64_t N, const double* sf, int6
     if ((rc = check_arch(dev))) return rc;
     if (!c.stream) GSC_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
 
-    const size_t b_table = gsc::mode_table_bytes(dim, M);
+    const size_t b_table = align256(direct_work_bytes(kind, dim, M, N, o));
     const size_t b_modes = align256(sizeof(double) * modes.size());
     const size_t b_pos = align256(sizeof(double) * dim * N);
     const size_t b_out = align256(elt * ncomp * N);
@@ -221,12 +254,7 @@ int host_summate(int kind, int dim, int64_t M, int64_t N, const double* sf, int6
     GSC_HIP(gsc::launch_prep(kind, dim, M, d_modes, d_modes + dim * M, d_modes + (dim + 1) * M,
                              kind == gsc::KIND_FOURIER ? d_modes + (dim + 2) * M : nullptr,
                              d_table, c.stream));
-    if (kind == gsc::KIND_INCOMPR)
-      GSC_HIP(gsc::launch_direct_incompr(dim, M, N, d_table, d_pos, N, d_out, N, o->out_dtype,
-                                         o->scale, o->direct_variant, c.stream));
-    else
-      GSC_HIP(gsc::launch_direct_scalar(dim, M, N, d_table, d_pos, N, d_out, o->out_dtype,
-                                        o->scale, o->direct_variant, c.stream));
+    GSC_HIP(launch_direct(kind, dim, M, N, d_table, d_pos, N, d_out, N, o, c.stream));
 
     std::vector<char> out_tmp;
     void* host_dst = out;
@@ -272,19 +300,14 @@ int dev_common(int kind, int dim, int64_t M, int64_t N, const double* sf, const 
                               (long long)N);
   if (kind == gsc::KIND_INCOMPR && out_ld < N)
     return fail(GSC_ERR_INVALID, "out_ld (%lld) < N (%lld)", (long long)out_ld, (long long)N);
-  const size_t need = gsc::mode_table_bytes(dim, M);
+  const size_t need = direct_work_bytes(kind, dim, M, N, o);
   if (!work || work_bytes < need)
     return fail(GSC_ERR_WORKSPACE, "workspace of %zu bytes is below the %zu required", work_bytes,
                 need);
   if (N == 0) return GSC_OK;
   hipStream_t s = static_cast<hipStream_t>(stream);
   GSC_HIP(gsc::launch_prep(kind, dim, M, cov, z1, z2, sf, work, s));
-  if (kind == gsc::KIND_INCOMPR)
-    GSC_HIP(gsc::launch_direct_incompr(dim, M, N, work, pos, pos_ld, out, out_ld, o->out_dtype,
-                                       o->scale, o->direct_variant, s));
-  else
-    GSC_HIP(gsc::launch_direct_scalar(dim, M, N, work, pos, pos_ld, out, o->out_dtype, o->scale,
-                                      o->direct_variant, s));
+  GSC_HIP(launch_direct(kind, dim, M, N, work, pos, pos_ld, out, out_ld, o, s));
   return GSC_OK;
 }
 
@@ -359,11 +382,11 @@ void gsc_opts_init(gsc_opts* o) {
   o->device = -1;
   o->num_threads = 0;
   o->accum = GSC_ACCUM_F32X2;
-  o->direct_variant = GSC_DIRECT_VALU;
+  o->direct_variant = GSC_DIRECT_LOCAL;
   o->scale = 1.0;
 }
 
-const char* gsc_version(void) { return "1.1.0-mi355x.1"; }
+const char* gsc_version(void) { return "1.1.0-mi355x.2"; }
 
 const char* gsc_last_error(void) { return g_last_error.c_str(); }
 
@@ -373,9 +396,12 @@ int gsc_device_count(void) {
   return n;
 }
 
-size_t gsc_workspace_bytes(int dim, int64_t M) {
-  if (dim < 1 || M < 0) return 0;
-  return gsc::mode_table_bytes(dim, M);
+size_t gsc_workspace_bytes(int dim, int64_t M, int64_t N, const gsc_opts* opts) {
+  if (dim < 1 || M < 0 || N < 0) return 0;
+  gsc_opts tmp;
+  const gsc_opts* o = opts_or_default(opts, &tmp);
+  // the scalar and Fourier forms need the same space; incompr never more
+  return direct_work_bytes(gsc::KIND_SCALAR, dim, M, N, o);
 }
 
 int gsc_summate(int dim, int64_t M, int64_t N, const double* cov, int64_t cov_sd, int64_t cov_sm,

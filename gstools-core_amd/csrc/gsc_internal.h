@@ -49,4 +49,13 @@ hipError_t launch_direct_incompr(int dim, int64_t M, int64_t N, const void* tabl
                                  const double* pos, int64_t pos_ld, void* out, int64_t out_ld,
                                  int out_dtype, double scale, int variant, hipStream_t stream);
 
+// Local-phase ("binned") direct kernel, dim 2 and 3 (direct_local.hip): a
+// spatial counting sort of the points, then phases kap . (x - X_cell) + r0 on
+// the FP32 matrix cores.  bin_work: binned_work_bytes(dim, M, N) bytes.
+bool binned_supported(int dim, int64_t N);
+size_t binned_work_bytes(int dim, int64_t M, int64_t N);
+hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table,
+                                const double* pos, int64_t pos_ld, void* out, int out_dtype,
+                                double scale, void* bin_work, hipStream_t stream);
+
 }  // namespace gsc

@@ -15,7 +15,9 @@ no CPU fallback.  Deliberate differences from the reference:
   * the GIL is released while the device works (ctypes drops it);
   * keyword-only extensions: out_dtype (np.float64 / np.float32), scale
     (multiplies the result, e.g. sqrt(var / M)), device (HIP ordinal),
-    variant ("valu": FP64 phases on the vector ALUs, the default;
+    variant ("local": points sorted into spatial cells, FP64 phase at each
+    cell centre and FP32 matrix-core phases inside the cell, the default
+    for summate in 2-D and 3-D; "valu": FP64 phases on the vector ALUs;
     "mfma": FP64 phases in matrix-core tiles, d <= 3).
 num_threads is accepted with the reference's meaning (worker threads); here
 it bounds the host threads that stage strided inputs.
@@ -95,7 +97,7 @@ def _check_modes(cov, z1, z2, pos):
 
 
 def summate(cov_samples, z1, z2, pos, num_threads=None, *, out_dtype=np.float64, scale=1.0,
-            device=-1, variant="valu"):
+            device=-1, variant="local"):
     """Scalar randomization-method field (field.rs:37-65, lib.rs:33-48).
 
     u(x_j) = sum_m z1[m] cos(k_m . x_j) + z2[m] sin(k_m . x_j), k_m = cov_samples[:, m].
@@ -122,7 +124,7 @@ def summate(cov_samples, z1, z2, pos, num_threads=None, *, out_dtype=np.float64,
 
 
 def summate_incompr(cov_samples, z1, z2, pos, num_threads=None, *, out_dtype=np.float64,
-                    scale=1.0, device=-1, variant="valu"):
+                    scale=1.0, device=-1, variant="local"):
     """Incompressible vector field (field.rs:97-182, lib.rs:50-65); d must be 2 or 3.
 
     Returns a (d, N) array in Fortran order, like the reference (field.rs:166-174).
@@ -152,7 +154,7 @@ def summate_incompr(cov_samples, z1, z2, pos, num_threads=None, *, out_dtype=np.
 
 
 def summate_fourier(spectrum_factor, modes, z1, z2, pos, num_threads=None, *,
-                    out_dtype=np.float64, scale=1.0, device=-1, variant="valu"):
+                    out_dtype=np.float64, scale=1.0, device=-1, variant="local"):
     """Fourier-method field (field.rs:219-249, lib.rs:67-84).
 
     u(x_j) = sum_m sf[m] (z1[m] cos(k_m . x_j) + z2[m] sin(k_m . x_j)), k_m = modes[:, m].

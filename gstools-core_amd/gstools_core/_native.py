@@ -60,7 +60,7 @@ SIGNATURES = {
     "gsc_version": (ctypes.c_char_p, []),
     "gsc_last_error": (ctypes.c_char_p, []),
     "gsc_device_count": (_int, []),
-    "gsc_workspace_bytes": (_sz, [_int, _i64]),
+    "gsc_workspace_bytes": (_sz, [_int, _i64, _i64, _optp]),
     "gsc_summate": (_int, [_int, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64,
                            _i64, _vp, _i64, _optp]),
     "gsc_summate_incompr": (_int, [_int, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp,
@@ -140,11 +140,11 @@ def lib():
 
 
 ACCUM = {"f64": 0, "f32x2": 1, "f32": 2}
-DIRECT_VARIANTS = {"mfma": 0, "valu": 1}
+DIRECT_VARIANTS = {"mfma": 0, "valu": 1, "local": 2}
 
 
 def make_opts(out_dtype=GSC_F64, scale=1.0, device=-1, num_threads=0, path=GSC_PATH_AUTO,
-              precision=0, accum=1, direct_variant=1) -> GscOpts:
+              precision=0, accum=1, direct_variant=2) -> GscOpts:
     o = GscOpts()
     lib().gsc_opts_init(ctypes.byref(o))
     o.out_dtype = out_dtype

@@ -76,12 +76,19 @@ typedef enum gsc_accum {
   GSC_ACCUM_F32 = 2     /* one FP32 set over all modes (error grows with M) */
 } gsc_accum;
 
-/* Where the direct (per-point) kernels evaluate the FP64 phases k . x.  On
- * gfx950 the FP64 matrix tiles do not overlap with the VALU sine work (the
- * measured time is the sum of both parts), so the VALU form is the default. */
+/* How the direct (per-point) kernels evaluate the phases k . x.
+ * LOCAL sorts the points into spatial cells and splits every phase at the
+ * cell centre X: (k . X + theta) in FP64 once per cell and mode, k . (x - X)
+ * on the FP32 matrix cores, which run beside the VALU sine work; modes whose
+ * local phase could exceed one revolution keep the FP64 phase.  It serves
+ * gsc_summate and gsc_summate_fourier in 2-D and 3-D (N < 2^30); elsewhere
+ * (incompressible fields, other dimensions) LOCAL runs the VALU kernel.
+ * On gfx950 the FP64 matrix tiles of MFMA do not overlap with the VALU sine
+ * work (the measured time is the sum of both parts). */
 typedef enum gsc_direct_variant {
   GSC_DIRECT_MFMA = 0,  /* v_mfma_f64_16x16x4 tiles of 16 modes x 16 points (dim <= 3) */
-  GSC_DIRECT_VALU = 1   /* FP64 FMA chains on the vector ALUs (default) */
+  GSC_DIRECT_VALU = 1,  /* FP64 FMA chains on the vector ALUs */
+  GSC_DIRECT_LOCAL = 2  /* cell-local phases on the FP32 matrix cores (default) */
 } gsc_direct_variant;
 
 typedef struct gsc_opts {
@@ -160,13 +167,13 @@ int gsc_summate_fourier(int dim, int64_t M, int64_t N,
  * arrays are device pointers in dense row-major layout: cov/modes (dim, M),
  * z1/z2/sf (M), pos (dim, N) with row pitch pos_ld >= N elements, out (N) for
  * the scalar forms and (dim, N) with row pitch out_ld >= N for incompr.
- * `work` is device scratch of at least gsc_workspace_bytes(dim, M) bytes
- * (the per-mode table); it must stay untouched until the stream has run the
- * call.  `sf` is used by the Fourier form only and may be NULL otherwise.
+ * `work` is device scratch of at least gsc_workspace_bytes(dim, M, N, opts)
+ * bytes (the per-mode table and, for the LOCAL variant, the cell sort); it
+ * must stay untouched until the stream has run the call.  `sf` is used by the Fourier form only and may be NULL otherwise.
  * These have no counterpart in the reference (which has no device side);
  * they are the device half of the three functions above.
  */
-size_t gsc_workspace_bytes(int dim, int64_t M);
+size_t gsc_workspace_bytes(int dim, int64_t M, int64_t N, const gsc_opts* opts);
 int gsc_summate_dev(int dim, int64_t M, int64_t N, const double* cov, const double* z1,
                     const double* z2, const double* pos, int64_t pos_ld, void* out,
                     void* work, size_t work_bytes, const gsc_opts* opts, void* stream);

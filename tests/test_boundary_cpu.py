@@ -43,15 +43,22 @@ def test_version_and_defaults():
     o = _native.GscOpts()
     lib.gsc_opts_init(ctypes.byref(o))
     assert (o.out_dtype, o.path, o.precision, o.device, o.scale) == (0, 0, 0, -1, 1.0)
-    assert (o.accum, o.direct_variant, o.blocks_per_cu) == (1, 1, 0)
+    assert (o.accum, o.direct_variant, o.blocks_per_cu) == (1, 2, 0)
     assert gstools_core.__version__ == "1.1.0"
 
 
 def test_workspace_bytes():
-    assert device.workspace_bytes(3, 1000) >= 48 * 1000
-    assert device.workspace_bytes(3, 1000) % 256 == 0
-    assert device.workspace_bytes(5, 10) >= 7 * 8 * 10
-    assert device.workspace_bytes(0, 10) == 0
+    valu = device.workspace_bytes(3, 1000, 10**6, variant="valu")
+    assert valu >= 48 * 1000 and valu % 256 == 0
+    assert device.workspace_bytes(3, 1000, 10, variant="valu") == valu  # no N term
+    # the local variant adds the cell sort: at least a cell index and a slot per point
+    loc = device.workspace_bytes(3, 1000, 10**6)
+    assert loc >= valu + 8 * 10**6
+    assert device.workspace_bytes(2, 1000, 10**6) >= valu + 8 * 10**6
+    # dimensions the local kernel does not serve need only the mode table
+    assert device.workspace_bytes(1, 1000, 10**6) == device.workspace_bytes(1, 1000, 10**6, "valu")
+    assert device.workspace_bytes(5, 10, 100) >= 7 * 8 * 10
+    assert device.workspace_bytes(0, 10, 10) == 0
 
 
 def _ok_args(d=3, m=4, n=5):

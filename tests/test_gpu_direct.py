@@ -198,7 +198,7 @@ def test_device_entry_point_matches_host_entry_point():
     z2 = torch.from_numpy(md.z2).to(dev)
     pos = torch.from_numpy(np.ascontiguousarray(w.pos)).to(dev)
     out = torch.empty(w.n_points, dtype=torch.float64, device=dev)
-    nb = device.workspace_bytes(2, md.m)
+    nb = device.workspace_bytes(2, md.m, w.n_points)
     work = torch.empty(nb, dtype=torch.uint8, device=dev)
     stream = torch.cuda.current_stream().cuda_stream
     device.summate_dev(2, md.m, w.n_points, cov.data_ptr(), z1.data_ptr(), z2.data_ptr(),
@@ -233,9 +233,10 @@ def test_committed_oracle_vectors():
     assert _err(g, v["f_out"]) <= sigma_tol(400) * v["f_sf"].max()
 
 
-@pytest.mark.parametrize("variant", ["mfma", "valu"])
+@pytest.mark.parametrize("variant", ["mfma", "valu", "local"])
 def test_direct_variants_against_oracle(variant):
-    # both phase engines (matrix-core FP64 tiles and VALU FP64 FMA chains)
+    # every phase engine (matrix-core FP64 tiles, VALU FP64 FMA chains, cell-local
+    # FP32 matrix-core phases; incompr runs the VALU kernel for "local")
     for d, m, n, seed in ((1, 300, 3001, 1), (2, 1000, 20_000, 2), (3, 1000, 20_000, 3),
                           (3, 17, 4099, 4), (2, 1, 77, 5)):
         rng = np.random.default_rng(500 + seed)
@@ -261,4 +262,85 @@ def test_direct_variants_agree_at_scale():
     md = w.modes
     a = gstools_core.summate(md.cov_samples, md.z1, md.z2, w.pos, variant="mfma")
     b = gstools_core.summate(md.cov_samples, md.z1, md.z2, w.pos, variant="valu")
+    c = gstools_core.summate(md.cov_samples, md.z1, md.z2, w.pos, variant="local")
     assert _err(a, b) * md.scale <= 2e-6
+    err = _err(c, b) * md.scale
+    print(f"local vs valu at 2e6 points: {err:.3e} sigma")
+    assert err <= 5e-6
+
+
+# -- the cell-local (binned) kernel ---------------------------------------------
+
+def _rand_modes(rng, d, m, s=0.05):
+    return rng.normal(0, s, size=(d, m)), rng.normal(size=m), rng.normal(size=m)
+
+
+@pytest.mark.parametrize("n", [1, 15, 16, 63, 64, 65, 1000, 4097])
+def test_local_partial_waves(n):
+    rng = np.random.default_rng(700 + n)
+    for d in (2, 3):
+        cov, z1, z2 = _rand_modes(rng, d, 300)
+        pos = rng.uniform(-50, 250, size=(d, n))
+        g = gstools_core.summate(cov, z1, z2, pos, variant="local")
+        assert _err(g, oracle.summate(cov, z1, z2, pos)) <= sigma_tol(300)
+
+
+def test_local_degenerate_point_sets():
+    rng = np.random.default_rng(31)
+    cov, z1, z2 = _rand_modes(rng, 3, 500, 0.2)
+    cases = {
+        "identical": np.full((3, 3000), 7.25),
+        "line": np.stack([np.linspace(0, 100, 3000), np.full(3000, 3.0), np.full(3000, -2.0)]),
+        "plane": np.stack([rng.uniform(0, 50, 3000), rng.uniform(0, 50, 3000), np.zeros(3000)]),
+        "two clusters": np.concatenate([rng.normal(0, 1, size=(3, 1500)),
+                                        rng.normal(1e4, 1, size=(3, 1500))], axis=1),
+        "far offset": rng.uniform(1e6, 1e6 + 30, size=(3, 3000)),
+    }
+    for name, pos in cases.items():
+        g = gstools_core.summate(cov, z1, z2, pos, variant="local")
+        err = _err(g, oracle.summate(cov, z1, z2, pos)) / np.sqrt(500)
+        print(f"{name}: {err:.3e}")
+        assert err <= 1e-5, name
+
+
+def test_local_non_finite_points_only_affect_themselves():
+    rng = np.random.default_rng(32)
+    cov, z1, z2 = _rand_modes(rng, 2, 200)
+    pos = rng.uniform(0, 100, size=(2, 5000))
+    pos[0, 10] = np.nan
+    pos[1, 20] = np.inf
+    pos[0, 30] = -np.inf
+    g = gstools_core.summate(cov, z1, z2, pos, variant="local")
+    ref = oracle.summate(cov, z1, z2, pos)
+    bad = np.array([10, 20, 30])
+    assert np.isnan(g[bad]).all() and np.isnan(ref[bad]).all()
+    ok = np.setdiff1d(np.arange(5000), bad)
+    assert _err(g[ok], ref[ok]) <= sigma_tol(200)
+
+
+def test_local_is_bitwise_reproducible_and_order_free():
+    # a point's result depends only on its cell (fixed by the bounding box) and
+    # the mode order, never on the slot the counting sort gives it
+    w = workloads.c2(n=300_000)
+    md = w.modes
+    a = gstools_core.summate(md.cov_samples, md.z1, md.z2, w.pos)
+    b = gstools_core.summate(md.cov_samples, md.z1, md.z2, w.pos)
+    np.testing.assert_array_equal(a, b)
+    perm = np.random.default_rng(33).permutation(w.n_points)
+    c = gstools_core.summate(md.cov_samples, md.z1, md.z2, w.pos[:, perm])
+    np.testing.assert_array_equal(c, a[perm])
+
+
+def test_local_rough_modes_keep_fp64_phase():
+    # a wide spectrum: most modes are too fast for the FP32 local phase and take
+    # the FP64 path; a few slow ones stay on the matrix cores
+    rng = np.random.default_rng(34)
+    m = 400
+    cov = rng.normal(0, 0.05, size=(3, m))
+    cov[:, ::3] *= 400.0
+    z1, z2 = rng.normal(size=m), rng.normal(size=m)
+    pos = rng.uniform(0, 1000, size=(3, 20_000))
+    g = gstools_core.summate(cov, z1, z2, pos, variant="local")
+    err = _err(g, oracle.summate(cov, z1, z2, pos)) / np.sqrt(m)
+    print(f"mixed rough/smooth: {err:.3e}")
+    assert err <= 1e-5
