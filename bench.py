@@ -4,12 +4,15 @@
 Workload: 3-D unstructured field, N = 10,000,000 points uniform in [0,500]^3,
 M = 1000 Gaussian-model modes (l = 20), seeded (gstools_core.workloads.c2),
 FP64 inputs resident in HBM, FP64 output.  One step = one full summation
-(mode-table preparation + the direct kernel) over all points through the
-device entry point gsc_summate_dev, on torch's current HIP stream.
+through the device entry point gsc_summate_dev on torch's current HIP stream:
+mode-table preparation, the spatial counting sort of the points and the
+cell-local direct kernel (the default variant).
 
-Prints ONE JSON line: metric, value (point*modes/s), roofline of the direct
-kernel (transcendental-issue bound) and the CPU FP64 oracle timed on a bounded
-sample of the same workload on this host.
+Prints ONE JSON line: metric, value (point*modes/s), roofline of the
+summation (transcendental-issue bound: one v_sin_f32 per point-mode), the
+CPU FP64 oracle timed on a bounded sample of the same workload on this host,
+the other direct variants, the structured-grid leg (256^3, factored MFMA vs
+direct) and the ensemble leg (512 realisations).
 """
 from __future__ import annotations
 
@@ -273,16 +276,26 @@ def main() -> int:
     # device call (cell sort, mode split and k_direct_local, all on `stream`)
     avg_launch_s = ms_per_step * 1e-3
     achieved = n * m / avg_launch_s / 1e12
+    traffic = None  # HBM bytes per step, measured by rocprofv3 --pmc (profiles/)
+    try:
+        with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as f:
+            t = json.load(f)["direct_local_step_c2"]
+        if n == 10_000_000 and m == 1000 and d == 3:
+            traffic = t["fetch_bytes"] + t["write_bytes"]
+    except (OSError, KeyError, ValueError):
+        pass
     roofline = {
         "bound": "valu-transcendental",
         "achieved": round(achieved, 4),
         "peak": round(TRANS_PEAK / 1e12, 4),
         "unit": "Tsin/s",
         "frac": round(achieved / (TRANS_PEAK / 1e12), 4),
-        "traffic": None,
+        "traffic": traffic,
         "kernel": "gsc_summate_dev: cell sort + k_direct_local<3,double>",
         "work_per_launch": f"{n}*{m} point-modes, 1 v_sin_f32 + 1 v_fma_f32 each on the VALU; "
-                           "phases on v_mfma_f32_16x16x4 (256 point-modes per instruction)",
+                           "phases on v_mfma_f32_16x16x32_bf16 (256 point-modes per instruction)",
+        "traffic_note": "bytes per step (whole device call) from FETCH_SIZE x2 + WRITE_SIZE x1 "
+                        "(copy-calibrated); algorithmic 320 MB",
     }
 
     grid_res = None

@@ -599,15 +599,17 @@ k_direct_local(const BinParams* __restrict__ prm, const int* __restrict__ wave_c
     const int mmax = n_smooth + kModeChunk - 1;  // the B table is padded by a chunk
     f4 ta[4], tb[4];
     float ama, amb;
-    auto issue = [&](f4 (&t)[4], float& am, int q) {
-      const int m = min(c0 + 16 * q + j, mmax);
-      const uint4 bq = bfrag[4 * m];
-      const float2 e = ra[16 * q + j];
+    // B fragments are requested two tiles ahead of their MFMAs
+    auto bload = [&](int q) { return bfrag[4 * min(c0 + 16 * q + j, mmax)]; };
+    uint4 ba = bload(0), bb = bload(1);
+    auto issue = [&](f4 (&t)[4], float& am, uint4& bq, int q) {
+      const float2 e = ra[16 * min(q, kTiles - 1) + j];
       am = e.y;
 #pragma unroll
       for (int g = 0; g < 4; ++g)
         t[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[g], __builtin_bit_cast(bf8, bq),
                                                        f4{e.x, e.x, e.x, e.x}, 0, 0, 0);
+      bq = bload(q + 2);
     };
     auto consume = [&](const f4 (&t)[4], float am) {
 #pragma unroll
@@ -616,12 +618,12 @@ k_direct_local(const BinParams* __restrict__ prm, const int* __restrict__ wave_c
         for (int v = 0; v < 4; ++v)
           acc[g][v] = __builtin_fmaf(am, __builtin_amdgcn_sinf(t[g][v]), acc[g][v]);
     };
-    issue(ta, ama, 0);
+    issue(ta, ama, ba, 0);
     for (int q = 0; q < ntile; q += 2) {
-      issue(tb, amb, min(q + 1, kTiles - 1));
+      issue(tb, amb, bb, q + 1);
       consume(ta, ama);
       if (q + 1 >= ntile) break;
-      issue(ta, ama, min(q + 2, kTiles - 1));
+      issue(ta, ama, ba, q + 2);
       consume(tb, amb);
     }
 #pragma unroll
