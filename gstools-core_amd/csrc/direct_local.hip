@@ -19,7 +19,8 @@
 // revolutions) and one FMA per point-mode (on gfx950 the matrix instructions
 // do not run beside the VALU, so the 16 cycles of one MFMA per 256
 // point-modes are the whole cost of the phases).  FP32 partial sums over
-// 128-mode chunks are added into FP64 totals.
+// kModeChunk modes are added into a second FP32 set (one add per chunk and
+// point-mode column); the 16 columns of a point are summed in FP64.
 //
 // Modes whose local phase could exceed kRoughRev revolutions in some cell
 // (sum_k |kap_k| h_k / 2 > kRoughRev, e.g. the far tail of a heavy-tailed
@@ -67,13 +68,20 @@ __device__ __forceinline__ void split3(float x, uint16_t (&p)[3]) {
   }
 }
 
-constexpr int kCellPts = 512;       // target points per cell
+constexpr int kCellPts = 2048;      // target points per cell
+constexpr int kMinCellPts = 64;     // the cell count never exceeds N / kMinCellPts
+constexpr double kCellRev = 0.75;   // cells keep sum_k |kap_k| h_k / 2 below this (revolutions)
+                                    // for all but the widest kBandSkip of the modes
+constexpr double kBandSkip = 0.005;
+constexpr int kQBins = 512;         // quarter-octave bins of sum_k |kap_k|
 constexpr int kMaxCells = 16384;    // cell counters of one block fill 64 KB of LDS
-constexpr int kMaxSortBlocks = 1024;
-constexpr int kColBlocks = 32;      // point blocks per group of the column scan
+constexpr int kMaxSortBlocks = 256;
+constexpr int kSortThreads = 1024;  // threads of the histogram and scatter blocks
+constexpr int kSortUnroll = 4;      // points in flight per thread there
+constexpr int kColBlocks = 16;      // point blocks per group of the column scan
 constexpr int kColGroups = kMaxSortBlocks / kColBlocks;
 constexpr int kWavePts = 64;        // point slots per wave (4 MFMA groups of 16)
-constexpr int kModeChunk = 256;     // modes per (r0, amplitude) table of a wave (16 tiles)
+constexpr int kModeChunk = 512;     // modes per (r0, amplitude) table of a wave (32 tiles)
 constexpr double kRoughRev = 1.0;   // bound on |kap . xi| (revolutions) of the FP32 phase
 constexpr int kBoxBlocks = 1024;    // partial bounding boxes
 constexpr int kRedLd = 65;          // LDS row of the final cross-lane sums (doubles)
@@ -82,6 +90,7 @@ struct alignas(16) BinParams {
   double lo[3];
   double h[3];
   double inv_h[3];
+  double inv_f[3];  // 1 / f
   int n[3];       // cells per dimension
   int f[3];       // cells per coarse cell and dimension
   int C[3];       // coarse cells per dimension
@@ -102,22 +111,22 @@ constexpr size_t al256(size_t b) { return (b + 255) & ~static_cast<size_t>(255);
 
 struct BinLayout {
   int64_t max_cells, max_waves, nblk, blk_pts;
-  size_t params, box, cell_of, bcount, gsum, counts, offsets, wave_cell, recs, btpl, amp, sm, rough,
+  size_t params, box, centre, bcount, gsum, counts, offsets, wave_cell, recs, btpl, amp, sm, rough,
       bytes;
 };
 
 BinLayout bin_layout(int64_t M, int64_t N) {
   BinLayout L{};
-  L.max_cells = std::min<int64_t>(kMaxCells, N / (kCellPts / 2) + 1);
+  L.max_cells = std::min<int64_t>(kMaxCells, N / kMinCellPts + 1);
   L.max_waves = (N + kWavePts - 1) / kWavePts + L.max_cells;
-  L.nblk = std::max<int64_t>(1, std::min<int64_t>(kMaxSortBlocks, (N + 4095) / 4096));
+  L.nblk = std::max<int64_t>(1, std::min<int64_t>(kMaxSortBlocks, (N + 16383) / 16384));
   L.blk_pts = (N + L.nblk - 1) / L.nblk;
   const size_t m = static_cast<size_t>(M > 0 ? M : 1);
   size_t off = 0;
   auto take = [&](size_t b) { const size_t o = off; off += al256(b); return o; };
   L.params = take(sizeof(BinParams));
   L.box = take(sizeof(double) * 6 * kBoxBlocks);
-  L.cell_of = take(sizeof(int) * static_cast<size_t>(N));
+  L.centre = take(sizeof(double) * 4 * static_cast<size_t>(L.max_cells));
   L.bcount = take(sizeof(int) * static_cast<size_t>(L.nblk * L.max_cells));
   L.gsum = take(sizeof(int) * static_cast<size_t>(kColGroups * L.max_cells));
   L.counts = take(sizeof(int) * static_cast<size_t>(L.max_cells));
@@ -185,13 +194,54 @@ k_box_partial(const double* __restrict__ pos, int64_t ld, int64_t N, double* __r
     }
 }
 
-// Cell grid from the bounding box: about N / kCellPts near-cubic cells over
-// the dimensions with a non-zero extent, at most max_cells in all.
+// Cell grid from the bounding box: near-cubic cells over the dimensions with
+// a non-zero extent, about N / kCellPts of them, or more where the modes
+// need it: the cell width h is kept below 2 kCellRev / s_q, s_q the (1 -
+// kBandSkip) quantile of s_m = sum_k |kap_mk| over the finite modes (to a
+// quarter octave), so the local phase of those modes stays below kCellRev
+// revolutions.  At most max_cells cells in all.
 template <int D>
 __global__ void __launch_bounds__(256)
 k_bin_params(const double* __restrict__ box, int64_t N, int64_t max_cells,
-             BinParams* __restrict__ prm) {
+             const ModeRec* __restrict__ modes, int M, BinParams* __restrict__ prm) {
   __shared__ double red[2 * D][256];
+  __shared__ int qh[kQBins];
+  for (int b = threadIdx.x; b < kQBins; b += 256) qh[b] = 0;
+  __syncthreads();
+  for (int m = threadIdx.x; m < M; m += 256) {
+    double sk = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) sk += fabs(modes[m].kap[k]);
+    if (!isfinite(sk)) continue;
+    const int b = sk > 0.0 ? static_cast<int>(fmin(fmax(floor(4.0 * log2(sk)) + kQBins / 2, 0.0),
+                                                   kQBins - 1.0))
+                           : 0;
+    atomicAdd(&qh[b], 1);
+  }
+  __syncthreads();
+  // the quantile's bin: inclusive scan of bin pairs, then the pair that
+  // crosses the wanted count
+  __shared__ int qs[256];
+  __shared__ double s_q_s;
+  const int c0 = qh[2 * threadIdx.x], c1 = qh[2 * threadIdx.x + 1];
+  qs[threadIdx.x] = c0 + c1;
+  if (threadIdx.x == 0) s_q_s = 0.0;
+  __syncthreads();
+  for (int st = 1; st < 256; st <<= 1) {
+    const int v = threadIdx.x >= st ? qs[threadIdx.x - st] : 0;
+    __syncthreads();
+    qs[threadIdx.x] += v;
+    __syncthreads();
+  }
+  const int nfin = qs[255];
+  if (nfin > 0) {
+    const int want = static_cast<int>(ceil((1.0 - kBandSkip) * nfin));
+    const int before = qs[threadIdx.x] - c0 - c1;
+    if (before < want && want <= before + c0 + c1) {
+      const int b = before + c0 >= want ? 2 * threadIdx.x : 2 * threadIdx.x + 1;
+      s_q_s = b == 0 ? 0.0 : exp2((b + 1 - kQBins / 2) / 4.0);  // upper edge of the bin
+    }
+  }
 #pragma unroll
   for (int k = 0; k < D; ++k) {
     double a = INFINITY, b = -INFINITY;
@@ -215,6 +265,7 @@ k_bin_params(const double* __restrict__ box, int64_t N, int64_t max_cells,
     __syncthreads();
   }
   if (threadIdx.x != 0) return;
+  const double s_q = s_q_s;
   double lo[3] = {0.0, 0.0, 0.0}, ext[3] = {0.0, 0.0, 0.0};
   for (int k = 0; k < D; ++k) {
     double a = red[2 * k][0], b = red[2 * k + 1][0];
@@ -222,7 +273,6 @@ k_bin_params(const double* __restrict__ box, int64_t N, int64_t max_cells,
     lo[k] = a;
     ext[k] = b - a;
   }
-  const double target = fmax(1.0, static_cast<double>(N) / kCellPts);
   double vol = 1.0;
   int active = 0;
   for (int k = 0; k < D; ++k)
@@ -230,6 +280,10 @@ k_bin_params(const double* __restrict__ box, int64_t N, int64_t max_cells,
       vol *= ext[k];
       ++active;
     }
+  double target = fmax(1.0, static_cast<double>(N) / kCellPts);
+  if (active > 0 && s_q > 0.0)
+    target = fmax(target, vol / pow(2.0 * kCellRev / s_q, static_cast<double>(active)));
+  target = fmin(target, static_cast<double>(max_cells));
   int n[3] = {1, 1, 1};
   if (active > 0) {
     const double h = pow(vol / target, 1.0 / active);
@@ -257,6 +311,7 @@ k_bin_params(const double* __restrict__ box, int64_t N, int64_t max_cells,
     P.C[k] = C[k];
     P.h[k] = ext[k] > 0.0 ? ext[k] / n[k] : 1.0;
     P.inv_h[k] = ext[k] > 0.0 ? 1.0 / P.h[k] : 0.0;
+    P.inv_f[k] = 1.0 / f[k];
   }
   P.F = f[0] * f[1] * f[2];
   P.ncoarse = C[0] * C[1] * C[2];
@@ -264,46 +319,61 @@ k_bin_params(const double* __restrict__ box, int64_t N, int64_t max_cells,
   *prm = P;
 }
 
-// Number of the cell of point x: coarse cell (c0 C1 + c1) C2 + c2, then the
-// cell within it (l0 f1 + l1) f2 + l2, where cell index i_k = c_k f_k + l_k.
+// Cell of point x: index i_k = floor((x_k - lo_k) / h_k) per dimension
+// (NaN and x < lo land in cell 0, x >= hi in the last cell), numbered coarse
+// cell (c0 C1 + c1) C2 + c2 first, then the cell within it (l0 f1 + l1) f2 +
+// l2, where i_k = c_k f_k + l_k.  c_k = floor((i_k + 0.5) / f_k) is taken in
+// FP64 (exact: the quotient is at least 0.5 / f_k from an integer) instead
+// of an integer division.  X receives the cell centre lo + (i + 1/2) h.
 template <int D>
-__device__ __forceinline__ int cell_of_point(const BinParams& P, const double* x) {
+__device__ __forceinline__ int cell_locate(const BinParams& P, const double* x, double (&X)[3]) {
   int cc = 0, l = 0;
+  X[0] = X[1] = X[2] = 0.0;
 #pragma unroll
   for (int k = 0; k < D; ++k) {
     const double fr = (x[k] - P.lo[k]) * P.inv_h[k];
-    int ik = 0;  // NaN and x < lo land in cell 0, x >= hi in the last cell
+    int ik = 0;
     if (fr >= 0.0) ik = fr < static_cast<double>(P.n[k]) ? static_cast<int>(fr) : P.n[k] - 1;
-    cc = cc * P.C[k] + ik / P.f[k];
-    l = l * P.f[k] + ik % P.f[k];
+    const int ck = static_cast<int>((ik + 0.5) * P.inv_f[k]);
+    cc = cc * P.C[k] + ck;
+    l = l * P.f[k] + (ik - ck * P.f[k]);
+    X[k] = P.lo[k] + (ik + 0.5) * P.h[k];
   }
   return cc * P.F + l;
 }
 
 // Point block b (points [b blk_pts, (b + 1) blk_pts)) counts its points per
-// cell in LDS; bcount[b][c] receives the counts.
+// cell in LDS; bcount[b][c] receives the counts.  1024 threads, kSortUnroll
+// points in flight per thread: the 64 KB counter array allows two blocks per
+// CU, so the memory latency is hidden by width, not by more blocks.
 template <int D>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kSortThreads)
 k_bin_hist(const double* __restrict__ pos, int64_t ld, int64_t N, int64_t blk_pts,
-           const BinParams* __restrict__ prm, int* __restrict__ cell_of,
-           int* __restrict__ bcount) {
+           const BinParams* __restrict__ prm, int* __restrict__ bcount) {
   __shared__ int hist[kMaxCells];
   const BinParams P = *prm;
-  for (int c = threadIdx.x; c < P.ncells; c += 256) hist[c] = 0;
+  for (int c = threadIdx.x; c < P.ncells; c += kSortThreads) hist[c] = 0;
   __syncthreads();
   const int64_t p0 = static_cast<int64_t>(blockIdx.x) * blk_pts;
   const int64_t p1 = min(p0 + blk_pts, N);
-  for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
-    double x[D];
+  for (int64_t pb = p0 + threadIdx.x; pb < p1; pb += kSortThreads * kSortUnroll) {
+    double x[kSortUnroll][D];
 #pragma unroll
-    for (int k = 0; k < D; ++k) x[k] = pos[k * ld + p];
-    const int c = cell_of_point<D>(P, x);
-    cell_of[p] = c;
-    atomicAdd(&hist[c], 1);
+    for (int u = 0; u < kSortUnroll; ++u) {
+      const int64_t p = min(pb + u * kSortThreads, p1 - 1);
+#pragma unroll
+      for (int k = 0; k < D; ++k) x[u][k] = pos[k * ld + p];
+    }
+#pragma unroll
+    for (int u = 0; u < kSortUnroll; ++u) {
+      double X[3];
+      const int c = cell_locate<D>(P, x[u], X);
+      if (pb + u * kSortThreads < p1) atomicAdd(&hist[c], 1);
+    }
   }
   __syncthreads();
   int* row = bcount + static_cast<int64_t>(blockIdx.x) * P.ncells;
-  for (int c = threadIdx.x; c < P.ncells; c += 256) row[c] = hist[c];
+  for (int c = threadIdx.x; c < P.ncells; c += kSortThreads) row[c] = hist[c];
 }
 
 // Exclusive scan of every column of bcount over the point blocks, in three
@@ -405,43 +475,60 @@ __device__ __forceinline__ void cell_centre(const BinParams& P, int e, double (&
   }
 }
 
-// Thread per cell: records its waves in the wave -> cell map and marks the
-// slots past its points as padding.
+// Thread per cell: records its centre and its waves in the wave -> cell map
+// and marks the slots past its points as padding.
+template <int D>
 __global__ void __launch_bounds__(256)
 k_bin_waves(const BinParams* __restrict__ prm, const int* __restrict__ counts,
-            const int* __restrict__ offsets, int* __restrict__ wave_cell,
+            const int* __restrict__ offsets, double* __restrict__ centre,
+            int* __restrict__ wave_cell,
             float4* __restrict__ recs) {
   const int e = blockIdx.x * 256 + threadIdx.x;
-  if (e >= prm->ncells) return;
+  const BinParams P = *prm;
+  if (e >= P.ncells) return;
+  double X[3];
+  cell_centre<D>(P, e, X);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) centre[4 * e + k] = X[k];
   const int s0 = offsets[e], s1 = offsets[e + 1];
   for (int s = s0; s < s1; s += kWavePts) wave_cell[s / kWavePts] = e;
   for (int s = s0 + counts[e]; s < s1; ++s) recs[s] = make_rec(0.0f, 0.0f, 0.0f, -1);
 }
 
-// Point block b again: ranks its points within each cell with LDS atomics
-// (starting from the block's base in the cell) and writes their records
-// (local coordinates, index) to their slots.
+// Point block b again (same threads and points as k_bin_hist): ranks its
+// points within each cell with LDS atomics (starting from the block's base in
+// the cell) and writes their records (local coordinates, index) to their
+// slots.
 template <int D>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kSortThreads)
 k_bin_scatter(const double* __restrict__ pos, int64_t ld, int64_t N, int64_t blk_pts,
-              const BinParams* __restrict__ prm, const int* __restrict__ cell_of,
-              const int* __restrict__ bcount, const int* __restrict__ offsets,
-              float4* __restrict__ recs) {
+              const BinParams* __restrict__ prm, const int* __restrict__ bcount,
+              const int* __restrict__ offsets, float4* __restrict__ recs) {
   __shared__ int rank[kMaxCells];
   const BinParams P = *prm;
   const int* base = bcount + static_cast<int64_t>(blockIdx.x) * P.ncells;
-  for (int e = threadIdx.x; e < P.ncells; e += 256) rank[e] = offsets[e] + base[e];
+  for (int e = threadIdx.x; e < P.ncells; e += kSortThreads) rank[e] = offsets[e] + base[e];
   __syncthreads();
   const int64_t p0 = static_cast<int64_t>(blockIdx.x) * blk_pts;
   const int64_t p1 = min(p0 + blk_pts, N);
-  for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
-    const int e = cell_of[p];
-    double X[3];
-    cell_centre<D>(P, e, X);
-    float xi[3] = {0.0f, 0.0f, 0.0f};
+  for (int64_t pb = p0 + threadIdx.x; pb < p1; pb += kSortThreads * kSortUnroll) {
+    double x[kSortUnroll][D];
 #pragma unroll
-    for (int k = 0; k < D; ++k) xi[k] = static_cast<float>(pos[k * ld + p] - X[k]);
-    recs[atomicAdd(&rank[e], 1)] = make_rec(xi[0], xi[1], xi[2], static_cast<int>(p));
+    for (int u = 0; u < kSortUnroll; ++u) {
+      const int64_t p = min(pb + u * kSortThreads, p1 - 1);
+#pragma unroll
+      for (int k = 0; k < D; ++k) x[u][k] = pos[k * ld + p];
+    }
+#pragma unroll
+    for (int u = 0; u < kSortUnroll; ++u) {
+      const int64_t p = pb + u * kSortThreads;
+      double X[3];
+      const int e = cell_locate<D>(P, x[u], X);  // the cell k_bin_hist counted it in
+      float xi[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int k = 0; k < D; ++k) xi[k] = static_cast<float>(x[u][k] - X[k]);
+      if (p < p1) recs[atomicAdd(&rank[e], 1)] = make_rec(xi[0], xi[1], xi[2], static_cast<int>(p));
+    }
   }
 }
 
@@ -526,15 +613,16 @@ k_mode_split(const ModeRec* __restrict__ modes, int M, BinParams* __restrict__ p
 // One wave per kWavePts slots of one cell; four independent waves per block,
 // each with its own LDS region (no block barriers).
 template <int D, typename OutT>
-__global__ void __launch_bounds__(256)
-k_direct_local(const BinParams* __restrict__ prm, const int* __restrict__ wave_cell,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4)))
+k_direct_local(const BinParams* __restrict__ prm, const double* __restrict__ centre,
+               const int* __restrict__ wave_cell,
                const float4* __restrict__ recs, const uint4* __restrict__ btpl,
                const float* __restrict__ ampt, const SmoothRec* __restrict__ sm,
                const ModeRec* __restrict__ rough, const double* __restrict__ pos, int64_t ld,
                OutT* __restrict__ out, double scale) {
   // per wave: (r0, amplitude) of one chunk of modes, and afterwards the
   // 16 x 64 partial sums (row stride kRedLd doubles)
-  constexpr int kWaveLds = kModeChunk * 8 > 16 * kRedLd * 8 ? kModeChunk * 8 : 16 * kRedLd * 8;
+  constexpr int kWaveLds = kModeChunk * 20 > 16 * kRedLd * 8 ? kModeChunk * 20 : 16 * kRedLd * 8;
   __shared__ __attribute__((aligned(16))) char lds[4][kWaveLds];
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -543,7 +631,12 @@ k_direct_local(const BinParams* __restrict__ prm, const int* __restrict__ wave_c
   if (w >= P.nwaves) return;
   const int n_smooth = P.n_smooth, n_rough = P.n_rough;
   double X[3];
-  cell_centre<D>(P, wave_cell[w], X);
+  {
+    const double* xc = centre + 4 * static_cast<int64_t>(wave_cell[w]);
+    X[0] = xc[0];
+    X[1] = xc[1];
+    X[2] = xc[2];
+  }
   // A operands: lane l holds slots 8 (l / 16) .. + 7 of point l % 16 of group g
   const int kg = lane >> 4, j = lane & 15;
   const float* rf = reinterpret_cast<const float*>(recs + static_cast<int64_t>(w) * kWavePts);
@@ -564,52 +657,61 @@ k_direct_local(const BinParams* __restrict__ prm, const int* __restrict__ wave_c
       v[e] = kg == 0 ? sl[e] : (kg == 1 ? sl[8 + e] : (kg == 2 ? sl[16 + e] : sl[24 + e]));
     a[g] = __builtin_bit_cast(bf8, v);
   }
-  float2* ra = reinterpret_cast<float2*>(lds[wv]);
-  const uint4* bfrag = btpl + kg;  // lane's k-range of mode j: btpl[4 m + kg]
-  float acc[4][4];
-  double tot[4][4];
+  // per-chunk tables in the wave's LDS region: the MFMA C operand of every
+  // mode (r0 replicated over the four rows a lane holds) and its amplitude
+  f4* rq = reinterpret_cast<f4*>(lds[wv]);
+  float* ram = reinterpret_cast<float*>(rq + kModeChunk);
+  float acc[4][4], tot[4][4];
 #pragma unroll
   for (int g = 0; g < 4; ++g)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       acc[g][v] = 0.0f;
-      tot[g][v] = 0.0;
+      tot[g][v] = 0.0f;
     }
   for (int c0 = 0; c0 < n_smooth; c0 += kModeChunk) {
-    // (r0, amplitude) of modes c0 .. c0 + kModeChunk - 1 for this cell: the
-    // FP64 phase at the cell centre, reduced exactly (zeros past n_smooth)
-#pragma unroll
+    // r0 of modes c0 .. c0 + kModeChunk - 1 for this cell: the FP64 phase at
+    // the cell centre, reduced exactly (zero phase and amplitude past n_smooth);
+    // four records in flight keep the register peak below the main loop's
+#pragma unroll 4
     for (int r = 0; r < kModeChunk / 64; ++r) {
       const int i = c0 + 64 * r + lane;
-      float2 e = make_float2(0.0f, 0.0f);
+      float r0 = 0.0f, am = 0.0f;
       if (i < n_smooth) {
         const SmoothRec sr = sm[i];
         double t = __builtin_fma(sr.kap[0], X[0], sr.th);
         t = __builtin_fma(sr.kap[1], X[1], t);
         if constexpr (D == 3) t = __builtin_fma(sr.kap[2], X[2], t);
-        e = make_float2(static_cast<float>(t - rint(t)), ampt[i]);
+        r0 = static_cast<float>(t - rint(t));
+        am = ampt[i];
       }
-      ra[64 * r + lane] = e;
+      rq[64 * r + lane] = f4{r0, r0, r0, r0};
+      ram[64 * r + lane] = am;
     }
     // software pipeline: the phases of tile q + 1 are issued before the
     // sines of tile q, so the MFMA results are ready when they are read; the
-    // loop is unrolled by two so the two result sets keep their registers
-    constexpr int kTiles = kModeChunk / 16;
-    const int ntile = min(kTiles, (n_smooth - c0 + 15) / 16);
-    const int mmax = n_smooth + kModeChunk - 1;  // the B table is padded by a chunk
+    // loop is unrolled by two so the two result sets keep their registers.
+    // B fragments are requested two tiles ahead of their MFMAs; the reads
+    // run at most 47 modes past n_smooth, inside the zeroed padding.
+    const int ntile = min(kModeChunk / 16, (n_smooth - c0 + 15) / 16);
+    const uint4* bp = btpl + 4 * (c0 + j) + kg;  // lane's k-range of mode c0 + j
+    const f4* rp = rq + j;
+    const float* mp = ram + j;
+    uint4 ba = bp[0], bb = bp[64];
+    bp += 128;
     f4 ta[4], tb[4];
     float ama, amb;
-    // B fragments are requested two tiles ahead of their MFMAs
-    auto bload = [&](int q) { return bfrag[4 * min(c0 + 16 * q + j, mmax)]; };
-    uint4 ba = bload(0), bb = bload(1);
-    auto issue = [&](f4 (&t)[4], float& am, uint4& bq, int q) {
-      const float2 e = ra[16 * min(q, kTiles - 1) + j];
-      am = e.y;
+    auto issue = [&](f4 (&t)[4], float& am, uint4& bq) {
+      const f4 c = *rp;
+      am = *mp;
+      rp += 16;
+      mp += 16;
 #pragma unroll
       for (int g = 0; g < 4; ++g)
-        t[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[g], __builtin_bit_cast(bf8, bq),
-                                                       f4{e.x, e.x, e.x, e.x}, 0, 0, 0);
-      bq = bload(q + 2);
+        t[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[g], __builtin_bit_cast(bf8, bq), c, 0, 0,
+                                                       0);
+      bq = *bp;
+      bp += 64;
     };
     auto consume = [&](const f4 (&t)[4], float am) {
 #pragma unroll
@@ -618,19 +720,28 @@ k_direct_local(const BinParams* __restrict__ prm, const int* __restrict__ wave_c
         for (int v = 0; v < 4; ++v)
           acc[g][v] = __builtin_fmaf(am, __builtin_amdgcn_sinf(t[g][v]), acc[g][v]);
     };
-    issue(ta, ama, ba, 0);
-    for (int q = 0; q < ntile; q += 2) {
-      issue(tb, amb, bb, q + 1);
+    // (the loop body issues unconditionally, so every wait for a B fragment
+    // leaves the younger prefetch in flight)
+    issue(ta, ama, ba);
+    int q = 0;
+    for (; q + 2 < ntile; q += 2) {
+      issue(tb, amb, bb);
       consume(ta, ama);
-      if (q + 1 >= ntile) break;
-      issue(ta, ama, ba, q + 2);
+      issue(ta, ama, ba);
       consume(tb, amb);
+    }
+    if (q + 1 < ntile) {
+      issue(tb, amb, bb);
+      consume(ta, ama);
+      consume(tb, amb);
+    } else {
+      consume(ta, ama);
     }
 #pragma unroll
     for (int g = 0; g < 4; ++g)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        tot[g][v] += static_cast<double>(acc[g][v]);
+        tot[g][v] += acc[g][v];
         acc[g][v] = 0.0f;
       }
   }
@@ -642,7 +753,7 @@ k_direct_local(const BinParams* __restrict__ prm, const int* __restrict__ wave_c
 #pragma unroll
   for (int g = 0; g < 4; ++g)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) red[j * kRedLd + 16 * g + 4 * kg + v] = tot[g][v];
+    for (int v = 0; v < 4; ++v) red[j * kRedLd + 16 * g + 4 * kg + v] = static_cast<double>(tot[g][v]);
   double sum = 0.0;
 #pragma unroll
   for (int jj = 0; jj < 16; ++jj) sum += red[jj * kRedLd + lane];
@@ -678,7 +789,7 @@ hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table
   const BinLayout L = bin_layout(M, N);
   BinParams* prm = at<BinParams>(bin_work, L.params);
   double* box = at<double>(bin_work, L.box);
-  int* cell_of = at<int>(bin_work, L.cell_of);
+  double* centre = at<double>(bin_work, L.centre);
   int* bcount = at<int>(bin_work, L.bcount);
   int* gsum = at<int>(bin_work, L.gsum);
   int* counts = at<int>(bin_work, L.counts);
@@ -700,21 +811,23 @@ hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table
 #define GSC_BIN(D)                                                                               \
   do {                                                                                           \
     k_box_partial<D><<<kBoxBlocks, 256, 0, s>>>(pos, pos_ld, N, box);                            \
-    k_bin_params<D><<<1, 256, 0, s>>>(box, N, L.max_cells, prm);                                 \
-    k_bin_hist<D><<<nblk, 256, 0, s>>>(pos, pos_ld, N, L.blk_pts, prm, cell_of, bcount);         \
+    k_bin_params<D><<<1, 256, 0, s>>>(box, N, L.max_cells, modes, Mi, prm);                      \
+    k_bin_hist<D><<<nblk, kSortThreads, 0, s>>>(pos, pos_ld, N, L.blk_pts, prm, bcount);         \
     k_col_sum<<<dim3(cb, ngroups), 256, 0, s>>>(prm, nblk, bcount, gsum);                        \
     k_col_groups<<<cb, 256, 0, s>>>(prm, ngroups, gsum, counts);                                 \
     k_col_fix<<<dim3(cb, ngroups), 256, 0, s>>>(prm, nblk, bcount, gsum);                        \
     k_bin_offsets<<<1, 1024, 0, s>>>(prm, counts, offsets);                                      \
-    k_bin_waves<<<cb, 256, 0, s>>>(prm, counts, offsets, wave_cell, recs);                       \
-    k_bin_scatter<D><<<nblk, 256, 0, s>>>(pos, pos_ld, N, L.blk_pts, prm, cell_of, bcount,       \
-                                          offsets, recs);                                        \
+    k_bin_waves<D><<<cb, 256, 0, s>>>(prm, counts, offsets, centre, wave_cell, recs);            \
+    k_bin_scatter<D><<<nblk, kSortThreads, 0, s>>>(pos, pos_ld, N, L.blk_pts, prm, bcount,       \
+                                                   offsets, recs);                               \
     k_mode_split<D><<<1, 256, 0, s>>>(modes, Mi, prm, btpl, ampt, sm, rough);                    \
     if (f32)                                                                                     \
-      k_direct_local<D, float><<<mb, 256, 0, s>>>(prm, wave_cell, recs, btpl, ampt, sm, rough,   \
+      k_direct_local<D, float><<<mb, 256, 0, s>>>(prm, centre, wave_cell, recs, btpl, ampt, sm,  \
+                                                  rough,                                         \
                                                   pos, pos_ld, static_cast<float*>(out), scale); \
     else                                                                                         \
-      k_direct_local<D, double><<<mb, 256, 0, s>>>(prm, wave_cell, recs, btpl, ampt, sm, rough,  \
+      k_direct_local<D, double><<<mb, 256, 0, s>>>(prm, centre, wave_cell, recs, btpl, ampt, sm, \
+                                                   rough,                                        \
                                                    pos, pos_ld, static_cast<double*>(out),       \
                                                    scale);                                       \
   } while (0)
