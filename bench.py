@@ -183,6 +183,41 @@ def ensemble_leg(steps: int, warmup: int) -> dict:
     return res
 
 
+def incompr_leg(steps: int, warmup: int, n: int = 4_000_000, m: int = 1000) -> dict:
+    """summate_incompr on the 3-D unstructured workload at N = 4e6, M = 1000:
+    the cell-local kernel (one sine per point-mode shared by the three
+    components) and the FP64-phase VALU kernel, device-resident, FP64 out."""
+    import numpy as np
+    import torch
+
+    from gstools_core import device, workloads
+
+    w = workloads.c2(n=n, m=m)
+    md = w.modes
+    d = md.dim
+    dev = torch.device("cuda:0")
+    cov = torch.from_numpy(np.ascontiguousarray(md.cov_samples)).to(dev)
+    z1 = torch.from_numpy(md.z1).to(dev)
+    z2 = torch.from_numpy(md.z2).to(dev)
+    pos = torch.from_numpy(np.ascontiguousarray(w.pos)).to(dev)
+    out = torch.empty((d, n), dtype=torch.float64, device=dev)
+    nb = max(device.workspace_bytes(d, m, n, v) for v in ("local", "valu"))
+    work = torch.empty(nb, dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream()
+    res = {"workload": f"3-D unstructured summate_incompr, N={n} uniform in [0,500]^3, "
+                       f"M={m} Gaussian modes (l=20)"}
+    for vname in ("local", "valu"):
+        def step(vname=vname):
+            device.summate_incompr_dev(d, m, n, cov.data_ptr(), z1.data_ptr(), z2.data_ptr(),
+                                       pos.data_ptr(), n, out.data_ptr(), n, work.data_ptr(), nb,
+                                       stream=stream.cuda_stream, variant=vname)
+
+        _, tot = time_steps(step, steps, warmup, stream)
+        ms = tot / steps
+        res[f"{vname}_phase"] = {"ms_per_step": ms, "point_modes_per_s": n * m / (ms * 1e-3)}
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,9 +229,9 @@ def main() -> int:
     ap.add_argument("--m", type=int, default=1000)
     ap.add_argument("--no-grid", action="store_true", help="skip the structured-grid leg")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble leg")
-    ap.add_argument("--legs", default="direct,grid,ensemble",
-                    help="comma list of legs: direct, grid, ensemble, copy (a 1 GiB device "
-                         "copy, the reference for HBM byte counters)")
+    ap.add_argument("--legs", default="direct,incompr,grid,ensemble",
+                    help="comma list of legs: direct, incompr, grid, ensemble, copy (a 1 GiB "
+                         "device copy, the reference for HBM byte counters)")
     args = ap.parse_args()
     legs = set(args.legs.split(","))
     if args.no_grid:
@@ -225,6 +260,8 @@ def main() -> int:
             return 0
     if "direct" not in legs:
         res = {}
+        if "incompr" in legs:
+            res["incompr"] = incompr_leg(max(3, args.steps // 2), 1)
         if "grid" in legs:
             res["structured_grid"] = grid_leg(args.steps, args.warmup)
         if "ensemble" in legs:
@@ -298,6 +335,11 @@ def main() -> int:
                         "(copy-calibrated); algorithmic 320 MB",
     }
 
+    inc_res = None
+    if "incompr" in legs:
+        log("incompressible leg (N = 4e6)")
+        inc_res = incompr_leg(max(3, args.steps // 2), 1)
+
     grid_res = None
     if "grid" in legs:
         log("grid leg (256^3, factored vs direct)")
@@ -344,6 +386,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "direct_variants": direct_variants,
+        "incompr": inc_res,
         "structured_grid": grid_res,
         "ensemble": ens_res,
     }

@@ -150,12 +150,11 @@ size_t align256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
 // Shared body of the three host-pointer summators.
 //   kind: gsc::KIND_*; for KIND_INCOMPR `out` is (dim, N) with strides
 //   (out_sd, out_sp), otherwise N values with stride out_s = out_sp.
-// The local-phase (binned) direct kernel serves the scalar and Fourier
-// summators in 2-D and 3-D; incompressible fields and other dimensions use the
-// FP64-phase VALU kernel for that variant.
+// The local-phase (binned) direct kernel serves all three summators in 2-D
+// and 3-D; other dimensions use the FP64-phase VALU kernel for that variant.
 bool use_binned(int kind, int dim, int64_t N, const gsc_opts* o) {
-  return o->direct_variant == GSC_DIRECT_LOCAL && kind != gsc::KIND_INCOMPR &&
-         gsc::binned_supported(dim, N);
+  (void)kind;
+  return o->direct_variant == GSC_DIRECT_LOCAL && gsc::binned_supported(dim, N);
 }
 
 size_t direct_work_bytes(int kind, int dim, int64_t M, int64_t N, const gsc_opts* o) {
@@ -168,15 +167,15 @@ size_t direct_work_bytes(int kind, int dim, int64_t M, int64_t N, const gsc_opts
 hipError_t launch_direct(int kind, int dim, int64_t M, int64_t N, void* work, const double* pos,
                          int64_t pos_ld, void* out, int64_t out_ld, const gsc_opts* o,
                          hipStream_t s) {
+  if (use_binned(kind, dim, N, o)) {
+    void* bin = static_cast<char*>(work) + align256(gsc::mode_table_bytes(dim, M));
+    return gsc::launch_direct_binned(dim, M, N, work, pos, pos_ld, out, out_ld,
+                                     kind == gsc::KIND_INCOMPR, o->out_dtype, o->scale, bin, s);
+  }
   if (kind == gsc::KIND_INCOMPR) {
     const int variant = o->direct_variant == GSC_DIRECT_MFMA ? GSC_DIRECT_MFMA : GSC_DIRECT_VALU;
     return gsc::launch_direct_incompr(dim, M, N, work, pos, pos_ld, out, out_ld, o->out_dtype,
                                       o->scale, variant, s);
-  }
-  if (use_binned(kind, dim, N, o)) {
-    void* bin = static_cast<char*>(work) + align256(gsc::mode_table_bytes(dim, M));
-    return gsc::launch_direct_binned(dim, M, N, work, pos, pos_ld, out, o->out_dtype, o->scale,
-                                     bin, s);
   }
   const int variant = o->direct_variant == GSC_DIRECT_MFMA ? GSC_DIRECT_MFMA : GSC_DIRECT_VALU;
   return gsc::launch_direct_scalar(dim, M, N, work, pos, pos_ld, out, o->out_dtype, o->scale,

@@ -135,7 +135,7 @@ BinLayout bin_layout(int64_t M, int64_t N) {
   L.wave_cell = take(sizeof(int) * static_cast<size_t>(L.max_waves));
   L.recs = take(sizeof(float4) * static_cast<size_t>(L.max_waves) * kWavePts);
   L.btpl = take(sizeof(uint4) * 4 * (m + kModeChunk));
-  L.amp = take(sizeof(float) * (m + kModeChunk));
+  L.amp = take(sizeof(float4) * (m + kModeChunk));
   L.sm = take(sizeof(SmoothRec) * (m + kModeChunk));
   L.rough = take(sizeof(ModeRec) * m);
   L.bytes = off;
@@ -537,7 +537,7 @@ k_bin_scatter(const double* __restrict__ pos, int64_t ld, int64_t N, int64_t blk
 template <int D>
 __global__ void __launch_bounds__(256)
 k_mode_split(const ModeRec* __restrict__ modes, int M, BinParams* __restrict__ prm,
-             uint4* __restrict__ btpl, float* __restrict__ amp, SmoothRec* __restrict__ sm,
+             uint4* __restrict__ btpl, float4* __restrict__ amp, SmoothRec* __restrict__ sm,
              ModeRec* __restrict__ rough) {
   __shared__ int part[256];
   __shared__ int ns_s, nr_s;
@@ -553,7 +553,9 @@ k_mode_split(const ModeRec* __restrict__ modes, int M, BinParams* __restrict__ p
       double b = 0.0;
 #pragma unroll
       for (int k = 0; k < D; ++k) b += fabs(r.kap[k]) * 0.5 * P.h[k];
-      smooth = b <= kRoughRev && isfinite(r.th) && isfinite(static_cast<double>(r.amp[0]));
+      smooth = b <= kRoughRev && isfinite(r.th);
+#pragma unroll
+      for (int c = 0; c < D; ++c) smooth = smooth && isfinite(static_cast<double>(r.amp[c]));
     }
     part[threadIdx.x] = smooth ? 1 : 0;
     __syncthreads();
@@ -585,7 +587,7 @@ k_mode_split(const ModeRec* __restrict__ modes, int M, BinParams* __restrict__ p
 #pragma unroll
         for (int q = 0; q < 4; ++q)
           btpl[4 * i + q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
-        amp[i] = r.amp[0];
+        amp[i] = make_float4(r.amp[0], r.amp[1], D == 3 ? r.amp[2] : 0.0f, 0.0f);
         sm[i] = SmoothRec{{r.kap[0], r.kap[1], r.kap[2]}, r.th};
       } else {
         rough[nr0 + (threadIdx.x - before)] = r;
@@ -617,7 +619,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))
 k_direct_local(const BinParams* __restrict__ prm, const double* __restrict__ centre,
                const int* __restrict__ wave_cell,
                const float4* __restrict__ recs, const uint4* __restrict__ btpl,
-               const float* __restrict__ ampt, const SmoothRec* __restrict__ sm,
+               const float4* __restrict__ ampt, const SmoothRec* __restrict__ sm,
                const ModeRec* __restrict__ rough, const double* __restrict__ pos, int64_t ld,
                OutT* __restrict__ out, double scale) {
   // per wave: (r0, amplitude) of one chunk of modes, and afterwards the
@@ -683,7 +685,7 @@ k_direct_local(const BinParams* __restrict__ prm, const double* __restrict__ cen
         t = __builtin_fma(sr.kap[1], X[1], t);
         if constexpr (D == 3) t = __builtin_fma(sr.kap[2], X[2], t);
         r0 = static_cast<float>(t - rint(t));
-        am = ampt[i];
+        am = ampt[i].x;
       }
       rq[64 * r + lane] = f4{r0, r0, r0, r0};
       ram[64 * r + lane] = am;
@@ -770,6 +772,200 @@ k_direct_local(const BinParams* __restrict__ prm, const double* __restrict__ cen
   store(out + p, sum * scale);
 }
 
+// FP64-phase sum over the rough modes for the D components of one point.
+template <int D>
+__device__ __forceinline__ void sum_modes_vec(const ModeRec* __restrict__ modes, int M,
+                                              const double (&x)[D], double (&tot)[D]) {
+#pragma unroll
+  for (int c = 0; c < D; ++c) tot[c] = 0.0;
+  for (int m0 = 0; m0 < M; m0 += kChunk) {
+    const int m1 = min(m0 + kChunk, M);
+    float acc[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) acc[c] = 0.0f;
+    for (int m = m0; m < m1; ++m) {
+      const ModeRec md = modes[m];
+      double t = __builtin_fma(md.kap[0], x[0], md.th);
+      t = __builtin_fma(md.kap[1], x[1], t);
+      if constexpr (D > 2) t = __builtin_fma(md.kap[2], x[2], t);
+      const float sv = sin_rev(t);
+#pragma unroll
+      for (int c = 0; c < D; ++c) acc[c] = __builtin_fmaf(md.amp[c], sv, acc[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < D; ++c) tot[c] += static_cast<double>(acc[c]);
+  }
+}
+
+// Vector-valued form (incompressible fields): component c of point x is
+// sum_m amp_c(m) sin(2 pi t_m(x)), one sine per point-mode shared by the D
+// components.  The phases come from the same matrix-core product as in
+// k_direct_local, for 32 slots per wave (half a record group, two MFMA
+// groups), which leaves registers for D accumulator sets.  Wave w works on
+// slots 32 (w & 1) .. + 31 of record group w >> 1.
+constexpr int kVecChunk = 256;  // modes per (r0, amplitudes) table of a wave
+constexpr int kVecRedLd = 33;   // LDS row of the cross-lane sums (doubles)
+template <int D, typename OutT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4)))
+k_direct_local_vec(const BinParams* __restrict__ prm, const double* __restrict__ centre,
+                   const int* __restrict__ wave_cell, const float4* __restrict__ recs,
+                   const uint4* __restrict__ btpl, const float4* __restrict__ ampt,
+                   const SmoothRec* __restrict__ sm, const ModeRec* __restrict__ rough,
+                   const double* __restrict__ pos, int64_t ld, OutT* __restrict__ out,
+                   int64_t out_ld, double scale) {
+  constexpr int kWaveLds = kVecChunk * 32 > 16 * kVecRedLd * 8 ? kVecChunk * 32
+                                                                 : 16 * kVecRedLd * 8;
+  __shared__ __attribute__((aligned(16))) char lds[4][kWaveLds];
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int w = blockIdx.x * 4 + wv;
+  const BinParams P = *prm;
+  if (w >= 2 * P.nwaves) return;
+  const int n_smooth = P.n_smooth, n_rough = P.n_rough;
+  const int rg = w >> 1;
+  double X[3];
+  {
+    const double* xc = centre + 4 * static_cast<int64_t>(wave_cell[rg]);
+    X[0] = xc[0];
+    X[1] = xc[1];
+    X[2] = xc[2];
+  }
+  const int kg = lane >> 4, j = lane & 15;
+  const float* rf =
+      reinterpret_cast<const float*>(recs + static_cast<int64_t>(rg) * kWavePts + 32 * (w & 1));
+  bf8 a[2];
+#pragma unroll
+  for (int g = 0; g < 2; ++g) {
+    uint16_t xp[3][3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) split3(d < D ? rf[(16 * g + j) * 4 + d] : 0.0f, xp[d]);
+    uint16_t sl[32];
+#pragma unroll
+    for (int k = 0; k < 32; ++k) sl[k] = k < 6 * D ? xp[k / 6][kXPart[k % 6]] : 0;
+    uint16_t v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+      v[e] = kg == 0 ? sl[e] : (kg == 1 ? sl[8 + e] : (kg == 2 ? sl[16 + e] : sl[24 + e]));
+    a[g] = __builtin_bit_cast(bf8, v);
+  }
+  f4* rq = reinterpret_cast<f4*>(lds[wv]);
+  f4* ram = rq + kVecChunk;
+  float acc[D][2][4], tot[D][2][4];
+#pragma unroll
+  for (int c = 0; c < D; ++c)
+#pragma unroll
+    for (int g = 0; g < 2; ++g)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        acc[c][g][v] = 0.0f;
+        tot[c][g][v] = 0.0f;
+      }
+  for (int c0 = 0; c0 < n_smooth; c0 += kVecChunk) {
+#pragma unroll 2
+    for (int r = 0; r < kVecChunk / 64; ++r) {
+      const int i = c0 + 64 * r + lane;
+      float r0 = 0.0f;
+      float4 am = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (i < n_smooth) {
+        const SmoothRec sr = sm[i];
+        double t = __builtin_fma(sr.kap[0], X[0], sr.th);
+        t = __builtin_fma(sr.kap[1], X[1], t);
+        if constexpr (D == 3) t = __builtin_fma(sr.kap[2], X[2], t);
+        r0 = static_cast<float>(t - rint(t));
+        am = ampt[i];
+      }
+      rq[64 * r + lane] = f4{r0, r0, r0, r0};
+      ram[64 * r + lane] = f4{am.x, am.y, am.z, 0.0f};
+    }
+    const int ntile = min(kVecChunk / 16, (n_smooth - c0 + 15) / 16);
+    const uint4* bp = btpl + 4 * (c0 + j) + kg;
+    const f4* rp = rq + j;
+    const f4* mp = ram + j;
+    uint4 ba = bp[0], bb = bp[64];
+    bp += 128;
+    f4 ta[2], tb[2];
+    f4 ama, amb;
+    auto issue = [&](f4 (&t)[2], f4& am, uint4& bq) {
+      const f4 cq = *rp;
+      am = *mp;
+      rp += 16;
+      mp += 16;
+#pragma unroll
+      for (int g = 0; g < 2; ++g)
+        t[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[g], __builtin_bit_cast(bf8, bq), cq, 0, 0,
+                                                       0);
+      bq = *bp;
+      bp += 64;
+    };
+    auto consume = [&](const f4 (&t)[2], const f4& am) {
+#pragma unroll
+      for (int g = 0; g < 2; ++g)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const float sv = __builtin_amdgcn_sinf(t[g][v]);
+#pragma unroll
+          for (int c = 0; c < D; ++c) acc[c][g][v] = __builtin_fmaf(am[c], sv, acc[c][g][v]);
+        }
+    };
+    issue(ta, ama, ba);
+    int q = 0;
+    for (; q + 2 < ntile; q += 2) {
+      issue(tb, amb, bb);
+      consume(ta, ama);
+      issue(ta, ama, ba);
+      consume(tb, amb);
+    }
+    if (q + 1 < ntile) {
+      issue(tb, amb, bb);
+      consume(ta, ama);
+      consume(tb, amb);
+    } else {
+      consume(ta, ama);
+    }
+#pragma unroll
+    for (int c = 0; c < D; ++c)
+#pragma unroll
+      for (int g = 0; g < 2; ++g)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          tot[c][g][v] += acc[c][g][v];
+          acc[c][g][v] = 0.0f;
+        }
+  }
+  // cross-lane sums, one component at a time: tot[c][g][v] is slot
+  // 16 g + 4 (lane / 16) + v of the half, mode column j
+  const int p = lane < 32 ? __builtin_bit_cast(int, rf[lane * 4 + 3]) : -1;
+  double comp[D];
+  double* red = reinterpret_cast<double*>(lds[wv]);
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    __asm__ volatile("" ::: "memory");
+#pragma unroll
+    for (int g = 0; g < 2; ++g)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        red[j * kVecRedLd + 16 * g + 4 * kg + v] = static_cast<double>(tot[c][g][v]);
+    double sum = 0.0;
+    if (lane < 32)
+#pragma unroll
+      for (int jj = 0; jj < 16; ++jj) sum += red[jj * kVecRedLd + lane];
+    comp[c] = sum;
+    __asm__ volatile("" ::: "memory");
+  }
+  if (p < 0) return;
+  if (n_rough > 0) {
+    double x[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) x[k] = pos[k * ld + p];
+    double rt[D];
+    sum_modes_vec<D>(rough, n_rough, x, rt);
+#pragma unroll
+    for (int c = 0; c < D; ++c) comp[c] += rt[c];
+  }
+#pragma unroll
+  for (int c = 0; c < D; ++c) store(out + c * out_ld + p, comp[c] * scale);
+}
+
 }  // namespace
 
 size_t binned_work_bytes(int dim, int64_t M, int64_t N) {
@@ -783,8 +979,9 @@ bool binned_supported(int dim, int64_t N) {
 }
 
 hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table,
-                                const double* pos, int64_t pos_ld, void* out, int out_dtype,
-                                double scale, void* bin_work, hipStream_t s) {
+                                const double* pos, int64_t pos_ld, void* out, int64_t out_ld,
+                                bool vec, int out_dtype, double scale, void* bin_work,
+                                hipStream_t s) {
   if (N <= 0) return hipSuccess;
   const BinLayout L = bin_layout(M, N);
   BinParams* prm = at<BinParams>(bin_work, L.params);
@@ -798,7 +995,7 @@ hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table
   float4* recs = at<float4>(bin_work, L.recs);
 
   uint4* btpl = at<uint4>(bin_work, L.btpl);
-  float* ampt = at<float>(bin_work, L.amp);
+  float4* ampt = at<float4>(bin_work, L.amp);
   SmoothRec* sm = at<SmoothRec>(bin_work, L.sm);
   ModeRec* rough = at<ModeRec>(bin_work, L.rough);
   const ModeRec* modes = static_cast<const ModeRec*>(table);
@@ -806,6 +1003,7 @@ hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table
   const int ngroups = (nblk + kColBlocks - 1) / kColBlocks;
   const unsigned cb = static_cast<unsigned>((L.max_cells + 255) / 256);
   const unsigned mb = static_cast<unsigned>((L.max_waves + 3) / 4);
+  const unsigned vb = static_cast<unsigned>((2 * L.max_waves + 3) / 4);
   const int Mi = static_cast<int>(M);
   const bool f32 = out_dtype == 1;
 #define GSC_BIN(D)                                                                               \
@@ -821,14 +1019,22 @@ hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table
     k_bin_scatter<D><<<nblk, kSortThreads, 0, s>>>(pos, pos_ld, N, L.blk_pts, prm, bcount,       \
                                                    offsets, recs);                               \
     k_mode_split<D><<<1, 256, 0, s>>>(modes, Mi, prm, btpl, ampt, sm, rough);                    \
-    if (f32)                                                                                     \
+    if (vec && f32)                                                                              \
+      k_direct_local_vec<D, float><<<vb, 256, 0, s>>>(prm, centre, wave_cell, recs, btpl, ampt,  \
+                                                      sm, rough, pos, pos_ld,                    \
+                                                      static_cast<float*>(out), out_ld, scale);  \
+    else if (vec)                                                                                \
+      k_direct_local_vec<D, double><<<vb, 256, 0, s>>>(prm, centre, wave_cell, recs, btpl, ampt, \
+                                                       sm, rough, pos, pos_ld,                   \
+                                                       static_cast<double*>(out), out_ld,        \
+                                                       scale);                                   \
+    else if (f32)                                                                                \
       k_direct_local<D, float><<<mb, 256, 0, s>>>(prm, centre, wave_cell, recs, btpl, ampt, sm,  \
-                                                  rough,                                         \
-                                                  pos, pos_ld, static_cast<float*>(out), scale); \
+                                                  rough, pos, pos_ld, static_cast<float*>(out),  \
+                                                  scale);                                        \
     else                                                                                         \
       k_direct_local<D, double><<<mb, 256, 0, s>>>(prm, centre, wave_cell, recs, btpl, ampt, sm, \
-                                                   rough,                                        \
-                                                   pos, pos_ld, static_cast<double*>(out),       \
+                                                   rough, pos, pos_ld, static_cast<double*>(out),\
                                                    scale);                                       \
   } while (0)
   if (dim == 2) GSC_BIN(2);

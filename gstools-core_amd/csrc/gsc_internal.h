@@ -51,11 +51,14 @@ hipError_t launch_direct_incompr(int dim, int64_t M, int64_t N, const void* tabl
 
 // Local-phase ("binned") direct kernel, dim 2 and 3 (direct_local.hip): a
 // spatial counting sort of the points, then phases kap . (x - X_cell) + r0 on
-// the FP32 matrix cores.  bin_work: binned_work_bytes(dim, M, N) bytes.
+// the matrix cores.  bin_work: binned_work_bytes(dim, M, N) bytes.
+// vec = false: out[j] (the scalar and Fourier summators, amp[0] per mode);
+// vec = true: out[c * out_ld + j] for c < dim (incompressible, amp[c]).
 bool binned_supported(int dim, int64_t N);
 size_t binned_work_bytes(int dim, int64_t M, int64_t N);
 hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table,
-                                const double* pos, int64_t pos_ld, void* out, int out_dtype,
-                                double scale, void* bin_work, hipStream_t stream);
+                                const double* pos, int64_t pos_ld, void* out, int64_t out_ld,
+                                bool vec, int out_dtype, double scale, void* bin_work,
+                                hipStream_t stream);
 
 }  // namespace gsc

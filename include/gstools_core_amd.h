@@ -79,12 +79,12 @@ typedef enum gsc_accum {
 /* How the direct (per-point) kernels evaluate the phases k . x.
  * LOCAL sorts the points into spatial cells and splits every phase at the
  * cell centre X: (k . X + theta) in FP64 once per cell and mode, k . (x - X)
- * on the FP32 matrix cores, which run beside the VALU sine work; modes whose
- * local phase could exceed one revolution keep the FP64 phase.  It serves
- * gsc_summate and gsc_summate_fourier in 2-D and 3-D (N < 2^30); elsewhere
- * (incompressible fields, other dimensions) LOCAL runs the VALU kernel.
- * On gfx950 the FP64 matrix tiles of MFMA do not overlap with the VALU sine
- * work (the measured time is the sum of both parts). */
+ * from three-part bf16 splits on the matrix cores (v_mfma_f32_16x16x32_bf16,
+ * 256 point-modes per instruction); modes whose local phase could exceed one
+ * revolution keep the FP64 phase.  It serves all three summators in 2-D and
+ * 3-D (N < 2^30); in other dimensions LOCAL runs the VALU kernel.
+ * On gfx950 matrix instructions do not overlap with the VALU sine work (the
+ * measured time is the sum of both parts). */
 typedef enum gsc_direct_variant {
   GSC_DIRECT_MFMA = 0,  /* v_mfma_f64_16x16x4 tiles of 16 modes x 16 points (dim <= 3) */
   GSC_DIRECT_VALU = 1,  /* FP64 FMA chains on the vector ALUs */

@@ -236,7 +236,7 @@ def test_committed_oracle_vectors():
 @pytest.mark.parametrize("variant", ["mfma", "valu", "local"])
 def test_direct_variants_against_oracle(variant):
     # every phase engine (matrix-core FP64 tiles, VALU FP64 FMA chains, cell-local
-    # FP32 matrix-core phases; incompr runs the VALU kernel for "local")
+    # split-bf16 matrix-core phases), scalar, incompressible and Fourier
     for d, m, n, seed in ((1, 300, 3001, 1), (2, 1000, 20_000, 2), (3, 1000, 20_000, 3),
                           (3, 17, 4099, 4), (2, 1, 77, 5)):
         rng = np.random.default_rng(500 + seed)
@@ -267,6 +267,12 @@ def test_direct_variants_agree_at_scale():
     err = _err(c, b) * md.scale
     print(f"local vs valu at 2e6 points: {err:.3e} sigma")
     assert err <= 5e-6
+    wi = workloads.c2(n=1_000_000)
+    bi = gstools_core.summate_incompr(md.cov_samples, md.z1, md.z2, wi.pos, variant="valu")
+    ci = gstools_core.summate_incompr(md.cov_samples, md.z1, md.z2, wi.pos, variant="local")
+    err = _err(ci, bi) * md.scale
+    print(f"incompr local vs valu at 1e6 points: {err:.3e} sigma")
+    assert err <= 5e-6
 
 
 # -- the cell-local (binned) kernel ---------------------------------------------
@@ -283,6 +289,8 @@ def test_local_partial_waves(n):
         pos = rng.uniform(-50, 250, size=(d, n))
         g = gstools_core.summate(cov, z1, z2, pos, variant="local")
         assert _err(g, oracle.summate(cov, z1, z2, pos)) <= sigma_tol(300)
+        gi = gstools_core.summate_incompr(cov, z1, z2, pos, variant="local")
+        assert _err(gi, oracle.summate_incompr(cov, z1, z2, pos)) <= sigma_tol(300)
 
 
 def test_local_degenerate_point_sets():
@@ -316,6 +324,10 @@ def test_local_non_finite_points_only_affect_themselves():
     assert np.isnan(g[bad]).all() and np.isnan(ref[bad]).all()
     ok = np.setdiff1d(np.arange(5000), bad)
     assert _err(g[ok], ref[ok]) <= sigma_tol(200)
+    gi = gstools_core.summate_incompr(cov, z1, z2, pos, variant="local")
+    refi = oracle.summate_incompr(cov, z1, z2, pos)
+    assert np.isnan(gi[:, bad]).all() and np.isnan(refi[:, bad]).all()
+    assert _err(gi[:, ok], refi[:, ok]) <= sigma_tol(200)
 
 
 def test_local_is_bitwise_reproducible_and_order_free():
@@ -329,6 +341,9 @@ def test_local_is_bitwise_reproducible_and_order_free():
     perm = np.random.default_rng(33).permutation(w.n_points)
     c = gstools_core.summate(md.cov_samples, md.z1, md.z2, w.pos[:, perm])
     np.testing.assert_array_equal(c, a[perm])
+    ai = gstools_core.summate_incompr(md.cov_samples, md.z1, md.z2, w.pos)
+    ci = gstools_core.summate_incompr(md.cov_samples, md.z1, md.z2, w.pos[:, perm])
+    np.testing.assert_array_equal(ci, ai[:, perm])
 
 
 def test_local_rough_modes_keep_fp64_phase():
@@ -343,4 +358,8 @@ def test_local_rough_modes_keep_fp64_phase():
     g = gstools_core.summate(cov, z1, z2, pos, variant="local")
     err = _err(g, oracle.summate(cov, z1, z2, pos)) / np.sqrt(m)
     print(f"mixed rough/smooth: {err:.3e}")
+    assert err <= 1e-5
+    gi = gstools_core.summate_incompr(cov, z1, z2, pos, variant="local")
+    err = _err(gi, oracle.summate_incompr(cov, z1, z2, pos)) / np.sqrt(m)
+    print(f"mixed rough/smooth, incompressible: {err:.3e}")
     assert err <= 1e-5
