@@ -121,6 +121,57 @@ def grid_leg(steps: int, warmup: int) -> dict:
     return res
 
 
+def c5_leg(steps: int, warmup: int) -> dict:
+    """Heavy-tailed 512^3 grid, M = 4000 exponential-model modes (workload c5),
+    FP32 output: the factored path per operand format vs the direct kernel on
+    the grid.  Accuracy against the FP64 oracle is checked by the GPU tests
+    (tests/test_gpu_grid.py), not here."""
+    import numpy as np
+    import torch
+
+    from gstools_core import grid, workloads
+
+    w = workloads.c5()
+    md = w.modes
+    dev = torch.device("cuda:0")
+    cov = torch.from_numpy(np.ascontiguousarray(md.cov_samples)).to(dev)
+    z1 = torch.from_numpy(md.z1).to(dev)
+    z2 = torch.from_numpy(md.z2).to(dev)
+    axes = [torch.from_numpy(a).to(dev) for a in w.axes]
+    shape = [len(a) for a in w.axes]
+    n = w.n_points
+    out = torch.empty(n, dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream()
+    res = {"workload": f"3-D grid {shape[0]}x{shape[1]}x{shape[2]}, M={md.m} exponential-model "
+                       "modes (multivariate t, 1 dof, l=8), FP32 output",
+           "n_points": n, "n_modes": md.m}
+    for path, prec in (("factored", "fp16x3"), ("factored", "bf16x3"), ("factored", "bf16x6"),
+                       ("direct", "auto")):
+        nb = grid.grid_workspace_bytes(3, md.m, shape, path=path, precision=prec)
+        work = torch.empty(nb, dtype=torch.uint8, device=dev)
+
+        def step():
+            grid.summate_grid_dev(3, md.m, cov.data_ptr(), z1.data_ptr(), z2.data_ptr(),
+                                  [a.data_ptr() for a in axes], shape, out.data_ptr(),
+                                  work.data_ptr(), nb, path=path, precision=prec, out_f32=True,
+                                  stream=stream.cuda_stream, accum="f32x2")
+
+        st = steps if path == "factored" else 2
+        _, total = time_steps(step, st, warmup, stream)
+        ms = total / st
+        key = f"factored_{prec}" if path == "factored" else "direct"
+        res[key] = {"ms_per_step": ms, "point_modes_per_s": n * md.m / (ms * 1e-3)}
+        if path == "factored":
+            nprod = {"fp16x3": 3, "bf16x3": 3, "bf16x6": 6}[prec]
+            tflops = 4.0 * n * md.m * nprod / (ms * 1e-3) / 1e12
+            res[key]["mfma_tflops"] = tflops
+            res[key]["mfma_frac_of_2500"] = tflops / 2500.0
+        del work
+    res["factored_fp16x3_over_direct"] = (res["direct"]["ms_per_step"] /
+                                          res["factored_fp16x3"]["ms_per_step"])
+    return res
+
+
 def ensemble_leg(steps: int, warmup: int) -> dict:
     """512 seeded 2-D realisations (workload c4) as one graph-captured plan."""
     import numpy as np
@@ -229,8 +280,8 @@ def main() -> int:
     ap.add_argument("--m", type=int, default=1000)
     ap.add_argument("--no-grid", action="store_true", help="skip the structured-grid leg")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble leg")
-    ap.add_argument("--legs", default="direct,incompr,grid,ensemble",
-                    help="comma list of legs: direct, incompr, grid, ensemble, copy (a 1 GiB "
+    ap.add_argument("--legs", default="direct,incompr,grid,c5,ensemble",
+                    help="comma list of legs: direct, incompr, grid, c5, ensemble, copy (a 1 GiB "
                          "device copy, the reference for HBM byte counters)")
     args = ap.parse_args()
     legs = set(args.legs.split(","))
@@ -264,6 +315,8 @@ def main() -> int:
             res["incompr"] = incompr_leg(max(3, args.steps // 2), 1)
         if "grid" in legs:
             res["structured_grid"] = grid_leg(args.steps, args.warmup)
+        if "c5" in legs:
+            res["heavy_tail_grid"] = c5_leg(max(3, args.steps // 4), 1)
         if "ensemble" in legs:
             res["ensemble"] = ensemble_leg(max(3, args.steps // 2), 1)
         print(json.dumps(res), flush=True)
@@ -345,6 +398,11 @@ def main() -> int:
         log("grid leg (256^3, factored vs direct)")
         grid_res = grid_leg(args.steps, args.warmup)
 
+    c5_res = None
+    if "c5" in legs:
+        log("heavy-tailed grid leg (512^3, M = 4000)")
+        c5_res = c5_leg(max(3, args.steps // 4), 1)
+
     ens_res = None
     if "ensemble" in legs:
         log("ensemble leg (512 realisations)")
@@ -388,6 +446,7 @@ def main() -> int:
         "direct_variants": direct_variants,
         "incompr": inc_res,
         "structured_grid": grid_res,
+        "heavy_tail_grid": c5_res,
         "ensemble": ens_res,
     }
     print(json.dumps(line), flush=True)

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -580,6 +581,9 @@ struct gsc_ensemble {
   int device = 0;
   void* mem = nullptr;
   size_t mem_bytes = 0;
+  void* mode_mem = nullptr;   // seeded plans: modes, mode offsets, scales, seeds
+  uint64_t* d_seeds = nullptr;
+  int count = 0;
   hipStream_t cap = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -595,13 +599,16 @@ void ensemble_release(gsc_ensemble* p) {
   if (p->graph) (void)hipGraphDestroy(p->graph);
   if (p->cap) (void)hipStreamDestroy(p->cap);
   if (p->mem) (void)hipFree(p->mem);
+  if (p->mode_mem) (void)hipFree(p->mode_mem);
   delete p;
 }
 
+// sample: null, or the mode sampler that fills cov / z1 / z2 at the start
+// of every replay (seeded plans).
 int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes,
                    const int64_t* m_counts, const int32_t* paths, const double* scales,
                    const double* cov, const double* z1, const double* z2, const double* axes,
-                   void* out, const gsc_opts* o) {
+                   void* out, const gsc_opts* o, const gsc::SampleArgs* sample) {
   const size_t elt = o->out_dtype == GSC_F32 ? 4 : 8;
   std::vector<gsc::EnsReal> rs((size_t)count);
   std::vector<int> fact, dir, group_len;
@@ -783,7 +790,9 @@ int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes
 
   GSC_HIP(hipStreamCreateWithFlags(&plan->cap, hipStreamNonBlocking));
   GSC_HIP(hipStreamBeginCapture(plan->cap, hipStreamCaptureModeThreadLocal));
-  const hipError_t le = gsc::launch_ensemble(el, o->precision, o->out_dtype, o->accum, plan->cap);
+  hipError_t le = sample ? gsc::launch_sample_modes(*sample, plan->cap) : hipSuccess;
+  if (le == hipSuccess)
+    le = gsc::launch_ensemble(el, o->precision, o->out_dtype, o->accum, plan->cap);
   hipGraph_t g = nullptr;
   const hipError_t ce = hipStreamEndCapture(plan->cap, &g);
   plan->graph = g;
@@ -793,14 +802,29 @@ int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes
   return GSC_OK;
 }
 
-}  // namespace
+struct SeededModes {
+  int model;
+  const double* len_scales;  // host, count
+  const uint64_t* seeds;     // host, count
+};
 
-extern "C" {
+int validate_sample(int model, int dim, int64_t M, double len_scale) {
+  if (model != GSC_MODEL_GAUSSIAN && model != GSC_MODEL_EXPONENTIAL)
+    return fail(GSC_ERR_INVALID, "unknown spectral model %d", model);
+  if (dim < 1 || dim > 3) return fail(GSC_ERR_DIM, "mode sampling supports dim 1..3 (got %d)", dim);
+  if (M < 0 || M > ((int64_t)1 << 40)) return fail(GSC_ERR_INVALID, "bad mode count");
+  if (!(len_scale > 0.0) || !std::isfinite(len_scale))
+    return fail(GSC_ERR_INVALID, "len_scale must be positive and finite");
+  return GSC_OK;
+}
 
-int gsc_ensemble_create(gsc_ensemble** plan, int count, int dim, const int64_t* shapes,
-                        const int64_t* m_counts, const int32_t* paths, const double* scales,
-                        const double* cov, const double* z1, const double* z2,
-                        const double* axes, void* out, const gsc_opts* opts) {
+// Shared body of the two creators; seeded: null (caller's mode arrays) or the
+// sampler description (the plan owns the mode arrays and draws them in its
+// graph).
+int ensemble_create(gsc_ensemble** plan, int count, int dim, const int64_t* shapes,
+                    const int64_t* m_counts, const int32_t* paths, const double* scales,
+                    const double* cov, const double* z1, const double* z2, const double* axes,
+                    void* out, const gsc_opts* opts, const SeededModes* seeded) {
   if (!plan) return fail(GSC_ERR_INVALID, "null plan pointer");
   *plan = nullptr;
   gsc_opts tmp;
@@ -823,11 +847,13 @@ int gsc_ensemble_create(gsc_ensemble** plan, int count, int dim, const int64_t* 
     m_total += m_counts[r];
     pts += grid_points(dim, shapes + (int64_t)r * dim);
   }
-  if (m_total > 0 && (!cov || !z1 || !z2)) return fail(GSC_ERR_INVALID, "null mode arrays");
+  if (!seeded && m_total > 0 && (!cov || !z1 || !z2))
+    return fail(GSC_ERR_INVALID, "null mode arrays");
   if (pts > 0 && (!axes || !out)) return fail(GSC_ERR_INVALID, "null axes or output");
   gsc_ensemble* p = nullptr;
   try {
     p = new gsc_ensemble();
+    p->count = count;
     if ((rc = resolve_device(o, &p->device))) {
       delete p;
       return rc;
@@ -840,7 +866,55 @@ int gsc_ensemble_create(gsc_ensemble** plan, int count, int dim, const int64_t* 
       delete p;
       return rc;
     }
-    rc = ensemble_build(p, count, dim, shapes, m_counts, paths, scales, cov, z1, z2, axes, out, o);
+    gsc::SampleArgs sa{};
+    if (seeded) {
+      // plan-owned modes (dim + 2 doubles per mode), then m_off, len_scale, seeds
+      const size_t b_modes = align256(sizeof(double) * (size_t)(dim + 2) * (size_t)(m_total + 1));
+      const size_t b_off = align256(sizeof(int64_t) * (size_t)(count + 1));
+      const size_t b_len = align256(sizeof(double) * (size_t)count);
+      const size_t b_seed = align256(sizeof(uint64_t) * (size_t)count);
+      if (hipMalloc(&p->mode_mem, b_modes + b_off + b_len + b_seed) != hipSuccess) {
+        p->mode_mem = nullptr;
+        rc = fail(GSC_ERR_ALLOC, "hipMalloc for the ensemble modes failed");
+      } else {
+        char* mb = static_cast<char*>(p->mode_mem);
+        double* d_cov = reinterpret_cast<double*>(mb);
+        int64_t* d_off = reinterpret_cast<int64_t*>(mb + b_modes);
+        double* d_len = reinterpret_cast<double*>(mb + b_modes + b_off);
+        p->d_seeds = reinterpret_cast<uint64_t*>(mb + b_modes + b_off + b_len);
+        std::vector<int64_t> off((size_t)count + 1, 0);
+        for (int r = 0; r < count; ++r) off[(size_t)r + 1] = off[(size_t)r] + m_counts[r];
+        hipError_t e = hipMemcpy(d_off, off.data(), sizeof(int64_t) * off.size(),
+                                 hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+          e = hipMemcpy(d_len, seeded->len_scales, sizeof(double) * (size_t)count,
+                        hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+          e = hipMemcpy(p->d_seeds, seeded->seeds, sizeof(uint64_t) * (size_t)count,
+                        hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+          rc = fail(GSC_ERR_HIP, "ensemble mode upload: %s", hipGetErrorString(e));
+        } else {
+          sa.model = seeded->model;
+          sa.dim = dim;
+          sa.count = count;
+          sa.m_off = d_off;
+          sa.len_scale = d_len;
+          sa.seeds = p->d_seeds;
+          sa.m_total = m_total;
+          sa.stream_id = 0;
+          sa.cov = d_cov;
+          sa.z1 = d_cov + dim * m_total;
+          sa.z2 = d_cov + (dim + 1) * m_total;
+          cov = sa.cov;
+          z1 = sa.z1;
+          z2 = sa.z2;
+        }
+      }
+    }
+    if (rc == GSC_OK)
+      rc = ensemble_build(p, count, dim, shapes, m_counts, paths, scales, cov, z1, z2, axes, out,
+                          o, seeded ? &sa : nullptr);
   } catch (const std::bad_alloc&) {
     rc = fail(GSC_ERR_ALLOC, "host allocation failed");
   }
@@ -849,6 +923,108 @@ int gsc_ensemble_create(gsc_ensemble** plan, int count, int dim, const int64_t* 
     return rc;
   }
   *plan = p;
+  return GSC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gsc_ensemble_create(gsc_ensemble** plan, int count, int dim, const int64_t* shapes,
+                        const int64_t* m_counts, const int32_t* paths, const double* scales,
+                        const double* cov, const double* z1, const double* z2,
+                        const double* axes, void* out, const gsc_opts* opts) {
+  return ensemble_create(plan, count, dim, shapes, m_counts, paths, scales, cov, z1, z2, axes,
+                         out, opts, nullptr);
+}
+
+int gsc_ensemble_create_seeded(gsc_ensemble** plan, int count, int dim, const int64_t* shapes,
+                               const int64_t* m_counts, const int32_t* paths,
+                               const double* scales, int model, const double* len_scales,
+                               const uint64_t* seeds, const double* axes, void* out,
+                               const gsc_opts* opts) {
+  if (model != GSC_MODEL_GAUSSIAN && model != GSC_MODEL_EXPONENTIAL)
+    return fail(GSC_ERR_INVALID, "unknown spectral model %d", model);
+  if (!len_scales || !seeds) return fail(GSC_ERR_INVALID, "null len_scales or seeds");
+  if (count < 1 || count > 20000 || !m_counts)
+    return fail(GSC_ERR_INVALID, "count must be in 1..20000 with m_counts given");
+  for (int r = 0; r < count; ++r)
+    if (!(len_scales[r] > 0.0) || !std::isfinite(len_scales[r]))
+      return fail(GSC_ERR_INVALID, "realisation %d: len_scale must be positive and finite", r);
+  SeededModes sm{model, len_scales, seeds};
+  return ensemble_create(plan, count, dim, shapes, m_counts, paths, scales, nullptr, nullptr,
+                         nullptr, axes, out, opts, &sm);
+}
+
+int gsc_ensemble_reseed(gsc_ensemble* plan, const uint64_t* seeds, void* stream) {
+  if (!plan || !plan->exec) return fail(GSC_ERR_INVALID, "invalid ensemble plan");
+  if (!plan->d_seeds) return fail(GSC_ERR_INVALID, "the plan was not created from seeds");
+  if (!seeds) return fail(GSC_ERR_INVALID, "null seeds");
+  GSC_HIP(hipMemcpyAsync(plan->d_seeds, seeds, sizeof(uint64_t) * (size_t)plan->count,
+                         hipMemcpyHostToDevice, static_cast<hipStream_t>(stream)));
+  return GSC_OK;
+}
+
+int gsc_sample_modes_dev(int model, int dim, int64_t M, double len_scale, uint64_t seed,
+                         uint64_t stream_id, double* cov, double* z1, double* z2,
+                         const gsc_opts* opts, void* stream) {
+  int rc = validate_sample(model, dim, M, len_scale);
+  if (rc) return rc;
+  gsc_opts tmp;
+  const gsc_opts* o = opts_or_default(opts, &tmp);
+  if ((rc = validate_opts(o))) return rc;
+  if (M == 0) return GSC_OK;
+  if (!cov || !z1 || !z2) return fail(GSC_ERR_INVALID, "null output arrays");
+  gsc::SampleArgs a{};
+  a.model = model;
+  a.dim = dim;
+  a.count = 1;
+  a.m_total = M;
+  a.len_scale1 = len_scale;
+  a.seed1 = seed;
+  a.stream_id = stream_id;
+  a.cov = cov;
+  a.z1 = z1;
+  a.z2 = z2;
+  GSC_HIP(gsc::launch_sample_modes(a, static_cast<hipStream_t>(stream)));
+  return GSC_OK;
+}
+
+int gsc_sample_modes(int model, int dim, int64_t M, double len_scale, uint64_t seed,
+                     uint64_t stream_id, double* cov, double* z1, double* z2,
+                     const gsc_opts* opts) {
+  int rc = validate_sample(model, dim, M, len_scale);
+  if (rc) return rc;
+  gsc_opts tmp;
+  const gsc_opts* o = opts_or_default(opts, &tmp);
+  if ((rc = validate_opts(o))) return rc;
+  if (M == 0) return GSC_OK;
+  if (!cov || !z1 || !z2) return fail(GSC_ERR_INVALID, "null output arrays");
+  int dev = 0;
+  if ((rc = resolve_device(o, &dev))) return rc;
+  GSC_HIP(hipSetDevice(dev));
+  if ((rc = check_arch(dev))) return rc;
+  const size_t n = static_cast<size_t>(dim + 2) * static_cast<size_t>(M);
+  double* d = nullptr;
+  if (hipMalloc(&d, sizeof(double) * n) != hipSuccess)
+    return fail(GSC_ERR_ALLOC, "hipMalloc of %zu bytes failed", sizeof(double) * n);
+  gsc::SampleArgs a{};
+  a.model = model;
+  a.dim = dim;
+  a.count = 1;
+  a.m_total = M;
+  a.len_scale1 = len_scale;
+  a.seed1 = seed;
+  a.stream_id = stream_id;
+  a.cov = d;
+  a.z1 = d + dim * M;
+  a.z2 = d + (dim + 1) * M;
+  hipError_t e = gsc::launch_sample_modes(a, nullptr);
+  if (e == hipSuccess) e = hipMemcpy(cov, a.cov, sizeof(double) * dim * M, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(z1, a.z1, sizeof(double) * M, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(z2, a.z2, sizeof(double) * M, hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(GSC_ERR_HIP, "mode sampling failed: %s", hipGetErrorString(e));
   return GSC_OK;
 }
 

@@ -61,4 +61,25 @@ hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table
                                 bool vec, int out_dtype, double scale, void* bin_work,
                                 hipStream_t stream);
 
+// Seeded spectral-mode sampling (sampling.hip): Philox4x64-10 words of key
+// (seed, stream_id), Box-Muller normals, k = g / l (model 0, Gaussian) or
+// g / (l |w|) (model 1, exponential), z1, z2 ~ N(0, 1).
+struct SampleArgs {
+  int model;
+  int dim;
+  int count;               // realisations described by m_off / len_scale / seeds
+  const int64_t* m_off;    // device, count + 1 prefix sums of the mode counts; null: one
+                           // realisation of m_total modes with len_scale1, seed1
+  const double* len_scale; // device, count
+  const uint64_t* seeds;   // device, count
+  int64_t m_total;
+  double len_scale1;
+  uint64_t seed1;
+  uint64_t stream_id;
+  double* cov;             // (dim, M_r) row-major blocks back to back
+  double* z1;
+  double* z2;
+};
+hipError_t launch_sample_modes(const SampleArgs& a, hipStream_t stream);
+
 }  // namespace gsc

@@ -86,6 +86,16 @@ SIGNATURES = {
     "gsc_ensemble_info": (_int, [_vp, _i64p, _i64p, ctypes.POINTER(_sz),
                                  ctypes.POINTER(ctypes.c_double)]),
     "gsc_ensemble_destroy": (None, [_vp]),
+    "gsc_sample_modes": (_int, [_int, _int, _i64, ctypes.c_double, ctypes.c_uint64,
+                                ctypes.c_uint64, _vp, _vp, _vp, _optp]),
+    "gsc_sample_modes_dev": (_int, [_int, _int, _i64, ctypes.c_double, ctypes.c_uint64,
+                                    ctypes.c_uint64, _vp, _vp, _vp, _optp, _vp]),
+    "gsc_ensemble_create_seeded": (_int, [ctypes.POINTER(_vp), _int, _int, _i64p, _i64p,
+                                          ctypes.POINTER(ctypes.c_int32),
+                                          ctypes.POINTER(ctypes.c_double), _int,
+                                          ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_uint64), _vp, _vp, _optp]),
+    "gsc_ensemble_reseed": (_int, [_vp, ctypes.POINTER(ctypes.c_uint64), _vp]),
     "gsc_device_alloc": (_vp, [_sz, _int]),
     "gsc_device_free": (None, [_vp]),
     "gsc_memcpy_to_device": (_int, [_vp, _vp, _sz]),

@@ -27,6 +27,8 @@ class Ensemble:
 
     realisations: sequence of (cov_samples (d, M_r), z1 (M_r), z2 (M_r), axes)
     with axes a list of d 1-D coordinate arrays; all realisations share d (2 or 3).
+    ``Ensemble.seeded`` builds a plan whose modes are drawn on the device from
+    per-realisation seeds (gstools_core.sampling) at the start of every run.
     """
 
     def __init__(self, realisations: Sequence, *, precision: str = "auto",
@@ -35,29 +37,92 @@ class Ensemble:
                  device: int = -1, blocks_per_cu: int = 0):
         if len(realisations) == 0:
             raise ValueError("an ensemble needs at least one realisation")
-        dims = {len(r[3]) for r in realisations}
+        covs, z1s, z2s, grids, ms = [], [], [], [], []
+        for cov, z1, z2, axes in realisations:
+            cov = np.ascontiguousarray(cov, dtype=np.float64)
+            if cov.ndim != 2 or cov.shape[1] != len(z1) or cov.shape[1] != len(z2):
+                raise ValueError("cov_samples must be (d, M) with len(z1) == len(z2) == M")
+            if cov.shape[0] != len(axes):
+                raise ValueError("cov_samples must have one row per axis")
+            covs.append(cov.ravel())
+            z1s.append(np.asarray(z1, dtype=np.float64))
+            z2s.append(np.asarray(z2, dtype=np.float64))
+            grids.append(axes)
+            ms.append(cov.shape[1])
+        args = self._setup(grids, ms, precision, out_dtype, accum, paths, scales, tune, tuner,
+                           device, blocks_per_cu)
+        cat = lambda xs: np.concatenate(xs) if xs else np.zeros(0)  # noqa: E731
+        self._cov = _native.DeviceBuffer.from_array(cat(covs), device)
+        self._z1 = _native.DeviceBuffer.from_array(cat(z1s), device)
+        self._z2 = _native.DeviceBuffer.from_array(cat(z2s), device)
+        plan = ctypes.c_void_p()
+        c_shapes, c_ms, path_codes, scale_arr, opts = args
+        rc = _native.lib().gsc_ensemble_create(
+            ctypes.byref(plan), self.count, self.dim, c_shapes, c_ms, path_codes, scale_arr,
+            self._cov.ptr, self._z1.ptr, self._z2.ptr, self._ax.ptr, self._out.ptr,
+            ctypes.byref(opts))
+        _native.check(rc)
+        self._plan = plan
+
+    @classmethod
+    def seeded(cls, model: str, grids: Sequence, m_counts: Sequence[int],
+               len_scales: Sequence[float], seeds: Sequence[int], **kw) -> "Ensemble":
+        """Plan whose realisation r draws m_counts[r] modes of `model`
+        ("gaussian" or "exponential") with len_scales[r] and seeds[r] on the
+        device (gsc_ensemble_create_seeded); grids[r] is its list of axes.
+        Keyword arguments as for the constructor."""
+        from .sampling import MODELS
+        if model not in MODELS:
+            raise ValueError(f"model must be one of {sorted(MODELS)}")
+        n = len(grids)
+        if n == 0:
+            raise ValueError("an ensemble needs at least one realisation")
+        if not (len(m_counts) == len(len_scales) == len(seeds) == n):
+            raise ValueError("one mode count, length scale and seed per realisation")
+        self = cls.__new__(cls)
+        self._cov = self._z1 = self._z2 = None
+        c_shapes, c_ms, path_codes, scale_arr, opts = self._setup(
+            grids, [int(m) for m in m_counts], kw.pop("precision", "auto"),
+            kw.pop("out_dtype", np.float32), kw.pop("accum", "f32x2"), kw.pop("paths", None),
+            kw.pop("scales", None), kw.pop("tune", False), kw.pop("tuner", None),
+            kw.pop("device", -1), kw.pop("blocks_per_cu", 0))
+        if kw:
+            raise TypeError(f"unexpected keyword arguments {sorted(kw)}")
+        c_len = (ctypes.c_double * n)(*[float(v) for v in len_scales])
+        c_seeds = (ctypes.c_uint64 * n)(*[int(v) & 0xFFFFFFFFFFFFFFFF for v in seeds])
+        plan = ctypes.c_void_p()
+        rc = _native.lib().gsc_ensemble_create_seeded(
+            ctypes.byref(plan), self.count, self.dim, c_shapes, c_ms, path_codes, scale_arr,
+            MODELS[model], c_len, c_seeds, self._ax.ptr, self._out.ptr, ctypes.byref(opts))
+        _native.check(rc)
+        self._plan = plan
+        return self
+
+    def reseed(self, seeds: Sequence[int], stream: int = 0) -> None:
+        """New per-realisation seeds for the following runs of a seeded plan."""
+        if len(seeds) != self.count:
+            raise ValueError("one seed per realisation")
+        c_seeds = (ctypes.c_uint64 * self.count)(*[int(v) & 0xFFFFFFFFFFFFFFFF for v in seeds])
+        _native.check(_native.lib().gsc_ensemble_reseed(self._plan, c_seeds,
+                                                        ctypes.c_void_p(stream or None)))
+
+    def _setup(self, grids, ms, precision, out_dtype, accum, paths, scales, tune, tuner, device,
+               blocks_per_cu):
+        """Geometry, output buffer and options shared by both constructors."""
+        dims = {len(g) for g in grids}
         if len(dims) != 1:
             raise ValueError("all realisations must have the same number of axes")
         self.dim = dims.pop()
         if self.dim not in (2, 3):
             raise ValueError("ensembles support 2-D and 3-D grids")
-        self.count = len(realisations)
+        self.count = len(grids)
         self.out_dtype = np.dtype(out_dtype)
         if self.out_dtype not in (np.float32, np.float64):
             raise ValueError("out_dtype must be float32 or float64")
-        covs, z1s, z2s, axs, shapes, ms = [], [], [], [], [], []
-        for cov, z1, z2, axes in realisations:
-            cov = np.ascontiguousarray(cov, dtype=np.float64)
-            if cov.shape[0] != self.dim or cov.shape[1] != len(z1) or cov.shape[1] != len(z2):
-                raise ValueError("cov_samples must be (d, M) with len(z1) == len(z2) == M")
-            covs.append(cov.ravel())
-            z1s.append(np.asarray(z1, dtype=np.float64))
-            z2s.append(np.asarray(z2, dtype=np.float64))
-            axs.extend(np.asarray(a, dtype=np.float64) for a in axes)
-            shapes.append([len(a) for a in axes])
-            ms.append(cov.shape[1])
+        axs = [np.asarray(a, dtype=np.float64) for g in grids for a in g]
+        shapes = [[len(a) for a in g] for g in grids]
         self.shapes = [tuple(s) for s in shapes]
-        self.m_counts = ms
+        self.m_counts = list(ms)
         if tune:
             from .tuner import Tuner
             tuner = tuner or Tuner()
@@ -72,11 +137,8 @@ class Ensemble:
             if len(scales) != self.count:
                 raise ValueError("one scale per realisation")
             scale_arr = (ctypes.c_double * self.count)(*[float(s) for s in scales])
-        cat = lambda xs: np.concatenate(xs) if xs else np.zeros(0)  # noqa: E731
-        self._cov = _native.DeviceBuffer.from_array(cat(covs), device)
-        self._z1 = _native.DeviceBuffer.from_array(cat(z1s), device)
-        self._z2 = _native.DeviceBuffer.from_array(cat(z2s), device)
-        self._ax = _native.DeviceBuffer.from_array(cat(axs), device)
+        self._ax = _native.DeviceBuffer.from_array(
+            np.concatenate(axs) if axs else np.zeros(0), device)
         self.n_points = [int(np.prod(s)) for s in shapes]
         self._out = _native.DeviceBuffer(max(1, sum(self.n_points)) * self.out_dtype.itemsize,
                                          device)
@@ -86,13 +148,7 @@ class Ensemble:
         opts.blocks_per_cu = int(blocks_per_cu)
         c_shapes = (ctypes.c_int64 * (self.count * self.dim))(*[v for s in shapes for v in s])
         c_ms = (ctypes.c_int64 * self.count)(*ms)
-        plan = ctypes.c_void_p()
-        rc = _native.lib().gsc_ensemble_create(
-            ctypes.byref(plan), self.count, self.dim, c_shapes, c_ms, path_codes, scale_arr,
-            self._cov.ptr, self._z1.ptr, self._z2.ptr, self._ax.ptr, self._out.ptr,
-            ctypes.byref(opts))
-        _native.check(rc)
-        self._plan = plan
+        return c_shapes, c_ms, path_codes, scale_arr, opts
 
     @property
     def info(self) -> dict:

@@ -88,7 +88,7 @@ typedef enum gsc_accum {
 typedef enum gsc_direct_variant {
   GSC_DIRECT_MFMA = 0,  /* v_mfma_f64_16x16x4 tiles of 16 modes x 16 points (dim <= 3) */
   GSC_DIRECT_VALU = 1,  /* FP64 FMA chains on the vector ALUs */
-  GSC_DIRECT_LOCAL = 2  /* cell-local phases on the FP32 matrix cores (default) */
+  GSC_DIRECT_LOCAL = 2  /* cell-local phases from split-bf16 matrix-core products (default) */
 } gsc_direct_variant;
 
 typedef struct gsc_opts {
@@ -245,6 +245,41 @@ int gsc_ensemble_run(gsc_ensemble* plan, void* stream);
 int gsc_ensemble_info(const gsc_ensemble* plan, int64_t* n_factored, int64_t* n_direct,
                       size_t* workspace_bytes, double* point_modes);
 void gsc_ensemble_destroy(gsc_ensemble* plan);
+
+/*
+ * Seeded spectral-mode sampling: the caller side of the summators (GSTools'
+ * RandMeth draws k from the model's spectral density and z1, z2 ~ N(0, 1) on
+ * the host before calling summate, gen_benchmark_inputs.py:17-29; GSTools
+ * itself is not part of the reference).  No counterpart in the reference.
+ * Fills cov (dim, M) row-major, z1 and z2 (M), dim 1..3:
+ *   z1, z2 ~ N(0, 1);
+ *   GSC_MODEL_GAUSSIAN:    k ~ N(0, len_scale^-2 I);
+ *   GSC_MODEL_EXPONENTIAL: k = g / (len_scale |w|), g ~ N(0, I), w ~ N(0, 1)
+ *                          (isotropic multivariate t with one degree of freedom).
+ * Random bits: Philox4x64-10 keyed (seed, stream_id), word for word the
+ * stream of numpy.random.Philox with key words (seed, stream_id); mode m takes words
+ * 8m .. 8m + 7 (four Box-Muller pairs), so mode m does not depend on M.
+ */
+typedef enum gsc_model { GSC_MODEL_GAUSSIAN = 0, GSC_MODEL_EXPONENTIAL = 1 } gsc_model;
+/* Host arrays (synchronous). */
+int gsc_sample_modes(int model, int dim, int64_t M, double len_scale, uint64_t seed,
+                     uint64_t stream_id, double* cov, double* z1, double* z2,
+                     const gsc_opts* opts);
+/* Device arrays, enqueued on `stream`. */
+int gsc_sample_modes_dev(int model, int dim, int64_t M, double len_scale, uint64_t seed,
+                         uint64_t stream_id, double* cov, double* z1, double* z2,
+                         const gsc_opts* opts, void* stream);
+/* Ensemble whose modes are drawn by the sampler above at the start of every
+ * replay (realisation r: model, len_scales[r], seeds[r], stream_id 0), into
+ * plan-owned device arrays; otherwise as gsc_ensemble_create.  No counterpart
+ * in the reference.  gsc_ensemble_reseed copies `count` new seeds (host
+ * array) into the plan, ordered on `stream` before the next run. */
+int gsc_ensemble_create_seeded(gsc_ensemble** plan, int count, int dim, const int64_t* shapes,
+                               const int64_t* m_counts, const int32_t* paths,
+                               const double* scales, int model, const double* len_scales,
+                               const uint64_t* seeds, const double* axes, void* out,
+                               const gsc_opts* opts);
+int gsc_ensemble_reseed(gsc_ensemble* plan, const uint64_t* seeds, void* stream);
 
 /* Device memory plumbing for hosts without their own allocator (e.g. plain
  * numpy callers of the device-resident entry points). */
