@@ -204,6 +204,18 @@ def ensemble_leg(steps: int, warmup: int) -> dict:
            "point_modes_per_s": info["point_modes"] / (ms * 1e-3)}
     ens.close()
     del ens
+    # the same grids with their modes drawn on the device from per-realisation
+    # seeds inside the graph (sampler + tables + GEMM per replay)
+    seeded = Ensemble.seeded("gaussian", [w.axes for w in ens_w], [w.modes.m for w in ens_w],
+                             [w.modes.len_scale for w in ens_w],
+                             [19031977 + r for r in range(len(ens_w))], paths=paths,
+                             scales=scales, out_dtype=np.float32, precision="fp16x3")
+    _, total_s = time_steps(lambda: seeded.run(stream.cuda_stream), steps, warmup, stream)
+    ms_s = total_s / steps
+    res["seeded_on_device"] = {"ms_per_step": ms_s,
+                               "point_modes_per_s": seeded.info["point_modes"] / (ms_s * 1e-3)}
+    seeded.close()
+    del seeded
     # the same realisations one grid call at a time (the caller-side loop)
     dev = torch.device("cuda:0")
     sub = ens_w[:64]

@@ -186,3 +186,15 @@ def test_grid_point_expansion():
     pts = workloads.grid_points(axes, idx)
     full = np.stack(np.meshgrid(*axes, indexing="ij")).reshape(3, -1)
     np.testing.assert_array_equal(pts, full[:, idx])
+
+
+def test_randmeth_argument_checks_need_no_device():
+    from gstools_core.randmeth import IncomprRandMeth, RandMeth
+    for kw in ({"model": "matern"}, {"dim": 4}, {"var": -1.0}, {"len_scale": 0.0},
+               {"mode_no": 0}, {"nugget": -0.1}):
+        args = {"model": "gaussian", "dim": 2}
+        args.update(kw)
+        with pytest.raises(ValueError):
+            RandMeth(args.pop("model"), args.pop("dim"), **args)
+    with pytest.raises(ValueError):
+        IncomprRandMeth("gaussian", 1)
