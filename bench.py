@@ -281,6 +281,51 @@ def incompr_leg(steps: int, warmup: int, n: int = 4_000_000, m: int = 1000) -> d
     return res
 
 
+def krige_leg(steps: int, warmup: int, n: int = 1000, N: int = 1_000_000) -> dict:
+    """Kriging calculators (krige.rs:24-118) on device-resident inputs: n
+    conditions, N targets, FP64.  field+error: 2 n^2 N flops on the FP64
+    matrix cores; field alone: one read of krig_vecs (8 n N bytes)."""
+    import ctypes
+
+    import torch
+
+    from gstools_core import _native
+
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(19031977)
+    a = torch.randn(n, n, dtype=torch.float64, device=dev, generator=g) / n ** 0.5
+    mat = (a @ a.T).contiguous()
+    vecs = torch.randn(n, N, dtype=torch.float64, device=dev, generator=g)
+    cond = torch.randn(n, dtype=torch.float64, device=dev, generator=g)
+    out = torch.empty(2 * N, dtype=torch.float64, device=dev)
+    lib = _native.lib()
+    nb = max(lib.gsc_krige_workspace_bytes(n, N, 1), lib.gsc_krige_workspace_bytes(n, N, 0))
+    work = torch.empty(nb, dtype=torch.uint8, device=dev)
+    opts = _native.make_opts()
+    stream = torch.cuda.current_stream()
+    res = {"workload": f"n={n} conditions, N={N} targets, FP64, synthetic SPD krige_mat"}
+    for name, err in (("field_and_variance", True), ("field", False)):
+        def step(err=err):
+            _native.check(lib.gsc_krige_dev(n, N, mat.data_ptr(), vecs.data_ptr(), N,
+                                            cond.data_ptr(), out.data_ptr(),
+                                            out.data_ptr() + 8 * N if err else None,
+                                            work.data_ptr(), nb, ctypes.byref(opts),
+                                            ctypes.c_void_p(stream.cuda_stream)))
+
+        _, tot = time_steps(step, steps, warmup, stream)
+        ms = tot / steps
+        r = {"ms_per_step": ms, "targets_per_s": N / (ms * 1e-3)}
+        if err:
+            tf = 2.0 * n * n * N / (ms * 1e-3) / 1e12
+            r.update({"fp64_mfma_tflops": tf, "frac_of_78_6": tf / 78.6})
+        else:
+            gbs = 8.0 * n * N / (ms * 1e-3) / 1e9
+            r.update({"vecs_read_gb_s": gbs, "frac_of_8000": gbs / 8000.0})
+        res[name] = r
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -292,9 +337,9 @@ def main() -> int:
     ap.add_argument("--m", type=int, default=1000)
     ap.add_argument("--no-grid", action="store_true", help="skip the structured-grid leg")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble leg")
-    ap.add_argument("--legs", default="direct,incompr,grid,c5,ensemble",
-                    help="comma list of legs: direct, incompr, grid, c5, ensemble, copy (a 1 GiB "
-                         "device copy, the reference for HBM byte counters)")
+    ap.add_argument("--legs", default="direct,incompr,grid,c5,ensemble,krige",
+                    help="comma list of legs: direct, incompr, grid, c5, ensemble, krige, copy "
+                         "(a 1 GiB device copy, the reference for HBM byte counters)")
     args = ap.parse_args()
     legs = set(args.legs.split(","))
     if args.no_grid:
@@ -329,6 +374,8 @@ def main() -> int:
             res["structured_grid"] = grid_leg(args.steps, args.warmup)
         if "c5" in legs:
             res["heavy_tail_grid"] = c5_leg(max(3, args.steps // 4), 1)
+        if "krige" in legs:
+            res["krige"] = krige_leg(max(3, args.steps // 4), 1)
         if "ensemble" in legs:
             res["ensemble"] = ensemble_leg(max(3, args.steps // 2), 1)
         print(json.dumps(res), flush=True)
@@ -415,6 +462,11 @@ def main() -> int:
         log("heavy-tailed grid leg (512^3, M = 4000)")
         c5_res = c5_leg(max(3, args.steps // 4), 1)
 
+    krige_res = None
+    if "krige" in legs:
+        log("kriging leg (n = 1000, N = 1e6)")
+        krige_res = krige_leg(max(3, args.steps // 4), 1)
+
     ens_res = None
     if "ensemble" in legs:
         log("ensemble leg (512 realisations)")
@@ -459,6 +511,7 @@ def main() -> int:
         "incompr": inc_res,
         "structured_grid": grid_res,
         "heavy_tail_grid": c5_res,
+        "krige": krige_res,
         "ensemble": ens_res,
     }
     print(json.dumps(line), flush=True)

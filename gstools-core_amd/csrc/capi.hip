@@ -369,9 +369,131 @@ int validate_precision(const gsc_opts* o) {
   return GSC_OK;
 }
 
+// ---- kriging -------------------------------------------------------------------
+
+int validate_krige(int64_t n, int64_t N, const void* mat, const void* vecs, const void* cond,
+                   const void* field) {
+  if (n < 0 || N < 0) return fail(GSC_ERR_INVALID, "negative n or N");
+  if (n > ((int64_t)1 << 24)) return fail(GSC_ERR_INVALID, "n=%lld is too large", (long long)n);
+  if (N > ((int64_t)1 << 40)) return fail(GSC_ERR_INVALID, "N=%lld is too large", (long long)N);
+  if (n > 0 && (!mat || !cond)) return fail(GSC_ERR_INVALID, "null krige_mat or cond");
+  if (N > 0 && !field) return fail(GSC_ERR_INVALID, "null field");
+  if (n > 0 && N > 0 && !vecs) return fail(GSC_ERR_INVALID, "null krig_vecs");
+  return GSC_OK;
+}
+
+// Host form of the two kriging calculators (error == null: field only).
+int host_krige(int64_t n, int64_t N, const double* mat, int64_t m0, int64_t m1,
+               const double* vecs, int64_t v0, int64_t v1, const double* cond, int64_t cs,
+               double* field, int64_t fs, double* error, int64_t es, const gsc_opts* opts_in) {
+  gsc_opts tmp;
+  const gsc_opts* o = opts_or_default(opts_in, &tmp);
+  int rc = validate_krige(n, N, mat, vecs, cond, field);
+  if (rc) return rc;
+  if ((rc = validate_opts(o))) return rc;
+  if (N == 0) return GSC_OK;
+  try {
+    std::vector<double> hm((size_t)std::max<int64_t>(n * n, 1)), hc((size_t)std::max<int64_t>(n, 1));
+    std::vector<double> hv;
+    const double* vsrc = vecs;
+    if (n > 0) {
+      gather2(mat, n, n, m0, m1, hm.data(), 1);
+      gather1(cond, n, cs, hc.data());
+      if (!(v1 == 1 && v0 == N)) {
+        hv.resize((size_t)(n * N));
+        gather2(vecs, n, N, v0, v1, hv.data(), o->num_threads);
+        vsrc = hv.data();
+      }
+    }
+    int dev = 0;
+    if ((rc = resolve_device(o, &dev))) return rc;
+    DeviceCtx& c = g_dev[dev];
+    std::lock_guard<std::mutex> lock(c.mu);
+    GSC_HIP(hipSetDevice(dev));
+    if ((rc = check_arch(dev))) return rc;
+    if (!c.stream) GSC_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    const int nout = error ? 2 : 1;
+    const size_t b_work = align256(gsc::krige_work_bytes(n, N, error != nullptr));
+    const size_t b_mat = align256(sizeof(double) * hm.size());
+    const size_t b_cond = align256(sizeof(double) * hc.size());
+    const size_t b_vecs = align256(sizeof(double) * (size_t)std::max<int64_t>(n * N, 1));
+    const size_t b_out = align256(sizeof(double) * nout * N);
+    const size_t need = b_work + b_mat + b_cond + b_vecs + b_out;
+    if (need > c.cap) {
+      if (c.buf) GSC_HIP(hipFree(c.buf));
+      c.buf = nullptr;
+      c.cap = 0;
+      if (hipMalloc(&c.buf, need) != hipSuccess)
+        return fail(GSC_ERR_ALLOC, "hipMalloc of %zu bytes failed", need);
+      c.cap = need;
+    }
+    char* base = static_cast<char*>(c.buf);
+    void* d_work = base;
+    double* d_mat = reinterpret_cast<double*>(base + b_work);
+    double* d_cond = reinterpret_cast<double*>(base + b_work + b_mat);
+    double* d_vecs = reinterpret_cast<double*>(base + b_work + b_mat + b_cond);
+    double* d_out = reinterpret_cast<double*>(base + b_work + b_mat + b_cond + b_vecs);
+    if (n > 0) {
+      GSC_HIP(hipMemcpyAsync(d_mat, hm.data(), sizeof(double) * n * n, hipMemcpyHostToDevice,
+                             c.stream));
+      GSC_HIP(hipMemcpyAsync(d_cond, hc.data(), sizeof(double) * n, hipMemcpyHostToDevice,
+                             c.stream));
+      GSC_HIP(hipMemcpyAsync(d_vecs, vsrc, sizeof(double) * n * N, hipMemcpyHostToDevice,
+                             c.stream));
+    }
+    GSC_HIP(gsc::launch_krige(n, N, d_mat, d_vecs, N, d_cond, d_out, 1, error ? d_out + N : nullptr,
+                              1, d_work, c.stream));
+    std::vector<double> res((size_t)(nout * N));
+    GSC_HIP(hipMemcpyAsync(res.data(), d_out, sizeof(double) * nout * N, hipMemcpyDeviceToHost,
+                           c.stream));
+    GSC_HIP(hipStreamSynchronize(c.stream));
+    for (int64_t j = 0; j < N; ++j) field[j * fs] = res[(size_t)j];
+    if (error)
+      for (int64_t j = 0; j < N; ++j) error[j * es] = res[(size_t)(N + j)];
+  } catch (const std::bad_alloc&) {
+    return fail(GSC_ERR_ALLOC, "host allocation failed");
+  } catch (...) {
+    return fail(GSC_ERR_INVALID, "unexpected host exception");
+  }
+  return GSC_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int gsc_krige(int64_t n, int64_t N, const double* mat, int64_t mat_s0, int64_t mat_s1,
+              const double* vecs, int64_t vecs_s0, int64_t vecs_s1, const double* cond,
+              int64_t cond_s, double* field, int64_t field_s, double* error, int64_t error_s,
+              const gsc_opts* opts) {
+  return host_krige(n, N, mat, mat_s0, mat_s1, vecs, vecs_s0, vecs_s1, cond, cond_s, field,
+                    field_s, error, error_s, opts);
+}
+
+size_t gsc_krige_workspace_bytes(int64_t n, int64_t N, int with_error) {
+  if (n < 0 || N < 0) return 0;
+  return gsc::krige_work_bytes(n, N, with_error != 0);
+}
+
+int gsc_krige_dev(int64_t n, int64_t N, const double* mat, const double* vecs, int64_t vecs_ld,
+                  const double* cond, double* field, double* error, void* work,
+                  size_t work_bytes, const gsc_opts* opts, void* stream) {
+  gsc_opts tmp;
+  const gsc_opts* o = opts_or_default(opts, &tmp);
+  int rc = validate_krige(n, N, mat, vecs, cond, field);
+  if (rc) return rc;
+  if ((rc = validate_opts(o))) return rc;
+  if (n > 0 && N > 0 && vecs_ld < N)
+    return fail(GSC_ERR_INVALID, "vecs_ld (%lld) < N (%lld)", (long long)vecs_ld, (long long)N);
+  const size_t need = gsc::krige_work_bytes(n, N, error != nullptr);
+  if (!work || work_bytes < need)
+    return fail(GSC_ERR_WORKSPACE, "workspace of %zu bytes is below the %zu required", work_bytes,
+                need);
+  if (N == 0) return GSC_OK;
+  GSC_HIP(gsc::launch_krige(n, N, mat, vecs, vecs_ld, cond, field, 1, error, 1, work,
+                            static_cast<hipStream_t>(stream)));
+  return GSC_OK;
+}
 
 void gsc_opts_init(gsc_opts* o) {
   if (!o) return;

@@ -82,4 +82,12 @@ struct SampleArgs {
 };
 hipError_t launch_sample_modes(const SampleArgs& a, hipStream_t stream);
 
+// Kriging calculators (krige.hip).  mat (n, n) and vecs (n, N) row-major
+// device arrays (vecs row pitch vld), cond (n); field[j * fs]; error[j * es]
+// or null for the field alone.  work: krige_work_bytes(n, N, error != null).
+size_t krige_work_bytes(int64_t n, int64_t N, bool with_error);
+hipError_t launch_krige(int64_t n, int64_t N, const double* mat, const double* vecs, int64_t vld,
+                        const double* cond, double* field, int64_t fs, double* error, int64_t es,
+                        void* work, hipStream_t stream);
+
 }  // namespace gsc

@@ -34,7 +34,8 @@ from . import _native
 
 __version__ = "1.1.0"
 
-__all__ = ["summate", "summate_incompr", "summate_fourier", "__version__"]
+__all__ = ["summate", "summate_incompr", "summate_fourier", "calc_field_krige_and_variance",
+           "calc_field_krige", "__version__"]
 
 
 def _array_arg(name: str, a, ndim: int):
@@ -181,3 +182,50 @@ def summate_fourier(spectrum_factor, modes, z1, z2, pos, num_threads=None, *,
                                            _ptr(out), 1, ctypes.byref(opts))
     _native.check(rc)
     return out
+
+
+def _krige_args(krige_mat, krig_vecs, cond, num_threads):
+    m = _array_arg("krige_mat", krige_mat, 2)
+    v = _array_arg("krig_vecs", krig_vecs, 2)
+    c = _array_arg("cond", cond, 1)
+    nt = _threads_arg(num_threads)
+    # the reference asserts these (krige.rs:31-33, 80-82) and aborts
+    if m.shape[0] != m.shape[1]:
+        raise ValueError("krige_mat must be square")
+    if m.shape[0] != v.shape[0]:
+        raise ValueError("krige_mat and krig_vecs disagree on the number of conditions")
+    if m.shape[0] != c.shape[0]:
+        raise ValueError("krige_mat and cond disagree on the number of conditions")
+    return m, v, c, nt
+
+
+def _krige(m, v, c, nt, device, with_error):
+    n, N = v.shape
+    field = np.empty(N)
+    error = np.empty(N) if with_error else None
+    if N == 0:
+        return field, error
+    opts = _native.make_opts(device=device, num_threads=nt)
+    (m0, m1), (v0, v1), (cs,) = _es(m), _es(v), _es(c)
+    rc = _native.lib().gsc_krige(n, N, _ptr(m), m0, m1, _ptr(v), v0, v1, _ptr(c), cs,
+                                 _ptr(field), 1, _ptr(error) if with_error else None, 1,
+                                 ctypes.byref(opts))
+    _native.check(rc)
+    return field, error
+
+
+def calc_field_krige_and_variance(krige_mat, krig_vecs, cond, num_threads=None, *, device=-1):
+    """Kriging field and error variance (krige.rs:24-70, lib.rs:86-103).
+
+    field[j] = sum_i cond[i] (krige_mat[:, i] . krig_vecs[:, j]),
+    error[j] = sum_i krig_vecs[i, j] (krige_mat[:, i] . krig_vecs[:, j]).
+    """
+    m, v, c, nt = _krige_args(krige_mat, krig_vecs, cond, num_threads)
+    return _krige(m, v, c, nt, device, True)
+
+
+def calc_field_krige(krige_mat, krig_vecs, cond, num_threads=None, *, device=-1):
+    """Kriging field alone (krige.rs:88-118, lib.rs:105-118)."""
+    m, v, c, nt = _krige_args(krige_mat, krig_vecs, cond, num_threads)
+    return _krige(m, v, c, nt, device, False)[0]
+

@@ -96,6 +96,10 @@ SIGNATURES = {
                                           ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_uint64), _vp, _vp, _optp]),
     "gsc_ensemble_reseed": (_int, [_vp, ctypes.POINTER(ctypes.c_uint64), _vp]),
+    "gsc_krige": (_int, [_i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _vp, _i64,
+                         _vp, _i64, _optp]),
+    "gsc_krige_workspace_bytes": (_sz, [_i64, _i64, _int]),
+    "gsc_krige_dev": (_int, [_i64, _i64, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _sz, _optp, _vp]),
     "gsc_device_alloc": (_vp, [_sz, _int]),
     "gsc_device_free": (None, [_vp]),
     "gsc_memcpy_to_device": (_int, [_vp, _vp, _sz]),

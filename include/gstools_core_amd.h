@@ -281,6 +281,33 @@ int gsc_ensemble_create_seeded(gsc_ensemble** plan, int count, int dim, const in
                                const gsc_opts* opts);
 int gsc_ensemble_reseed(gsc_ensemble* plan, const uint64_t* seeds, void* stream);
 
+/*
+ * Kriging calculators, n conditions and N targets:
+ *   krig_fac_i(j) = sum_k mat[k][i] vecs[k][j]
+ *   field[j] = sum_i cond[i] krig_fac_i(j),   error[j] = sum_i vecs[i][j] krig_fac_i(j)
+ * Replaces krige::calculator_field_krige_and_variance (src/krige.rs:24-70),
+ * gstools_core.calc_field_krige_and_variance (src/lib.rs:86-103) and, with
+ * error == NULL, krige::calculator_field_krige (krige.rs:88-118) /
+ * gstools_core.calc_field_krige (lib.rs:105-118).  mat is (n, n), vecs
+ * (n, N), cond (n), all FP64 with signed element strides; field and error
+ * receive N values with strides field_s / error_s.  With error, the 2 n^2 N
+ * products run as FP64 matrix-core tiles; the field alone costs n^2 + 2 n N
+ * (mat cond first).  Host pointers, synchronous.
+ */
+int gsc_krige(int64_t n, int64_t N, const double* mat, int64_t mat_s0, int64_t mat_s1,
+              const double* vecs, int64_t vecs_s0, int64_t vecs_s1, const double* cond,
+              int64_t cond_s, double* field, int64_t field_s, double* error, int64_t error_s,
+              const gsc_opts* opts);
+/* Workspace of gsc_krige_dev: 2 * 8 * ceil(n / 64) * N bytes with the error,
+ * 8 n bytes for the field alone. */
+size_t gsc_krige_workspace_bytes(int64_t n, int64_t N, int with_error);
+/* Device form: mat dense (n, n) row-major, vecs (n, N) with row pitch
+ * vecs_ld >= N, cond (n), field and error (N, dense; error may be NULL);
+ * enqueued on `stream`. */
+int gsc_krige_dev(int64_t n, int64_t N, const double* mat, const double* vecs, int64_t vecs_ld,
+                  const double* cond, double* field, double* error, void* work,
+                  size_t work_bytes, const gsc_opts* opts, void* stream);
+
 /* Device memory plumbing for hosts without their own allocator (e.g. plain
  * numpy callers of the device-resident entry points). */
 void* gsc_device_alloc(size_t bytes, int device);

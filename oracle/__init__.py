@@ -54,6 +54,9 @@ def _load():
                                                          _i64, _dp, _i64, _dp, _i64, _i64, _dp,
                                                          _i64, _i64, _i64, _int]
     lib.gsc_oracle_summate_incompr_modechunk.restype = _int
+    lib.gsc_oracle_krige.argtypes = [_i64, _i64, _dp, _i64, _i64, _dp, _i64, _i64, _dp, _i64,
+                                     _dp, _i64, _dp, _i64]
+    lib.gsc_oracle_krige.restype = None
     lib.gsc_oracle_max_threads.argtypes = []
     lib.gsc_oracle_max_threads.restype = _int
     _lib = lib
@@ -161,3 +164,24 @@ def grid_points(axes) -> np.ndarray:
     """Expand per-axis coordinates into a (d, Nx*Ny[*Nz]) point list, C order (last axis fastest)."""
     mesh = np.meshgrid(*[np.asarray(a, dtype=np.float64) for a in axes], indexing="ij")
     return np.stack([m.ravel() for m in mesh])
+
+
+def calc_field_krige_and_variance(krige_mat, krig_vecs, cond):
+    """(field, error) of krige.rs:24-70 (sequential condition order)."""
+    m, mp, (m0, m1) = _view(krige_mat, 2)
+    v, vp, (v0, v1) = _view(krig_vecs, 2)
+    c, cp, (cs,) = _view(cond, 1)
+    n, N = v.shape
+    if m.shape != (n, n) or c.shape[0] != n:
+        raise ValueError("shape mismatch")
+    field = np.zeros(N)
+    error = np.zeros(N)
+    _load().gsc_oracle_krige(n, N, mp, m0, m1, vp, v0, v1, cp, cs,
+                             ctypes.c_void_p(field.ctypes.data), 1,
+                             ctypes.c_void_p(error.ctypes.data), 1)
+    return field, error
+
+
+def calc_field_krige(krige_mat, krig_vecs, cond):
+    """field of krige.rs:88-118."""
+    return calc_field_krige_and_variance(krige_mat, krig_vecs, cond)[0]
