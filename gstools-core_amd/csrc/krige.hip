@@ -61,16 +61,31 @@ k_krige_tiles(int64_t n, int64_t N, const double* __restrict__ mat,
   f64x4 acc[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) acc[t] = f64x4{0.0, 0.0, 0.0, 0.0};
-  for (int64_t k0 = 0; k0 < n; k0 += kKK) {
-    // stage mat[k0 .. +31][i0 .. +63] and vecs[k0 .. +31][j0 .. +63] (zero outside)
+  // this thread's share of a chunk: mat[k0 + kk][i0 + ii] and vecs[k0 + kk][j0 + ii]
+  // for e = threadIdx.x + 256 u, kk = e / 64, ii = e % 64 (zero outside)
+  constexpr int kPer = kKK * kKI / 256;
+  double rm[kPer], rv[kPer];
+  auto gload = [&](int64_t k0) {
 #pragma unroll
-    for (int e = threadIdx.x; e < kKK * kKI; e += 256) {
-      const int kk = e / kKI, ii = e % kKI;
+    for (int u = 0; u < kPer; ++u) {
+      const int e = threadIdx.x + 256 * u, kk = e / kKI, ii = e % kKI;
       const int64_t k = k0 + kk, i = i0 + ii, j = j0 + ii;
-      sm_m[kk][ii] = (k < n && i < n) ? mat[k * n + i] : 0.0;
-      sm_v[kk][ii] = (k < n && j < N) ? vecs[k * vld + j] : 0.0;
+      rm[u] = (k < n && i < n) ? mat[k * n + i] : 0.0;
+      rv[u] = (k < n && j < N) ? vecs[k * vld + j] : 0.0;
+    }
+  };
+  gload(0);
+  for (int64_t k0 = 0; k0 < n; k0 += kKK) {
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int e = threadIdx.x + 256 * u, kk = e / kKI, ii = e % kKI;
+      sm_m[kk][ii] = rm[u];
+      sm_v[kk][ii] = rv[u];
     }
     __syncthreads();
+    // the next chunk's loads are in flight during this chunk's MFMAs
+    if (k0 + kKK < n) gload(k0 + kKK);
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int kk = 0; kk < kKK; kk += 4) {
       const double a = sm_m[kk + g][16 * w + jl];
