@@ -8,7 +8,7 @@
 // flops and is computed tile by tile on the FP64 matrix cores
 // (v_mfma_f64_16x16x4f64); the two contractions with cond and vecs are fused
 // into the tile epilogue, so W never reaches memory.  A block owns 64 rows i
-// (four waves of 16) and 64 targets j (four 16-column MFMA tiles per wave);
+// (four waves of 16) and 128 targets j (eight 16-column MFMA tiles per wave);
 // K runs over the n conditions in LDS-staged chunks of 32.  Each block writes
 // its partial sums over its 64 rows; a second kernel adds the partials of
 // all row blocks in a fixed order (results are bitwise reproducible).
@@ -26,14 +26,14 @@ namespace {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kKI = 64;  // rows i per block
-constexpr int kKJ = 64;  // targets j per block
+constexpr int kKI = 64;  // rows i per block (four waves of 16)
 constexpr int kKK = 32;  // conditions k per LDS chunk
+constexpr int kTJ = 8;   // 16-column MFMA tiles per wave: kKJ = 16 kTJ targets per block
+constexpr int kKJ = 16 * kTJ;
 constexpr int kXcds = 8; // blocks are dispatched to the XCDs round-robin
-static_assert(kKI == kKJ, "the staging loop fills both tiles with one index");
 
 // Block b -> (row block ib, target block jb): the row blocks of one target
-// block run on one XCD, so its 64 columns of vecs stay in that XCD's L2.
+// block run on one XCD, so its columns of vecs stay in that XCD's L2.
 __device__ __forceinline__ bool krige_block(int nib, int njb, int& ib, int& jb) {
   const int b = blockIdx.x;
   const int xcd = b % kXcds, s = b / kXcds;
@@ -42,63 +42,85 @@ __device__ __forceinline__ bool krige_block(int nib, int njb, int& ib, int& jb) 
   return jb < njb;
 }
 
+// LDS of a block: the staged chunk of mat (kKK x kKI) and vecs (kKK x kKJ),
+// reused after the K loop for the epilogue's partial sums (2 x 16 x kKJ).
+struct KrigeSmem {
+  union {
+    struct {
+      double m[kKK][kKI + 1];
+      double v[kKK][kKJ + 1];
+    } st;
+    struct {
+      double f[16][kKJ + 1];
+      double e[16][kKJ + 1];
+    } red;
+  };
+};
+
 // MFMA operand / result layouts (v_mfma_f64_16x16x4f64, verified on gfx950):
 //   A: lane l holds A[l % 16][l / 16], B: lane l holds B[l / 16][l % 16],
 //   D: lane l, item it holds D[4 it + l / 16][l % 16].
 // Here A[i][k] = mat[k][i] (16 rows i of the wave), B[k][j] = vecs[k][j].
+// Every block reads all of mat once per kKJ targets, so the target width
+// sets mat's share of the traffic (n^2 8 / kKJ bytes per target).
 __global__ void __launch_bounds__(256)
 k_krige_tiles(int64_t n, int64_t N, const double* __restrict__ mat,
               const double* __restrict__ vecs, int64_t vld, const double* __restrict__ cond,
               int nib, int njb, double* __restrict__ part_f, double* __restrict__ part_e) {
-  __shared__ double sm_m[kKK][kKI + 1];
-  __shared__ double sm_v[kKK][kKJ + 1];
-  __shared__ double red_f[16][kKJ + 1];
-  __shared__ double red_e[16][kKJ + 1];
+  __shared__ KrigeSmem sm;
   int ib, jb;
   if (!krige_block(nib, njb, ib, jb)) return;
   const int64_t i0 = static_cast<int64_t>(ib) * kKI, j0 = static_cast<int64_t>(jb) * kKJ;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, jl = lane & 15;
-  f64x4 acc[4];
+  f64x4 acc[kTJ];
 #pragma unroll
-  for (int t = 0; t < 4; ++t) acc[t] = f64x4{0.0, 0.0, 0.0, 0.0};
-  // this thread's share of a chunk: mat[k0 + kk][i0 + ii] and vecs[k0 + kk][j0 + ii]
-  // for e = threadIdx.x + 256 u, kk = e / 64, ii = e % 64 (zero outside)
-  constexpr int kPer = kKK * kKI / 256;
-  double rm[kPer], rv[kPer];
+  for (int t = 0; t < kTJ; ++t) acc[t] = f64x4{0.0, 0.0, 0.0, 0.0};
+  // this thread's share of a chunk: mat[k0 + tm_k + kStepM u][i0 + tm_i] and
+  // vecs[k0 + tv_k + kStepV u][j0 + tv_j] (zero outside), i.e. element
+  // e = threadIdx.x + 256 u of the row-major chunk tiles
+  constexpr int kPerM = kKK * kKI / 256, kPerV = kKK * kKJ / 256;
+  constexpr int kStepM = 256 / kKI, kStepV = 256 / kKJ;
+  const int tm_k = threadIdx.x / kKI, tm_i = threadIdx.x % kKI;
+  const int tv_k = threadIdx.x / kKJ, tv_j = threadIdx.x % kKJ;
+  const bool m_ok = i0 + tm_i < n, v_ok = j0 + tv_j < N;
+  const double* mp = mat + (m_ok ? i0 + tm_i : 0);
+  const double* vp = vecs + (v_ok ? j0 + tv_j : 0);
+  double rm[kPerM], rv[kPerV];
   auto gload = [&](int64_t k0) {
 #pragma unroll
-    for (int u = 0; u < kPer; ++u) {
-      const int e = threadIdx.x + 256 * u, kk = e / kKI, ii = e % kKI;
-      const int64_t k = k0 + kk, i = i0 + ii, j = j0 + ii;
-      rm[u] = (k < n && i < n) ? mat[k * n + i] : 0.0;
-      rv[u] = (k < n && j < N) ? vecs[k * vld + j] : 0.0;
+    for (int u = 0; u < kPerM; ++u) {
+      const int64_t k = k0 + tm_k + kStepM * u;
+      rm[u] = (m_ok && k < n) ? mp[k * n] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kPerV; ++u) {
+      const int64_t k = k0 + tv_k + kStepV * u;
+      rv[u] = (v_ok && k < n) ? vp[k * vld] : 0.0;
     }
   };
   gload(0);
   for (int64_t k0 = 0; k0 < n; k0 += kKK) {
 #pragma unroll
-    for (int u = 0; u < kPer; ++u) {
-      const int e = threadIdx.x + 256 * u, kk = e / kKI, ii = e % kKI;
-      sm_m[kk][ii] = rm[u];
-      sm_v[kk][ii] = rv[u];
-    }
+    for (int u = 0; u < kPerM; ++u) sm.st.m[tm_k + kStepM * u][tm_i] = rm[u];
+#pragma unroll
+    for (int u = 0; u < kPerV; ++u) sm.st.v[tv_k + kStepV * u][tv_j] = rv[u];
     __syncthreads();
     // the next chunk's loads are in flight during this chunk's MFMAs
     if (k0 + kKK < n) gload(k0 + kKK);
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int kk = 0; kk < kKK; kk += 4) {
-      const double a = sm_m[kk + g][16 * w + jl];
+      const double a = sm.st.m[kk + g][16 * w + jl];
 #pragma unroll
-      for (int t = 0; t < 4; ++t)
-        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sm_v[kk + g][16 * t + jl], acc[t], 0, 0,
-                                                      0);
+      for (int t = 0; t < kTJ; ++t)
+        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sm.st.v[kk + g][16 * t + jl], acc[t], 0,
+                                                      0, 0);
     }
     __syncthreads();
   }
   // epilogue: acc[t][it] = W[i0 + 16 w + 4 it + g][j0 + 16 t + jl]
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
+  for (int t = 0; t < kTJ; ++t) {
     const int64_t j = j0 + 16 * t + jl;
     double pf = 0.0, pe = 0.0;
 #pragma unroll
@@ -109,17 +131,16 @@ k_krige_tiles(int64_t n, int64_t N, const double* __restrict__ mat,
       pf = __builtin_fma(ok ? cond[i] : 0.0, wv, pf);
       pe = __builtin_fma(ok ? vecs[i * vld + j] : 0.0, wv, pe);
     }
-    red_f[4 * w + g][16 * t + jl] = pf;
-    red_e[4 * w + g][16 * t + jl] = pe;
+    sm.red.f[4 * w + g][16 * t + jl] = pf;
+    sm.red.e[4 * w + g][16 * t + jl] = pe;
   }
   __syncthreads();
-  if (threadIdx.x < kKJ) {
-    const int c = threadIdx.x;
+  for (int c = threadIdx.x; c < kKJ; c += 256) {
     double f = 0.0, e = 0.0;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      f += red_f[r][c];
-      e += red_e[r][c];
+      f += sm.red.f[r][c];
+      e += sm.red.e[r][c];
     }
     const int64_t j = j0 + c;
     if (j < N) {
