@@ -94,6 +94,32 @@ def test_graph_replays_with_eager_work_between():
     ens.close()
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_many_realisations_match_single_grid_calls(dim):
+    # a larger plan (48 factored realisations of uneven 2-D / 3-D grids): every
+    # realisation must equal its single-grid evaluation, on every replay
+    rng = np.random.default_rng(40 + dim)
+    count = 48
+    if dim == 2:
+        shapes = [tuple(int(v) for v in rng.integers(17, 300, size=2)) for _ in range(count)]
+    else:
+        shapes = [tuple(int(v) for v in rng.integers(17, 48, size=3)) for _ in range(count)]
+    ms = [int(v) for v in rng.integers(16, 400, size=count)]
+    reals = _reals(dim, shapes, ms, 50 + dim)
+    ens = Ensemble(reals, paths=["factored"] * count, out_dtype=np.float64, precision="fp16x3")
+    assert ens.info["factored"] == count
+    singles = [grid.summate_grid(cov, z1, z2, axes, path="factored", precision="fp16x3")
+               for cov, z1, z2, axes in reals]
+    for rep in range(3):
+        ens.run()
+        res = ens.results()
+        for i, (x, y) in enumerate(zip(res, singles)):
+            np.testing.assert_allclose(x, y, rtol=0, atol=1e-9 * np.sqrt(ms[i]),
+                                       err_msg=f"replay {rep} realisation {i}")
+        grid.summate_grid(reals[0][0], reals[0][1], reals[0][2], reals[0][3])
+    ens.close()
+
+
 def test_c4_realisations_in_one_plan():
     ens_w = workloads.c4(count=512)[:24]
     reals = [(w.modes.cov_samples, w.modes.z1, w.modes.z2, w.axes) for w in ens_w]
