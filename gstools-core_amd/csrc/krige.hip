@@ -63,7 +63,7 @@ struct KrigeSmem {
 // Here A[i][k] = mat[k][i] (16 rows i of the wave), B[k][j] = vecs[k][j].
 // Every block reads all of mat once per kKJ targets, so the target width
 // sets mat's share of the traffic (n^2 8 / kKJ bytes per target).
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_krige_tiles(int64_t n, int64_t N, const double* __restrict__ mat,
               const double* __restrict__ vecs, int64_t vld, const double* __restrict__ cond,
               int nib, int njb, double* __restrict__ part_f, double* __restrict__ part_e) {
@@ -86,7 +86,21 @@ k_krige_tiles(int64_t n, int64_t N, const double* __restrict__ mat,
   const double* mp = mat + (m_ok ? i0 + tm_i : 0);
   const double* vp = vecs + (v_ok ? j0 + tv_j : 0);
   double rm[kPerM], rv[kPerV];
+  // interior tiles and whole chunks need no bounds checks: one running row
+  // offset per operand and uniform strides between this thread's rows
+  const bool interior = i0 + kKI <= n && j0 + kKJ <= N;
+  const int64_t m_step = kStepM * n, v_step = kStepV * vld;
+  const int64_t m_row0 = tm_k * n, v_row0 = tv_k * vld;
   auto gload = [&](int64_t k0) {
+    if (interior && k0 + kKK <= n) {
+      const double* pm = mp + m_row0 + k0 * n;
+      const double* pv = vp + v_row0 + k0 * vld;
+#pragma unroll
+      for (int u = 0; u < kPerM; ++u) rm[u] = pm[u * m_step];
+#pragma unroll
+      for (int u = 0; u < kPerV; ++u) rv[u] = pv[u * v_step];
+      return;
+    }
 #pragma unroll
     for (int u = 0; u < kPerM; ++u) {
       const int64_t k = k0 + tm_k + kStepM * u;
