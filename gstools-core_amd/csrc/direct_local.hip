@@ -84,6 +84,7 @@ constexpr int kWavePts = 64;        // point slots per wave (4 MFMA groups of 16
 constexpr int kModeChunk = 512;     // modes per (r0, amplitude) table of a wave (32 tiles)
 constexpr double kRoughRev = 1.0;   // bound on |kap . xi| (revolutions) of the FP32 phase
 constexpr int kBoxBlocks = 1024;    // partial bounding boxes
+constexpr int kBoxUnroll = 4;       // points in flight per thread there
 constexpr int kRedLd = 65;          // LDS row of the final cross-lane sums (doubles)
 
 struct alignas(16) BinParams {
@@ -158,16 +159,26 @@ k_box_partial(const double* __restrict__ pos, int64_t ld, int64_t N, double* __r
     lo[k] = INFINITY;
     hi[k] = -INFINITY;
   }
-  for (int64_t p = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; p < N;
-       p += static_cast<int64_t>(gridDim.x) * 256) {
+  // kBoxUnroll points in flight per thread; out-of-range ones repeat point
+  // N - 1, which changes no extreme
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * 256;
+  for (int64_t pb = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; pb < N;
+       pb += kBoxUnroll * stride) {
+    double x[kBoxUnroll][D];
 #pragma unroll
-    for (int k = 0; k < D; ++k) {
-      const double x = pos[k * ld + p];
-      if (isfinite(x)) {
-        lo[k] = fmin(lo[k], x);
-        hi[k] = fmax(hi[k], x);
-      }
+    for (int u = 0; u < kBoxUnroll; ++u) {
+      const int64_t p = min(pb + u * stride, N - 1);
+#pragma unroll
+      for (int k = 0; k < D; ++k) x[u][k] = pos[k * ld + p];
     }
+#pragma unroll
+    for (int u = 0; u < kBoxUnroll; ++u)
+#pragma unroll
+      for (int k = 0; k < D; ++k)
+        if (isfinite(x[u][k])) {
+          lo[k] = fmin(lo[k], x[u][k]);
+          hi[k] = fmax(hi[k], x[u][k]);
+        }
   }
 #pragma unroll
   for (int k = 0; k < D; ++k) {
@@ -194,6 +205,86 @@ k_box_partial(const double* __restrict__ pos, int64_t ld, int64_t N, double* __r
     }
 }
 
+// One block of 256 threads: modes in order, split into smooth (FP32 local
+// phase) and rough (FP64 phase) lists by a block scan, so both keep the
+// original mode order; h is the cell width.  Thread 0 receives the list
+// lengths.
+template <int D>
+__device__ void split_modes(const ModeRec* __restrict__ modes, int M, const double* h,
+                            uint4* __restrict__ btpl, float4* __restrict__ amp,
+                            SmoothRec* __restrict__ sm, ModeRec* __restrict__ rough,
+                            int* n_smooth, int* n_rough) {
+  __shared__ int part[256];
+  __shared__ int ns_s, nr_s;
+  if (threadIdx.x == 0) ns_s = nr_s = 0;
+  __syncthreads();
+  for (int base = 0; base < M; base += 256) {
+    const int m = base + threadIdx.x;
+    ModeRec r{};
+    bool smooth = false;
+    if (m < M) {
+      r = modes[m];
+      double b = 0.0;
+#pragma unroll
+      for (int k = 0; k < D; ++k) b += fabs(r.kap[k]) * 0.5 * h[k];
+      smooth = b <= kRoughRev && isfinite(r.th);
+#pragma unroll
+      for (int c = 0; c < D; ++c) smooth = smooth && isfinite(static_cast<double>(r.amp[c]));
+    }
+    part[threadIdx.x] = smooth ? 1 : 0;
+    __syncthreads();
+    for (int s = 1; s < 256; s <<= 1) {
+      const int v = threadIdx.x >= s ? part[threadIdx.x - s] : 0;
+      __syncthreads();
+      part[threadIdx.x] += v;
+      __syncthreads();
+    }
+    const int ns0 = ns_s, nr0 = nr_s;
+    const int before = part[threadIdx.x] - (smooth ? 1 : 0);  // smooth modes before m in block
+    if (m < M) {
+      if (smooth) {
+        const int i = ns0 + before;
+        // B template of the mode: the kap parts in their slots, r0 slots zero
+        uint32_t w[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) w[q] = 0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          uint16_t kp[3];
+          split3(static_cast<float>(r.kap[d]), kp);
+#pragma unroll
+          for (int t = 0; t < 6; ++t) {
+            const int k = 6 * d + t;
+            w[k >> 1] |= static_cast<uint32_t>(kp[kKPart[t]]) << (16 * (k & 1));
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          btpl[4 * i + q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
+        amp[i] = make_float4(r.amp[0], r.amp[1], D == 3 ? r.amp[2] : 0.0f, 0.0f);
+        sm[i] = SmoothRec{{r.kap[0], r.kap[1], r.kap[2]}, r.th};
+      } else {
+        rough[nr0 + (threadIdx.x - before)] = r;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 255) {
+      const int blk = min(256, M - base);
+      ns_s = ns0 + part[255];
+      nr_s = nr0 + blk - part[255];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    *n_smooth = ns_s;
+    *n_rough = nr_s;
+  }
+  // the main kernel reads B fragments of up to kModeChunk modes past the
+  // last smooth one (with zero amplitude): keep them finite
+  for (int i = threadIdx.x; i < 4 * kModeChunk; i += 256)
+    btpl[4 * ns_s + i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
 // Cell grid from the bounding box: near-cubic cells over the dimensions with
 // a non-zero extent, about N / kCellPts of them, or more where the modes
 // need it: the cell width h is kept below 2 kCellRev / s_q, s_q the (1 -
@@ -203,7 +294,10 @@ k_box_partial(const double* __restrict__ pos, int64_t ld, int64_t N, double* __r
 template <int D>
 __global__ void __launch_bounds__(256)
 k_bin_params(const double* __restrict__ box, int64_t N, int64_t max_cells,
-             const ModeRec* __restrict__ modes, int M, BinParams* __restrict__ prm) {
+             const ModeRec* __restrict__ modes, int M, BinParams* __restrict__ prm,
+             uint4* __restrict__ btpl, float4* __restrict__ amp, SmoothRec* __restrict__ sm,
+             ModeRec* __restrict__ rough) {
+  __shared__ BinParams Ps;
   __shared__ double red[2 * D][256];
   __shared__ int qh[kQBins];
   for (int b = threadIdx.x; b < kQBins; b += 256) qh[b] = 0;
@@ -264,59 +358,65 @@ k_bin_params(const double* __restrict__ box, int64_t N, int64_t max_cells,
     }
     __syncthreads();
   }
-  if (threadIdx.x != 0) return;
-  const double s_q = s_q_s;
-  double lo[3] = {0.0, 0.0, 0.0}, ext[3] = {0.0, 0.0, 0.0};
-  for (int k = 0; k < D; ++k) {
-    double a = red[2 * k][0], b = red[2 * k + 1][0];
-    if (!(a <= b)) a = b = 0.0;  // no finite coordinate
-    lo[k] = a;
-    ext[k] = b - a;
-  }
-  double vol = 1.0;
-  int active = 0;
-  for (int k = 0; k < D; ++k)
-    if (ext[k] > 0.0) {
-      vol *= ext[k];
-      ++active;
+  if (threadIdx.x == 0) {
+    const double s_q = s_q_s;
+    double lo[3] = {0.0, 0.0, 0.0}, ext[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < D; ++k) {
+      double a = red[2 * k][0], b = red[2 * k + 1][0];
+      if (!(a <= b)) a = b = 0.0;  // no finite coordinate
+      lo[k] = a;
+      ext[k] = b - a;
     }
-  double target = fmax(1.0, static_cast<double>(N) / kCellPts);
-  if (active > 0 && s_q > 0.0)
-    target = fmax(target, vol / pow(2.0 * kCellRev / s_q, static_cast<double>(active)));
-  target = fmin(target, static_cast<double>(max_cells));
-  int n[3] = {1, 1, 1};
-  if (active > 0) {
-    const double h = pow(vol / target, 1.0 / active);
+    double vol = 1.0;
+    int active = 0;
     for (int k = 0; k < D; ++k)
-      if (ext[k] > 0.0) n[k] = static_cast<int>(fmin(fmax(ceil(ext[k] / h), 1.0), 65535.0));
-  }
-  // coarse cells of f_k cells per dimension, at most 8 per dimension; the
-  // numbering covers whole coarse cells, so it may exceed n0 n1 n2
-  int f[3], C[3];
-  for (;;) {
-    int64_t total = 1;
-    for (int k = 0; k < 3; ++k) {
-      f[k] = (n[k] + 7) / 8;
-      C[k] = (n[k] + f[k] - 1) / f[k];
-      total *= static_cast<int64_t>(f[k]) * C[k];
+      if (ext[k] > 0.0) {
+        vol *= ext[k];
+        ++active;
+      }
+    double target = fmax(1.0, static_cast<double>(N) / kCellPts);
+    if (active > 0 && s_q > 0.0)
+      target = fmax(target, vol / pow(2.0 * kCellRev / s_q, static_cast<double>(active)));
+    target = fmin(target, static_cast<double>(max_cells));
+    int n[3] = {1, 1, 1};
+    if (active > 0) {
+      const double h = pow(vol / target, 1.0 / active);
+      for (int k = 0; k < D; ++k)
+        if (ext[k] > 0.0) n[k] = static_cast<int>(fmin(fmax(ceil(ext[k] / h), 1.0), 65535.0));
     }
-    if (total <= max_cells) break;
-    for (int k = 0; k < 3; ++k) n[k] = n[k] > 1 ? n[k] * 4 / 5 : 1;
+    // coarse cells of f_k cells per dimension, at most 8 per dimension; the
+    // numbering covers whole coarse cells, so it may exceed n0 n1 n2
+    int f[3], C[3];
+    for (;;) {
+      int64_t total = 1;
+      for (int k = 0; k < 3; ++k) {
+        f[k] = (n[k] + 7) / 8;
+        C[k] = (n[k] + f[k] - 1) / f[k];
+        total *= static_cast<int64_t>(f[k]) * C[k];
+      }
+      if (total <= max_cells) break;
+      for (int k = 0; k < 3; ++k) n[k] = n[k] > 1 ? n[k] * 4 / 5 : 1;
+    }
+    BinParams P{};
+    for (int k = 0; k < 3; ++k) {
+      P.lo[k] = lo[k];
+      P.n[k] = n[k];
+      P.f[k] = f[k];
+      P.C[k] = C[k];
+      P.h[k] = ext[k] > 0.0 ? ext[k] / n[k] : 1.0;
+      P.inv_h[k] = ext[k] > 0.0 ? 1.0 / P.h[k] : 0.0;
+      P.inv_f[k] = 1.0 / f[k];
+    }
+    P.F = f[0] * f[1] * f[2];
+    P.ncoarse = C[0] * C[1] * C[2];
+    P.ncells = P.ncoarse * P.F;
+    Ps = P;
   }
-  BinParams P{};
-  for (int k = 0; k < 3; ++k) {
-    P.lo[k] = lo[k];
-    P.n[k] = n[k];
-    P.f[k] = f[k];
-    P.C[k] = C[k];
-    P.h[k] = ext[k] > 0.0 ? ext[k] / n[k] : 1.0;
-    P.inv_h[k] = ext[k] > 0.0 ? 1.0 / P.h[k] : 0.0;
-    P.inv_f[k] = 1.0 / f[k];
-  }
-  P.F = f[0] * f[1] * f[2];
-  P.ncoarse = C[0] * C[1] * C[2];
-  P.ncells = P.ncoarse * P.F;
-  *prm = P;
+  __syncthreads();
+  // the modes' smooth / rough split depends on the cell width
+  split_modes<D>(modes, M, Ps.h, btpl, amp, sm, rough, &Ps.n_smooth, &Ps.n_rough);
+  __syncthreads();
+  if (threadIdx.x == 0) *prm = Ps;
 }
 
 // Cell of point x: index i_k = floor((x_k - lo_k) / h_k) per dimension
@@ -342,10 +442,24 @@ __device__ __forceinline__ int cell_locate(const BinParams& P, const double* x, 
   return cc * P.F + l;
 }
 
+// Positions of points pb + u kSortThreads (u < kSortUnroll), clamped to
+// plast so that every load is in bounds.
+template <int D>
+__device__ __forceinline__ void load_points(const double* __restrict__ pos, int64_t ld, int64_t pb,
+                                            int64_t plast, double (&x)[kSortUnroll][D]) {
+#pragma unroll
+  for (int u = 0; u < kSortUnroll; ++u) {
+    const int64_t p = min(pb + u * kSortThreads, plast);
+#pragma unroll
+    for (int k = 0; k < D; ++k) x[u][k] = pos[k * ld + p];
+  }
+}
+
 // Point block b (points [b blk_pts, (b + 1) blk_pts)) counts its points per
 // cell in LDS; bcount[b][c] receives the counts.  1024 threads, kSortUnroll
-// points in flight per thread: the 64 KB counter array allows two blocks per
-// CU, so the memory latency is hidden by width, not by more blocks.
+// points per thread and round.  Whole rounds (every point in range) run
+// without branches and load the next round's positions before the current
+// round's atomics; the last, partial round checks each point.
 template <int D>
 __global__ void __launch_bounds__(kSortThreads)
 k_bin_hist(const double* __restrict__ pos, int64_t ld, int64_t N, int64_t blk_pts,
@@ -354,21 +468,34 @@ k_bin_hist(const double* __restrict__ pos, int64_t ld, int64_t N, int64_t blk_pt
   const BinParams P = *prm;
   for (int c = threadIdx.x; c < P.ncells; c += kSortThreads) hist[c] = 0;
   __syncthreads();
+  constexpr int64_t kRound = kSortThreads * kSortUnroll;
   const int64_t p0 = static_cast<int64_t>(blockIdx.x) * blk_pts;
   const int64_t p1 = min(p0 + blk_pts, N);
-  for (int64_t pb = p0 + threadIdx.x; pb < p1; pb += kSortThreads * kSortUnroll) {
-    double x[kSortUnroll][D];
+  int64_t base = p0;
+  if (base + kRound <= p1) {
+    double x[kSortUnroll][D], xn[kSortUnroll][D];
+    load_points<D>(pos, ld, base + threadIdx.x, p1 - 1, xn);
+    for (; base + kRound <= p1; base += kRound) {
 #pragma unroll
-    for (int u = 0; u < kSortUnroll; ++u) {
-      const int64_t p = min(pb + u * kSortThreads, p1 - 1);
+      for (int u = 0; u < kSortUnroll; ++u)
 #pragma unroll
-      for (int k = 0; k < D; ++k) x[u][k] = pos[k * ld + p];
+        for (int k = 0; k < D; ++k) x[u][k] = xn[u][k];
+      load_points<D>(pos, ld, base + kRound + threadIdx.x, p1 - 1, xn);
+#pragma unroll
+      for (int u = 0; u < kSortUnroll; ++u) {
+        double X[3];
+        atomicAdd(&hist[cell_locate<D>(P, x[u], X)], 1);
+      }
     }
+  }
 #pragma unroll
-    for (int u = 0; u < kSortUnroll; ++u) {
-      double X[3];
-      const int c = cell_locate<D>(P, x[u], X);
-      if (pb + u * kSortThreads < p1) atomicAdd(&hist[c], 1);
+  for (int u = 0; u < kSortUnroll; ++u) {
+    const int64_t p = base + threadIdx.x + u * kSortThreads;
+    if (p < p1) {
+      double x[D], X[3];
+#pragma unroll
+      for (int k = 0; k < D; ++k) x[k] = pos[k * ld + p];
+      atomicAdd(&hist[cell_locate<D>(P, x, X)], 1);
     }
   }
   __syncthreads();
@@ -495,10 +622,21 @@ k_bin_waves(const BinParams* __restrict__ prm, const int* __restrict__ counts,
   for (int s = s0 + counts[e]; s < s1; ++s) recs[s] = make_rec(0.0f, 0.0f, 0.0f, -1);
 }
 
-// Point block b again (same threads and points as k_bin_hist): ranks its
+// Point block b again (same threads and rounds as k_bin_hist): ranks its
 // points within each cell with LDS atomics (starting from the block's base in
 // the cell) and writes their records (local coordinates, index) to their
 // slots.
+template <int D>
+__device__ __forceinline__ void scatter_point(const BinParams& P, const double* x, int64_t p,
+                                              int* rank, float4* __restrict__ recs) {
+  double X[3];
+  const int e = cell_locate<D>(P, x, X);  // the cell k_bin_hist counted it in
+  float xi[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int k = 0; k < D; ++k) xi[k] = static_cast<float>(x[k] - X[k]);
+  recs[atomicAdd(&rank[e], 1)] = make_rec(xi[0], xi[1], xi[2], static_cast<int>(p));
+}
+
 template <int D>
 __global__ void __launch_bounds__(kSortThreads)
 k_bin_scatter(const double* __restrict__ pos, int64_t ld, int64_t N, int64_t blk_pts,
@@ -506,109 +644,37 @@ k_bin_scatter(const double* __restrict__ pos, int64_t ld, int64_t N, int64_t blk
               const int* __restrict__ offsets, float4* __restrict__ recs) {
   __shared__ int rank[kMaxCells];
   const BinParams P = *prm;
-  const int* base = bcount + static_cast<int64_t>(blockIdx.x) * P.ncells;
-  for (int e = threadIdx.x; e < P.ncells; e += kSortThreads) rank[e] = offsets[e] + base[e];
+  const int* cbase = bcount + static_cast<int64_t>(blockIdx.x) * P.ncells;
+  for (int e = threadIdx.x; e < P.ncells; e += kSortThreads) rank[e] = offsets[e] + cbase[e];
   __syncthreads();
+  constexpr int64_t kRound = kSortThreads * kSortUnroll;
   const int64_t p0 = static_cast<int64_t>(blockIdx.x) * blk_pts;
   const int64_t p1 = min(p0 + blk_pts, N);
-  for (int64_t pb = p0 + threadIdx.x; pb < p1; pb += kSortThreads * kSortUnroll) {
-    double x[kSortUnroll][D];
+  int64_t base = p0;
+  if (base + kRound <= p1) {
+    double x[kSortUnroll][D], xn[kSortUnroll][D];
+    load_points<D>(pos, ld, base + threadIdx.x, p1 - 1, xn);
+    for (; base + kRound <= p1; base += kRound) {
 #pragma unroll
-    for (int u = 0; u < kSortUnroll; ++u) {
-      const int64_t p = min(pb + u * kSortThreads, p1 - 1);
+      for (int u = 0; u < kSortUnroll; ++u)
 #pragma unroll
-      for (int k = 0; k < D; ++k) x[u][k] = pos[k * ld + p];
-    }
+        for (int k = 0; k < D; ++k) x[u][k] = xn[u][k];
+      load_points<D>(pos, ld, base + kRound + threadIdx.x, p1 - 1, xn);
 #pragma unroll
-    for (int u = 0; u < kSortUnroll; ++u) {
-      const int64_t p = pb + u * kSortThreads;
-      double X[3];
-      const int e = cell_locate<D>(P, x[u], X);  // the cell k_bin_hist counted it in
-      float xi[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-      for (int k = 0; k < D; ++k) xi[k] = static_cast<float>(x[u][k] - X[k]);
-      if (p < p1) recs[atomicAdd(&rank[e], 1)] = make_rec(xi[0], xi[1], xi[2], static_cast<int>(p));
+      for (int u = 0; u < kSortUnroll; ++u)
+        scatter_point<D>(P, x[u], base + threadIdx.x + u * kSortThreads, rank, recs);
     }
   }
-}
-
-// One block: modes in order, split into smooth (FP32 local phase) and rough
-// (FP64 phase) lists by a block scan, so both keep the original mode order.
-template <int D>
-__global__ void __launch_bounds__(256)
-k_mode_split(const ModeRec* __restrict__ modes, int M, BinParams* __restrict__ prm,
-             uint4* __restrict__ btpl, float4* __restrict__ amp, SmoothRec* __restrict__ sm,
-             ModeRec* __restrict__ rough) {
-  __shared__ int part[256];
-  __shared__ int ns_s, nr_s;
-  const BinParams P = *prm;
-  if (threadIdx.x == 0) ns_s = nr_s = 0;
-  __syncthreads();
-  for (int base = 0; base < M; base += 256) {
-    const int m = base + threadIdx.x;
-    ModeRec r{};
-    bool smooth = false;
-    if (m < M) {
-      r = modes[m];
-      double b = 0.0;
 #pragma unroll
-      for (int k = 0; k < D; ++k) b += fabs(r.kap[k]) * 0.5 * P.h[k];
-      smooth = b <= kRoughRev && isfinite(r.th);
+  for (int u = 0; u < kSortUnroll; ++u) {
+    const int64_t p = base + threadIdx.x + u * kSortThreads;
+    if (p < p1) {
+      double x[D];
 #pragma unroll
-      for (int c = 0; c < D; ++c) smooth = smooth && isfinite(static_cast<double>(r.amp[c]));
+      for (int k = 0; k < D; ++k) x[k] = pos[k * ld + p];
+      scatter_point<D>(P, x, p, rank, recs);
     }
-    part[threadIdx.x] = smooth ? 1 : 0;
-    __syncthreads();
-    for (int s = 1; s < 256; s <<= 1) {
-      const int v = threadIdx.x >= s ? part[threadIdx.x - s] : 0;
-      __syncthreads();
-      part[threadIdx.x] += v;
-      __syncthreads();
-    }
-    const int ns0 = ns_s, nr0 = nr_s;
-    const int before = part[threadIdx.x] - (smooth ? 1 : 0);  // smooth modes before m in block
-    if (m < M) {
-      if (smooth) {
-        const int i = ns0 + before;
-        // B template of the mode: the kap parts in their slots, r0 slots zero
-        uint32_t w[16];
-#pragma unroll
-        for (int q = 0; q < 16; ++q) w[q] = 0;
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          uint16_t kp[3];
-          split3(static_cast<float>(r.kap[d]), kp);
-#pragma unroll
-          for (int t = 0; t < 6; ++t) {
-            const int k = 6 * d + t;
-            w[k >> 1] |= static_cast<uint32_t>(kp[kKPart[t]]) << (16 * (k & 1));
-          }
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-          btpl[4 * i + q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
-        amp[i] = make_float4(r.amp[0], r.amp[1], D == 3 ? r.amp[2] : 0.0f, 0.0f);
-        sm[i] = SmoothRec{{r.kap[0], r.kap[1], r.kap[2]}, r.th};
-      } else {
-        rough[nr0 + (threadIdx.x - before)] = r;
-      }
-    }
-    __syncthreads();
-    if (threadIdx.x == 255) {
-      const int blk = min(256, M - base);
-      ns_s = ns0 + part[255];
-      nr_s = nr0 + blk - part[255];
-    }
-    __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    prm->n_smooth = ns_s;
-    prm->n_rough = nr_s;
-  }
-  // the main kernel reads B fragments of up to kModeChunk modes past the
-  // last smooth one (with zero amplitude): keep them finite
-  for (int i = threadIdx.x; i < 4 * kModeChunk; i += 256)
-    btpl[4 * ns_s + i] = make_uint4(0u, 0u, 0u, 0u);
 }
 
 // ---- main kernel -------------------------------------------------------------
@@ -1009,7 +1075,8 @@ hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table
 #define GSC_BIN(D)                                                                               \
   do {                                                                                           \
     k_box_partial<D><<<kBoxBlocks, 256, 0, s>>>(pos, pos_ld, N, box);                            \
-    k_bin_params<D><<<1, 256, 0, s>>>(box, N, L.max_cells, modes, Mi, prm);                      \
+    k_bin_params<D><<<1, 256, 0, s>>>(box, N, L.max_cells, modes, Mi, prm, btpl, ampt, sm,       \
+                                      rough);                                                    \
     k_bin_hist<D><<<nblk, kSortThreads, 0, s>>>(pos, pos_ld, N, L.blk_pts, prm, bcount);         \
     k_col_sum<<<dim3(cb, ngroups), 256, 0, s>>>(prm, nblk, bcount, gsum);                        \
     k_col_groups<<<cb, 256, 0, s>>>(prm, ngroups, gsum, counts);                                 \
@@ -1018,7 +1085,6 @@ hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table
     k_bin_waves<D><<<cb, 256, 0, s>>>(prm, counts, offsets, centre, wave_cell, recs);            \
     k_bin_scatter<D><<<nblk, kSortThreads, 0, s>>>(pos, pos_ld, N, L.blk_pts, prm, bcount,       \
                                                    offsets, recs);                               \
-    k_mode_split<D><<<1, 256, 0, s>>>(modes, Mi, prm, btpl, ampt, sm, rough);                    \
     if (vec && f32)                                                                              \
       k_direct_local_vec<D, float><<<vb, 256, 0, s>>>(prm, centre, wave_cell, recs, btpl, ampt,  \
                                                       sm, rough, pos, pos_ld,                    \
