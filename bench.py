@@ -46,6 +46,20 @@ def cpu_baseline(w, n_sample: int) -> dict:
     t0 = time.perf_counter()
     oracle.summate(md.cov_samples, md.z1, md.z2, pos, num_threads=threads)
     dt = time.perf_counter() - t0
+    # one thread on a sixteenth of the sample: the per-core rate beside it
+    n1 = max(1, n_sample // 16)
+    t0 = time.perf_counter()
+    oracle.summate(md.cov_samples, md.z1, md.z2, np.ascontiguousarray(pos[:, :n1]), num_threads=1)
+    dt1 = time.perf_counter() - t0
+    cpu_model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {
         "value": n_sample * md.m / dt,
         "unit": "point·modes/s",
@@ -53,6 +67,9 @@ def cpu_baseline(w, n_sample: int) -> dict:
         "kind": "port",
         "sample": f"first {n_sample} points of the workload x {md.m} modes, FP64 oracle "
                   f"(C restatement of field.rs:53-62, OpenMP), {dt:.2f} s wall",
+        "single_core": {"value": n1 * md.m / dt1,
+                        "sample": f"first {n1} points x {md.m} modes, 1 thread, {dt1:.2f} s wall"},
+        "host": {"cpu_model": cpu_model, "cpu_count": os.cpu_count()},
     }
 
 

@@ -22,7 +22,11 @@ no CPU fallback.  Deliberate differences from the reference:
 num_threads is accepted with the reference's meaning (worker threads); here
 it bounds the host threads that stage strided inputs.
 
-Structured-grid and ensemble entry points live in gstools_core.grid.
+Extensions: summate_structured (the field on a structured grid, here or
+in gstools_core.grid with its workspace form), gstools_core.ensemble.Ensemble
+(many realisations in one graph-captured plan), gstools_core.randmeth
+(RandMeth / IncomprRandMeth caller glue), and the kriging calculators
+calc_field_krige_and_variance / calc_field_krige (krige.rs:24-118).
 """
 from __future__ import annotations
 
@@ -34,8 +38,8 @@ from . import _native
 
 __version__ = "1.1.0"
 
-__all__ = ["summate", "summate_incompr", "summate_fourier", "calc_field_krige_and_variance",
-           "calc_field_krige", "__version__"]
+__all__ = ["summate", "summate_incompr", "summate_fourier", "summate_structured",
+           "calc_field_krige_and_variance", "calc_field_krige", "__version__"]
 
 
 def _array_arg(name: str, a, ndim: int):
@@ -229,3 +233,15 @@ def calc_field_krige(krige_mat, krig_vecs, cond, num_threads=None, *, device=-1)
     m, v, c, nt = _krige_args(krige_mat, krig_vecs, cond, num_threads)
     return _krige(m, v, c, nt, device, False)[0]
 
+
+
+def summate_structured(cov_samples, z1, z2, axes, num_threads=None, **options):
+    """``summate`` on the structured grid axes[0] x ... x axes[d-1] (d in 1..3),
+    without expanding the grid into points: the field in C order, shape
+    (len(axes[0]), ...).  The keyword-only options (path, precision,
+    out_dtype, scale, device, accum, variant) are those of
+    ``gstools_core.grid.summate_grid``, which this calls.
+    """
+    from .grid import summate_grid
+
+    return summate_grid(cov_samples, z1, z2, axes, num_threads, **options)

@@ -47,6 +47,17 @@ def test_version_and_defaults():
     assert gstools_core.__version__ == "1.1.0"
 
 
+def test_structured_entry_point_signature():
+    import inspect
+
+    assert "summate_structured" in gstools_core.__all__
+    ps = list(inspect.signature(gstools_core.summate_structured).parameters)
+    assert ps[:5] == ["cov_samples", "z1", "z2", "axes", "num_threads"]
+    with pytest.raises(ValueError):  # argument checks run before any device work
+        gstools_core.summate_structured(np.zeros((3, 4)), np.zeros(4), np.zeros(4),
+                                        [np.arange(3.0), np.arange(2.0)])
+
+
 def test_workspace_bytes():
     valu = device.workspace_bytes(3, 1000, 10**6, variant="valu")
     assert valu >= 48 * 1000 and valu % 256 == 0

@@ -81,6 +81,17 @@ def test_strided_axes_and_f32_output():
     assert _scaled_err(g.ravel() / 0.5, ref, 200) <= 1e-5
 
 
+def test_summate_structured_is_the_grid_entry_point():
+    cov, z1, z2 = _modes(3, 300, 4.0, 6)
+    axes = [np.arange(40.0), np.arange(0.0, 30.0, 0.5), np.arange(33.0)]
+    a = gstools_core.summate_structured(cov, z1, z2, axes)
+    b = grid.summate_grid(cov, z1, z2, axes)
+    np.testing.assert_array_equal(a, b)
+    f = gstools_core.summate_structured(cov, z1, z2, axes, out_dtype=np.float32, path="direct")
+    ref = oracle.summate(cov, z1, z2, oracle.grid_points(axes))
+    assert f.dtype == np.float32 and _scaled_err(f.ravel(), ref, 300) <= 1e-5
+
+
 def test_c3_grid_256_factored_and_direct():
     w = workloads.c3()
     md = w.modes
