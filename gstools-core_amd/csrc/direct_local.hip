@@ -9,18 +9,19 @@
 // 16 modes per v_mfma_f32_16x16x32_bf16:
 //   D[i][j] = sum_k A[i][k] B[k][j]
 // with xi and kap split into three bf16 parts each (x = h + m + l, 24
-// significant bits) and K holding, per dimension, the six products of parts
-// whose orders sum to at most 2 (h h, h m, m h, h l, m m, l h): 18 of the 32
-// slots in 3-D.  r0 enters as the FP32 accumulator input C.  Every product of
-// two bf16 parts is exact in FP32; the dropped products are below 2^-24 of
-// the phase.  The B fragments depend on the mode only and are read from one
-// table shared by all waves; a wave keeps (r0, amplitude) of its cell for
-// one chunk of modes in LDS.  The VALU is left with one v_sin_f32 (argument in
-// revolutions) and one FMA per point-mode (on gfx950 the matrix instructions
-// do not run beside the VALU, so the 16 cycles of one MFMA per 256
-// point-modes are the whole cost of the phases).  FP32 partial sums over
-// kModeChunk modes are added into a second FP32 set (one add per chunk and
-// point-mode column); the 16 columns of a point are summed in FP64.
+// significant bits) and K holding, per dimension d, the six products of parts
+// whose orders sum to at most 2 (h h, h m, m h, h l, m m, l h) in slots
+// 8 d .. 8 d + 5: the eight slots a lane holds belong to one coordinate, so
+// a lane splits only that coordinate of its point.  r0 enters as the FP32
+// accumulator input C.  Every product of two bf16 parts is exact in FP32; the
+// dropped products are below 2^-24 of the phase.  The B fragments depend on
+// the mode only and are read from one table shared by all waves; a wave keeps
+// (r0, amplitude) of its cell for one chunk of modes in LDS.  The VALU is left
+// with one v_sin_f32 (argument in revolutions) and one FMA per point-mode; the
+// matrix instructions run only partly beside that VALU stream on gfx950 (about
+// half of their 16 cycles per 256 point-modes stay exposed).  FP32 partial
+// sums over kModeChunk modes are added into a second FP32 set (one add per
+// chunk and point-mode column); the 16 columns of a point are summed in FP64.
 //
 // Modes whose local phase could exceed kRoughRev revolutions in some cell
 // (sum_k |kap_k| h_k / 2 > kRoughRev, e.g. the far tail of a heavy-tailed
@@ -53,8 +54,8 @@ namespace {
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
-// K slots of the split phase product, per dimension d: slots 6 d + s hold
-// (x part kXPart[s]) * (kap part kKPart[s]); the rest are zero.
+// K slots of the split phase product, per dimension d: slots 8 d + s (s < 6)
+// hold (x part kXPart[s]) * (kap part kKPart[s]); the rest are zero.
 constexpr int kXPart[6] = {0, 0, 1, 0, 1, 2};
 constexpr int kKPart[6] = {0, 1, 0, 2, 1, 0};
 
@@ -66,6 +67,21 @@ __device__ __forceinline__ void split3(float x, uint16_t (&p)[3]) {
     p[i] = __builtin_bit_cast(uint16_t, h);
     r = r - static_cast<float>(h);
   }
+}
+
+// A fragment (16 points x 32 K slots, this lane's 8 slots) of the point
+// whose record starts at rec: lane group kg = lane / 16 holds slots
+// 8 kg .. 8 kg + 7, the split parts of coordinate kg (zero for kg >= D).
+template <int D>
+__device__ __forceinline__ bf8 a_fragment(const float* rec, int kg) {
+  const float x = rec[kg];  // kg = 3: the index field, not used
+  uint16_t p[3];
+  split3(kg < D ? x : 0.0f, p);
+  uint16_t v[8];
+#pragma unroll
+  for (int t = 0; t < 6; ++t) v[t] = p[kXPart[t]];
+  v[6] = v[7] = 0;
+  return __builtin_bit_cast(bf8, v);
 }
 
 constexpr int kCellPts = 2048;      // target points per cell
@@ -254,7 +270,7 @@ __device__ void split_modes(const ModeRec* __restrict__ modes, int M, const doub
           split3(static_cast<float>(r.kap[d]), kp);
 #pragma unroll
           for (int t = 0; t < 6; ++t) {
-            const int k = 6 * d + t;
+            const int k = 8 * d + t;
             w[k >> 1] |= static_cast<uint32_t>(kp[kKPart[t]]) << (16 * (k & 1));
           }
         }
@@ -279,10 +295,14 @@ __device__ void split_modes(const ModeRec* __restrict__ modes, int M, const doub
     *n_smooth = ns_s;
     *n_rough = nr_s;
   }
-  // the main kernel reads B fragments of up to kModeChunk modes past the
-  // last smooth one (with zero amplitude): keep them finite
+  // the main kernels read the tables up to kModeChunk modes past the last
+  // smooth one: zero phase data, zero amplitude and finite B fragments there
   for (int i = threadIdx.x; i < 4 * kModeChunk; i += 256)
     btpl[4 * ns_s + i] = make_uint4(0u, 0u, 0u, 0u);
+  for (int i = threadIdx.x; i < kModeChunk; i += 256) {
+    amp[ns_s + i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    sm[ns_s + i] = SmoothRec{{0.0, 0.0, 0.0}, 0.0};
+  }
 }
 
 // Cell grid from the bounding box: near-cubic cells over the dimensions with
@@ -710,21 +730,7 @@ k_direct_local(const BinParams* __restrict__ prm, const double* __restrict__ cen
   const float* rf = reinterpret_cast<const float*>(recs + static_cast<int64_t>(w) * kWavePts);
   bf8 a[4];
 #pragma unroll
-  for (int g = 0; g < 4; ++g) {
-    uint16_t xp[3][3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) split3(d < D ? rf[(16 * g + j) * 4 + d] : 0.0f, xp[d]);
-    uint16_t sl[32];  // all 32 slots of the point, then the lane's eight
-#pragma unroll
-    for (int k = 0; k < 32; ++k) {
-      sl[k] = k < 6 * D ? xp[k / 6][kXPart[k % 6]] : 0;
-    }
-    uint16_t v[8];
-#pragma unroll
-    for (int e = 0; e < 8; ++e)
-      v[e] = kg == 0 ? sl[e] : (kg == 1 ? sl[8 + e] : (kg == 2 ? sl[16 + e] : sl[24 + e]));
-    a[g] = __builtin_bit_cast(bf8, v);
-  }
+  for (int g = 0; g < 4; ++g) a[g] = a_fragment<D>(rf + (16 * g + j) * 4, kg);
   // per-chunk tables in the wave's LDS region: the MFMA C operand of every
   // mode (r0 replicated over the four rows a lane holds) and its amplitude
   f4* rq = reinterpret_cast<f4*>(lds[wv]);
@@ -743,18 +749,14 @@ k_direct_local(const BinParams* __restrict__ prm, const double* __restrict__ cen
     // four records in flight keep the register peak below the main loop's
 #pragma unroll 4
     for (int r = 0; r < kModeChunk / 64; ++r) {
-      const int i = c0 + 64 * r + lane;
-      float r0 = 0.0f, am = 0.0f;
-      if (i < n_smooth) {
-        const SmoothRec sr = sm[i];
-        double t = __builtin_fma(sr.kap[0], X[0], sr.th);
-        t = __builtin_fma(sr.kap[1], X[1], t);
-        if constexpr (D == 3) t = __builtin_fma(sr.kap[2], X[2], t);
-        r0 = static_cast<float>(t - rint(t));
-        am = ampt[i].x;
-      }
+      const int i = c0 + 64 * r + lane;  // past n_smooth: zero padding
+      const SmoothRec sr = sm[i];
+      double t = __builtin_fma(sr.kap[0], X[0], sr.th);
+      t = __builtin_fma(sr.kap[1], X[1], t);
+      if constexpr (D == 3) t = __builtin_fma(sr.kap[2], X[2], t);
+      const float r0 = static_cast<float>(t - rint(t));
       rq[64 * r + lane] = f4{r0, r0, r0, r0};
-      ram[64 * r + lane] = am;
+      ram[64 * r + lane] = ampt[i].x;
     }
     // software pipeline: the phases of tile q + 1 are issued before the
     // sines of tile q, so the MFMA results are ready when they are read; the
@@ -762,10 +764,13 @@ k_direct_local(const BinParams* __restrict__ prm, const double* __restrict__ cen
     // B fragments are requested two tiles ahead of their MFMAs; the reads
     // run at most 47 modes past n_smooth, inside the zeroed padding.
     const int ntile = min(kModeChunk / 16, (n_smooth - c0 + 15) / 16);
-    const uint4* bp = btpl + 4 * (c0 + j) + kg;  // lane's k-range of mode c0 + j
+    // lane's k-range of mode c0 + j: a uniform base that steps by whole
+    // tiles plus a fixed 32-bit lane offset
+    const uint4* bp = btpl + 4 * c0;
+    const unsigned bo = 4u * j + kg;
     const f4* rp = rq + j;
     const float* mp = ram + j;
-    uint4 ba = bp[0], bb = bp[64];
+    uint4 ba = bp[bo], bb = bp[bo + 64];
     bp += 128;
     f4 ta[4], tb[4];
     float ama, amb;
@@ -778,7 +783,7 @@ k_direct_local(const BinParams* __restrict__ prm, const double* __restrict__ cen
       for (int g = 0; g < 4; ++g)
         t[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[g], __builtin_bit_cast(bf8, bq), c, 0, 0,
                                                        0);
-      bq = *bp;
+      bq = bp[bo];
       bp += 64;
     };
     auto consume = [&](const f4 (&t)[4], float am) {
@@ -901,19 +906,7 @@ k_direct_local_vec(const BinParams* __restrict__ prm, const double* __restrict__
       reinterpret_cast<const float*>(recs + static_cast<int64_t>(rg) * kWavePts + 32 * (w & 1));
   bf8 a[2];
 #pragma unroll
-  for (int g = 0; g < 2; ++g) {
-    uint16_t xp[3][3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) split3(d < D ? rf[(16 * g + j) * 4 + d] : 0.0f, xp[d]);
-    uint16_t sl[32];
-#pragma unroll
-    for (int k = 0; k < 32; ++k) sl[k] = k < 6 * D ? xp[k / 6][kXPart[k % 6]] : 0;
-    uint16_t v[8];
-#pragma unroll
-    for (int e = 0; e < 8; ++e)
-      v[e] = kg == 0 ? sl[e] : (kg == 1 ? sl[8 + e] : (kg == 2 ? sl[16 + e] : sl[24 + e]));
-    a[g] = __builtin_bit_cast(bf8, v);
-  }
+  for (int g = 0; g < 2; ++g) a[g] = a_fragment<D>(rf + (16 * g + j) * 4, kg);
   f4* rq = reinterpret_cast<f4*>(lds[wv]);
   f4* ram = rq + kVecChunk;
   float acc[D][2][4], tot[D][2][4];
@@ -929,25 +922,22 @@ k_direct_local_vec(const BinParams* __restrict__ prm, const double* __restrict__
   for (int c0 = 0; c0 < n_smooth; c0 += kVecChunk) {
 #pragma unroll 2
     for (int r = 0; r < kVecChunk / 64; ++r) {
-      const int i = c0 + 64 * r + lane;
-      float r0 = 0.0f;
-      float4 am = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      if (i < n_smooth) {
-        const SmoothRec sr = sm[i];
-        double t = __builtin_fma(sr.kap[0], X[0], sr.th);
-        t = __builtin_fma(sr.kap[1], X[1], t);
-        if constexpr (D == 3) t = __builtin_fma(sr.kap[2], X[2], t);
-        r0 = static_cast<float>(t - rint(t));
-        am = ampt[i];
-      }
+      const int i = c0 + 64 * r + lane;  // past n_smooth: zero padding
+      const SmoothRec sr = sm[i];
+      double t = __builtin_fma(sr.kap[0], X[0], sr.th);
+      t = __builtin_fma(sr.kap[1], X[1], t);
+      if constexpr (D == 3) t = __builtin_fma(sr.kap[2], X[2], t);
+      const float r0 = static_cast<float>(t - rint(t));
+      const float4 am = ampt[i];
       rq[64 * r + lane] = f4{r0, r0, r0, r0};
       ram[64 * r + lane] = f4{am.x, am.y, am.z, 0.0f};
     }
     const int ntile = min(kVecChunk / 16, (n_smooth - c0 + 15) / 16);
-    const uint4* bp = btpl + 4 * (c0 + j) + kg;
+    const uint4* bp = btpl + 4 * c0;
+    const unsigned bo = 4u * j + kg;
     const f4* rp = rq + j;
     const f4* mp = ram + j;
-    uint4 ba = bp[0], bb = bp[64];
+    uint4 ba = bp[bo], bb = bp[bo + 64];
     bp += 128;
     f4 ta[2], tb[2];
     f4 ama, amb;
@@ -960,7 +950,7 @@ k_direct_local_vec(const BinParams* __restrict__ prm, const double* __restrict__
       for (int g = 0; g < 2; ++g)
         t[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[g], __builtin_bit_cast(bf8, bq), cq, 0, 0,
                                                        0);
-      bq = *bp;
+      bq = bp[bo];
       bp += 64;
     };
     auto consume = [&](const f4 (&t)[2], const f4& am) {
