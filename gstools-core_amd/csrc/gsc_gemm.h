@@ -236,6 +236,41 @@ struct TileJob {
   double scale;
 };
 
+// Output of one wave's 64 x 32 JN tile (rows past nvalid and columns past
+// ncols are not stored).
+template <int ACC, int JN, typename OutT, typename A2>
+__device__ __forceinline__ void gemm_store(OutT* __restrict__ out, const TileJob& jb,
+                                           const f32x16_t (&acc)[2][JN], const A2& acc2,
+                                           int wave, int hl, int lr) {
+  const int64_t ldo = jb.ldo;
+  const int nvalid = static_cast<int>(jb.nvalid < kBlockRows ? jb.nvalid : kBlockRows);
+  const float scale32 = static_cast<float>(jb.scale);
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int r0 = 64 * wave + 32 * i + 4 * hl;
+#pragma unroll
+    for (int j = 0; j < JN; ++j) {
+      const int64_t col = jb.col0 + 32 * j + lr;
+      if (col >= jb.ncols) continue;
+      OutT* const p = out + (jb.out_row0 + r0) * ldo + col;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int rr = 8 * (v >> 2) + (v & 3);
+        if (r0 + rr >= nvalid) continue;
+        OutT val;
+        if constexpr (ACC == ACC_F64) {
+          val = static_cast<OutT>(acc2[i][j][v] * jb.scale);
+        } else {
+          const float a = ACC == ACC_F32 ? acc[i][j][v] : static_cast<float>(acc2[i][j][v]);
+          if constexpr (sizeof(OutT) == 8) val = static_cast<double>(a) * jb.scale;
+          else val = a * scale32;
+        }
+        __builtin_nontemporal_store(val, p + rr * ldo);
+      }
+    }
+  }
+}
+
 template <typename T, int NS, bool KR, int ACC, int JN>
 __device__ __forceinline__ void gemm_tile(GemmSmem<T, NS, JN>& sm, const TileJob& jb) {
   using V8 = typename Fmt<T>::v8;
@@ -431,25 +466,12 @@ __device__ __forceinline__ void gemm_tile(GemmSmem<T, NS, JN>& sm, const TileJob
   }
   __syncthreads();  // the caller may reuse the LDS tile for its next tile
 
-  // ---- epilogue: D[8 (v/4) + 4 (lane/32) + v%4][lane%32] of each 32 x 32 block
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < JN; ++j) {
-      const int64_t col = jb.col0 + 32 * j + lr;
-      if (col >= jb.ncols) continue;
-#pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        const int r = 64 * wave + 32 * i + 8 * (v >> 2) + 4 * hl + (v & 3);
-        if (r >= jb.nvalid) continue;
-        double val;
-        if constexpr (ACC == ACC_F32) val = static_cast<double>(acc[i][j][v]);
-        else val = static_cast<double>(acc2[i][j][v]);
-        const int64_t o = (jb.out_row0 + r) * jb.ldo + col;
-        if (jb.out_f64) static_cast<double*>(jb.out)[o] = val * jb.scale;
-        else static_cast<float*>(jb.out)[o] = static_cast<float>(val * jb.scale);
-      }
-    }
+  // ---- epilogue: D[8 (v/4) + 4 (lane/32) + v%4][lane%32] of each 32 x 32 block.
+  // One pointer per column block; the row steps 8 (v/4) + v%4 are uniform
+  // multiples of ldo.  FP32 output is scaled in FP32 (the sums are FP32
+  // already unless ACC_F64).
+  if (jb.out_f64) gemm_store<ACC, JN>(static_cast<double*>(jb.out), jb, acc, acc2, wave, hl, lr);
+  else gemm_store<ACC, JN>(static_cast<float*>(jb.out), jb, acc, acc2, wave, hl, lr);
 }
 
 // Tile job of row tile rt, column tile ct of one contraction.
