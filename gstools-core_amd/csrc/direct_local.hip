@@ -569,36 +569,65 @@ k_col_fix(const BinParams* __restrict__ prm, int nblk, int* __restrict__ bcount,
   }
 }
 
+// Exclusive scan of a[0, n) in place (n <= 4 kSortThreads) by all
+// kSortThreads threads: four entries per thread, wave scans with shuffles,
+// one LDS word per wave.  Returns the total.
+__device__ __forceinline__ int block_scan4(int* a, int n, int* wsum) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  int v[4], s = 0;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int i = 4 * t + u;
+    v[u] = i < n ? a[i] : 0;
+    s += v[u];
+  }
+  int x = s;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int y = __shfl_up(x, d);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  int pre = 0, total = 0;
+#pragma unroll
+  for (int k = 0; k < kSortThreads / 64; ++k) {
+    pre += k < w ? wsum[k] : 0;
+    total += wsum[k];
+  }
+  int run = pre + x - s;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int i = 4 * t + u;
+    if (i < n) a[i] = run;
+    run += v[u];
+  }
+  __syncthreads();
+  return total;
+}
+
 // One block: cell c gets ceil(count / kWavePts) waves starting at slot
 // offsets[c]; offsets[ncells] and the wave count close the list.
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(kSortThreads)
 k_bin_offsets(BinParams* __restrict__ prm, const int* __restrict__ counts,
               int* __restrict__ offsets) {
-  __shared__ int part[1024];
-  __shared__ int carry_s;
+  __shared__ int wv[4 * kSortThreads];
+  __shared__ int wsum[kSortThreads / 64];
   const int ncells = prm->ncells;
-  if (threadIdx.x == 0) carry_s = 0;
-  __syncthreads();
-  for (int base = 0; base < ncells; base += 1024) {
-    const int c = base + threadIdx.x;
-    const int waves = c < ncells ? (counts[c] + kWavePts - 1) / kWavePts : 0;
-    part[threadIdx.x] = waves;
+  int carry = 0;  // uniform
+  for (int base = 0; base < ncells; base += 4 * kSortThreads) {
+    const int n = min(4 * kSortThreads, ncells - base);
+    for (int i = threadIdx.x; i < n; i += kSortThreads)
+      wv[i] = (counts[base + i] + kWavePts - 1) / kWavePts;
     __syncthreads();
-    for (int st = 1; st < 1024; st <<= 1) {  // inclusive scan (Hillis-Steele)
-      const int v = threadIdx.x >= st ? part[threadIdx.x - st] : 0;
-      __syncthreads();
-      part[threadIdx.x] += v;
-      __syncthreads();
-    }
-    const int carry = carry_s;
-    if (c < ncells) offsets[c] = (carry + part[threadIdx.x] - waves) * kWavePts;
-    __syncthreads();
-    if (threadIdx.x == 1023) carry_s = carry + part[1023];
+    const int tw = block_scan4(wv, n, wsum);
+    for (int i = threadIdx.x; i < n; i += kSortThreads) offsets[base + i] = (carry + wv[i]) * kWavePts;
+    carry += tw;
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    offsets[ncells] = carry_s * kWavePts;
-    prm->nwaves = carry_s;
+    offsets[ncells] = carry * kWavePts;
+    prm->nwaves = carry;
   }
 }
 
@@ -622,24 +651,26 @@ __device__ __forceinline__ void cell_centre(const BinParams& P, int e, double (&
   }
 }
 
-// Thread per cell: records its centre and its waves in the wave -> cell map
-// and marks the slots past its points as padding.
+// One wave per cell: records its centre and its waves in the wave -> cell
+// map and marks the slots past its points as padding.
 template <int D>
 __global__ void __launch_bounds__(256)
 k_bin_waves(const BinParams* __restrict__ prm, const int* __restrict__ counts,
             const int* __restrict__ offsets, double* __restrict__ centre,
-            int* __restrict__ wave_cell,
-            float4* __restrict__ recs) {
-  const int e = blockIdx.x * 256 + threadIdx.x;
+            int* __restrict__ wave_cell, float4* __restrict__ recs) {
+  const int e = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   const BinParams P = *prm;
   if (e >= P.ncells) return;
-  double X[3];
-  cell_centre<D>(P, e, X);
+  if (lane == 0) {
+    double X[3];
+    cell_centre<D>(P, e, X);
 #pragma unroll
-  for (int k = 0; k < 3; ++k) centre[4 * e + k] = X[k];
+    for (int k = 0; k < 3; ++k) centre[4 * e + k] = X[k];
+  }
   const int s0 = offsets[e], s1 = offsets[e + 1];
-  for (int s = s0; s < s1; s += kWavePts) wave_cell[s / kWavePts] = e;
-  for (int s = s0 + counts[e]; s < s1; ++s) recs[s] = make_rec(0.0f, 0.0f, 0.0f, -1);
+  for (int s = s0 + kWavePts * lane; s < s1; s += kWavePts * 64) wave_cell[s / kWavePts] = e;
+  const int sp = s0 + counts[e] + lane;  // padding: fewer than kWavePts slots
+  if (sp < s1) recs[sp] = make_rec(0.0f, 0.0f, 0.0f, -1);
 }
 
 // Point block b again (same threads and rounds as k_bin_hist): ranks its
@@ -1058,6 +1089,7 @@ hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table
   const int nblk = static_cast<int>(L.nblk);
   const int ngroups = (nblk + kColBlocks - 1) / kColBlocks;
   const unsigned cb = static_cast<unsigned>((L.max_cells + 255) / 256);
+  const unsigned wb = static_cast<unsigned>((L.max_cells + 3) / 4);  // a wave per cell
   const unsigned mb = static_cast<unsigned>((L.max_waves + 3) / 4);
   const unsigned vb = static_cast<unsigned>((2 * L.max_waves + 3) / 4);
   const int Mi = static_cast<int>(M);
@@ -1071,8 +1103,8 @@ hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table
     k_col_sum<<<dim3(cb, ngroups), 256, 0, s>>>(prm, nblk, bcount, gsum);                        \
     k_col_groups<<<cb, 256, 0, s>>>(prm, ngroups, gsum, counts);                                 \
     k_col_fix<<<dim3(cb, ngroups), 256, 0, s>>>(prm, nblk, bcount, gsum);                        \
-    k_bin_offsets<<<1, 1024, 0, s>>>(prm, counts, offsets);                                      \
-    k_bin_waves<D><<<cb, 256, 0, s>>>(prm, counts, offsets, centre, wave_cell, recs);            \
+    k_bin_offsets<<<1, kSortThreads, 0, s>>>(prm, counts, offsets);                              \
+    k_bin_waves<D><<<wb, 256, 0, s>>>(prm, counts, offsets, centre, wave_cell, recs);            \
     k_bin_scatter<D><<<nblk, kSortThreads, 0, s>>>(pos, pos_ld, N, L.blk_pts, prm, bcount,       \
                                                    offsets, recs);                               \
     if (vec && f32)                                                                              \
