@@ -365,7 +365,8 @@ def main() -> int:
         legs.discard("ensemble")
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the summation shards by points with no exchange; see DESIGN.md)", file=sys.stderr)
+              "(the summation would shard by points, with no exchange between GPUs)",
+              file=sys.stderr)
         return 2
 
     import numpy as np
@@ -501,6 +502,20 @@ def main() -> int:
         v_ms = tot_v / max(3, args.steps // 2)
         direct_variants[key] = {"ms_per_step": v_ms, "point_modes_per_s": n * m / (v_ms * 1e-3)}
 
+    # the drop-in on host arrays (numpy in, numpy out): the same sum plus the
+    # PCIe copies of 24 N bytes in and 8 N bytes out; wall clock per call
+    import gstools_core
+
+    host_pos = np.ascontiguousarray(w.pos)
+    gstools_core.summate(md.cov_samples, md.z1, md.z2, host_pos)
+    calls = 5
+    t_h = time.perf_counter()
+    for _ in range(calls):
+        gstools_core.summate(md.cov_samples, md.z1, md.z2, host_pos)
+    h_ms = (time.perf_counter() - t_h) * 1e3 / calls
+    host_arrays = {"ms_per_call": h_ms, "point_modes_per_s": n * m / (h_ms * 1e-3),
+                   "note": "gstools_core.summate on numpy arrays, wall clock, copies included"}
+
     cpu = None
     if args.cpu_sample > 0:
         log(f"cpu baseline on {args.cpu_sample} points")
@@ -525,6 +540,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "direct_variants": direct_variants,
+        "host_arrays": host_arrays,
         "incompr": inc_res,
         "structured_grid": grid_res,
         "heavy_tail_grid": c5_res,
