@@ -906,7 +906,8 @@ int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes
   el.counters = reinterpret_cast<int*>(base + c_desc + c_fact + c_ts + c_dir + c_bs + c_tk + c_gs);
   el.dim = dim;
   el.max_tiles = max_tiles;
-  const int64_t bpc = o->blocks_per_cu > 0 ? o->blocks_per_cu : 1;
+  // the FP64-flush GEMM runs two blocks per CU (it fits 256 registers)
+  const int64_t bpc = o->blocks_per_cu > 0 ? o->blocks_per_cu : (o->accum == GSC_ACCUM_F64 ? 2 : 1);
   el.gemm_blocks = (int)std::max<int64_t>(1, std::min<int64_t>(tile_start.back(), bpc * cus));
   el.direct_blocks = (int)std::max<int64_t>(1, std::min<int64_t>(blk_start.back(), 8 * (int64_t)cus));
 

@@ -81,6 +81,14 @@ constexpr int cols_blocks() {
   return ACC == ACC_F64 ? 1 : (NS == 3 ? 2 : 4);
 }
 
+// Waves per SIMD the GEMM kernels are built for: the one-column-block FP64
+// flush variant fits 256 registers, so a second block hides the first one's
+// barrier and load waits; the wider tiles need more than 256 registers.
+template <int ACC>
+constexpr int gemm_waves() {
+  return ACC == ACC_F64 ? 2 : 1;
+}
+
 static_assert(32 * cols_blocks<2, ACC_F32X2>() == gemm_tile_cols(2, ACC_F32X2) &&
                   32 * cols_blocks<3, ACC_F32>() == gemm_tile_cols(3, ACC_F32) &&
                   32 * cols_blocks<1, ACC_F64>() == gemm_tile_cols(1, ACC_F64),
@@ -491,7 +499,8 @@ __device__ __forceinline__ TileJob make_job(const uint32_t* A, int64_t arows, in
 }
 
 template <typename T, int NS, bool KR, int ACC>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(gemm_waves<ACC>(), gemm_waves<ACC>())))
 k_factored_gemm(GemmArgs g) {
   constexpr int JN = cols_blocks<NS, ACC>();
   __shared__ __attribute__((aligned(16))) GemmSmem<T, NS, JN> sm;
@@ -529,7 +538,8 @@ k_ens_tables(const EnsReal* __restrict__ d, const int* __restrict__ fact,
 // and, once its group is exhausted, helps the others in turn.  Every group
 // counter only grows, so every block leaves every group and exits.
 template <typename T, int NS, bool KR, int ACC>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(gemm_waves<ACC>(), gemm_waves<ACC>())))
 k_ens_gemm(const EnsReal* __restrict__ d, const int* __restrict__ fact,
            const int64_t* __restrict__ tile_start, int nfact,
            const int64_t* __restrict__ group_start, int* __restrict__ counters, bool out_f64) {

@@ -99,7 +99,8 @@ typedef struct gsc_opts {
   int32_t num_threads;     /* host threads for staging strided arrays, 0 = all */
   int32_t accum;           /* gsc_accum, factored path only */
   int32_t direct_variant;  /* gsc_direct_variant, direct kernels only */
-  int32_t blocks_per_cu;   /* persistent factored-GEMM blocks per CU in ensembles, 0 = 1 */
+  int32_t blocks_per_cu;   /* persistent factored-GEMM blocks per CU in ensembles, 0 = auto
+                              (2 with GSC_ACCUM_F64, else 1) */
   double scale;            /* multiplies every output value (1.0 = reference) */
 } gsc_opts;
 

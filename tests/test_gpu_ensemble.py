@@ -120,6 +120,26 @@ def test_many_realisations_match_single_grid_calls(dim):
     ens.close()
 
 
+def test_fp64_flush_plan_two_blocks_per_cu():
+    # the FP64-flush GEMM runs two persistent blocks per CU by default; every
+    # realisation equals its single-grid evaluation with the same accumulation
+    rng = np.random.default_rng(77)
+    count = 20
+    shapes = [tuple(int(v) for v in rng.integers(30, 260, size=2)) for _ in range(count)]
+    ms = [int(v) for v in rng.integers(16, 600, size=count)]
+    reals = _reals(2, shapes, ms, 78)
+    ens = Ensemble(reals, paths=["factored"] * count, out_dtype=np.float64, precision="fp16x3",
+                   accum="f64")
+    singles = [grid.summate_grid(cov, z1, z2, axes, path="factored", precision="fp16x3",
+                                 accum="f64") for cov, z1, z2, axes in reals]
+    for rep in range(2):
+        ens.run()
+        for i, (x, y) in enumerate(zip(ens.results(), singles)):
+            np.testing.assert_allclose(x, y, rtol=0, atol=1e-9 * np.sqrt(ms[i]),
+                                       err_msg=f"replay {rep} realisation {i}")
+    ens.close()
+
+
 def test_c4_realisations_in_one_plan():
     ens_w = workloads.c4(count=512)[:24]
     reals = [(w.modes.cov_samples, w.modes.z1, w.modes.z2, w.axes) for w in ens_w]
