@@ -58,6 +58,7 @@ typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 typedef float f2v __attribute__((ext_vector_type(2)));
+typedef float f4v __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kLdB = 40;    // LDS row of the B tile: 32 k + 8 pad (T elements, 80 bytes)
@@ -135,9 +136,19 @@ __device__ __forceinline__ f2v table_value(const ModeRec& md, int axis, double x
   return f2v{amp * __builtin_amdgcn_cosf(r), amp * __builtin_amdgcn_sinf(r)};
 }
 
+// Global-memory view of a pointer.  The ensemble passes its tables through
+// descriptors in device memory; without the address space the compiler
+// would use flat accesses, whose waits also cover the LDS counter.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gmem(T* p) {
+  return (__attribute__((address_space(1))) T*)(p);  // address-space conversion
+}
+
 // Four table entries: role 0 -> S (3-D axis 0), 1 -> A (split), 2 -> C; row
-// `row` of that table, modes 16 c + 4 g .. + 3 (g < 4).  Each role's store
-// is one 16-byte vector per part, consecutive rows contiguous for A and C.
+// `row` of that table, modes 16 c + 4 g .. + 3 (g < 4).  The four modes'
+// data are loaded together (index clamped to M - 1, entries past M are
+// zero), then evaluated.  Each role's store is one 16-byte vector per part,
+// consecutive rows contiguous for A and C.
 template <typename T, int NS>
 __device__ __forceinline__ void table_quad(const ModeRec* __restrict__ modes, int M, int dim,
                                            int nch, const double* __restrict__ xs,
@@ -147,7 +158,7 @@ __device__ __forceinline__ void table_quad(const ModeRec* __restrict__ modes, in
                                            int64_t ncp, float2* __restrict__ S,
                                            uint32_t* __restrict__ A, float2* __restrict__ C,
                                            int role, int64_t row, int c, int g) {
-  if (c >= nch) return;
+  if (c >= nch || M <= 0) return;
   const int m0 = c * 16 + 4 * g;
   const int64_t cc = c;
   int axis;
@@ -158,40 +169,56 @@ __device__ __forceinline__ void table_quad(const ModeRec* __restrict__ modes, in
     axis = 0;
     amp = true;
     valid = true;
-    x = xs[row];
+    x = gmem(xs)[row];
   } else if (role == 1) {
     if (row >= arows) return;
     axis = dim - 2;
     amp = dim == 2;
     valid = row < nrow;
-    if (valid) x = xa[row];
+    if (valid) x = gmem(xa)[row];
   } else {
     if (row >= ncp) return;
     axis = dim - 1;
     amp = false;
     valid = row < ncols;
-    if (valid) x = xc[row];
+    if (valid) x = gmem(xc)[row];
+  }
+  const __attribute__((address_space(1))) ModeRec* gm = gmem(modes);
+  double kx[4], th[4];
+  float am[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int m = min(m0 + e, M - 1);
+    kx[e] = gm[m].kap[axis];
+    th[e] = gm[m].th;
+    am[e] = gm[m].amp[0];
   }
   f2v v[4];
 #pragma unroll
-  for (int e = 0; e < 4; ++e)
-    v[e] = (valid && m0 + e < M) ? table_value(modes[m0 + e], axis, x, amp) : f2v{0.0f, 0.0f};
+  for (int e = 0; e < 4; ++e) {
+    double t = kx[e] * x;
+    if (amp) t += th[e];
+    const float r = static_cast<float>(t - rint(t));
+    const float a = amp ? am[e] : 1.0f;
+    const f2v w = f2v{a * __builtin_amdgcn_cosf(r), a * __builtin_amdgcn_sinf(r)};
+    v[e] = (valid && m0 + e < M) ? w : f2v{0.0f, 0.0f};
+  }
   if (role == 0) {
-    float4* d = reinterpret_cast<float4*>(S + (cc * na + row) * 16 + 4 * g);
-    d[0] = make_float4(v[0].x, v[0].y, v[1].x, v[1].y);
-    d[1] = make_float4(v[2].x, v[2].y, v[3].x, v[3].y);
+    auto* d = gmem(reinterpret_cast<f4v*>(S + (cc * na + row) * 16 + 4 * g));
+    d[0] = f4v{v[0].x, v[0].y, v[1].x, v[1].y};
+    d[1] = f4v{v[2].x, v[2].y, v[3].x, v[3].y};
   } else if (role == 1) {
     uint32_t o[4][NS];
 #pragma unroll
     for (int e = 0; e < 4; ++e) split_pair<T, NS>(v[e], o[e]);
 #pragma unroll
     for (int s = 0; s < NS; ++s)
-      *reinterpret_cast<u32x4*>(A + ((((s * nch + cc) * 4) + g) * arows + row) * 4) =
+      *gmem(reinterpret_cast<u32x4*>(A + ((((s * nch + cc) * 4) + g) * arows + row) * 4)) =
           u32x4{o[0][s], o[1][s], o[2][s], o[3][s]};
   } else {
-    float4* d = reinterpret_cast<float4*>(C);
-    d[(8 * cc + 2 * g) * ncp + row] = make_float4(v[0].x, v[0].y, v[1].x, v[1].y);
-    d[(8 * cc + 2 * g + 1) * ncp + row] = make_float4(v[2].x, v[2].y, v[3].x, v[3].y);
+    auto* d = gmem(reinterpret_cast<f4v*>(C));
+    d[(8 * cc + 2 * g) * ncp + row] = f4v{v[0].x, v[0].y, v[1].x, v[1].y};
+    d[(8 * cc + 2 * g + 1) * ncp + row] = f4v{v[2].x, v[2].y, v[3].x, v[3].y};
   }
 }
 
