@@ -50,6 +50,9 @@ k_ens_direct(const EnsReal* __restrict__ d, const int* __restrict__ dir,
     if (t >= total) break;
     const int i = find_item(blk_start, ndir, t);
     const EnsReal& r = d[dir[i]];
+    // the descriptor's pointers are device (global) memory: their global
+    // views let the compiler use global instead of flat accesses
+    const auto ax0 = global_ptr(r.ax0), ax1 = global_ptr(r.ax1), ax2 = global_ptr(r.ax2);
     const int64_t base = (t - blk_start[i]) * kEnsPts + threadIdx.x;
     const int64_t N = r.npts;
     double x[kEnsP][D];
@@ -58,21 +61,21 @@ k_ens_direct(const EnsReal* __restrict__ d, const int* __restrict__ dir,
       int64_t j = base + p * 256;
       j = j < N ? j : N - 1;
       if constexpr (D == 2) {
-        x[p][0] = r.ax0[j / r.s1];
-        x[p][1] = r.ax1[j % r.s1];
+        x[p][0] = ax0[j / r.s1];
+        x[p][1] = ax1[j % r.s1];
       } else {
         const int64_t i0 = j / (r.s1 * r.s2), rem = j - i0 * (r.s1 * r.s2);
-        x[p][0] = r.ax0[i0];
-        x[p][1] = r.ax1[rem / r.s2];
-        x[p][2] = r.ax2[rem % r.s2];
+        x[p][0] = ax0[i0];
+        x[p][1] = ax1[rem / r.s2];
+        x[p][2] = ax2[rem % r.s2];
       }
     }
     double tot[kEnsP];
-    sum_modes_scalar<D, kEnsP>(r.table, static_cast<int>(r.M), x, tot);
+    sum_modes_scalar<D, kEnsP>(global_ptr(r.table), static_cast<int>(r.M), x, tot);
 #pragma unroll
     for (int p = 0; p < kEnsP; ++p) {
       const int64_t j = base + p * 256;
-      if (j < N) store(static_cast<OutT*>(r.out) + j, tot[p] * r.scale);
+      if (j < N) *(global_ptr(static_cast<OutT*>(r.out)) + j) = static_cast<OutT>(tot[p] * r.scale);
     }
   }
 }

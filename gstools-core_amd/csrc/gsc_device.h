@@ -77,15 +77,35 @@ __device__ __forceinline__ void prep_mode(int kind, int dim, int64_t M, int64_t 
 
 // Mode loop shared by the point-list and grid forms: tot[p] = sum over all
 // modes of amp0 sin(2 pi (kap . x_p + th)).
-template <int D, int P>
-__device__ __forceinline__ void sum_modes_scalar(const ModeRec* __restrict__ modes, int M,
+// Global-memory view of a pointer (for pointers read from device memory,
+// which the compiler would otherwise access with flat instructions).
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* global_ptr(T* p) {
+  return (__attribute__((address_space(1))) T*)(p);  // address-space conversion
+}
+
+// Mode record m read field by field (works for either pointer view).
+template <typename MP>
+__device__ __forceinline__ ModeRec load_mode(MP modes, int m) {
+  ModeRec r;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) r.kap[k] = modes[m].kap[k];
+  r.th = modes[m].th;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) r.amp[k] = modes[m].amp[k];
+  return r;
+}
+
+// MP: const ModeRec* or its global_ptr view.
+template <int D, int P, typename MP>
+__device__ __forceinline__ void sum_modes_scalar(MP __restrict__ modes, int M,
                                                  const double (&x)[P][D], double (&tot)[P]) {
 #pragma unroll
   for (int p = 0; p < P; ++p) tot[p] = 0.0;
   // Software-pipelined mode loads: the record for m + 1 is requested while
   // mode m is being summed, so the scalar-load latency hides behind VALU work.
   ModeRec nxt;
-  if (M > 0) nxt = modes[0];
+  if (M > 0) nxt = load_mode(modes, 0);
   for (int m0 = 0; m0 < M; m0 += kChunk) {
     const int m1 = min(m0 + kChunk, M);
     float acc[P];
@@ -93,7 +113,7 @@ __device__ __forceinline__ void sum_modes_scalar(const ModeRec* __restrict__ mod
     for (int p = 0; p < P; ++p) acc[p] = 0.0f;
     for (int m = m0; m < m1; ++m) {
       const ModeRec md = nxt;
-      nxt = modes[min(m + 1, M - 1)];
+      nxt = load_mode(modes, min(m + 1, M - 1));
 #pragma unroll
       for (int p = 0; p < P; ++p) {
         double t = __builtin_fma(md.kap[0], x[p][0], md.th);

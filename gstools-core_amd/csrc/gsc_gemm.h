@@ -287,7 +287,7 @@ __device__ __forceinline__ void gemm_store(OutT* __restrict__ out, const TileJob
     for (int j = 0; j < JN; ++j) {
       const int64_t col = jb.col0 + 32 * j + lr;
       if (col >= jb.ncols) continue;
-      OutT* const p = out + (jb.out_row0 + r0) * ldo + col;
+      __attribute__((address_space(1))) OutT* const p = gmem(out) + (jb.out_row0 + r0) * ldo + col;
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         const int rr = 8 * (v >> 2) + (v & 3);
@@ -319,7 +319,8 @@ __device__ __forceinline__ void gemm_tile(GemmSmem<T, NS, JN>& sm, const TileJob
   const int nch = jb.nch;
 
   // A fragments: row arow0 + 64 wave + 32 i + lr, packed pairs 8 ks + 4 hl .. + 3
-  const uint32_t* abase = jb.A + (hl * jb.arows + jb.arow0 + 64 * wave + lr) * 4;
+  const __attribute__((address_space(1))) uint32_t* abase =
+      gmem(jb.A) + (hl * jb.arows + jb.arow0 + 64 * wave + lr) * 4;
   const int64_t a_ks = 2 * jb.arows * 4;
   const int64_t a_chunk = 2 * a_ks;
   const int64_t a_part = a_chunk * nch;
@@ -329,27 +330,35 @@ __device__ __forceinline__ void gemm_tile(GemmSmem<T, NS, JN>& sm, const TileJob
   const int gcol = JN == 1 ? lr : lane;
   // mode pair 8 c + 2 wave + h of column col0 + gcol + 64 g: float4 index
   // (8 c + 2 wave + h) ncp + col0 + gcol + 64 g
-  const float4* cbase = reinterpret_cast<const float4*>(jb.C) + 2 * wave * jb.ncp + jb.col0 + gcol;
+  const __attribute__((address_space(1))) f4v* cbase =
+      gmem(reinterpret_cast<const f4v*>(jb.C)) + 2 * wave * jb.ncp + jb.col0 + gcol;
   const int64_t c_pair = jb.ncp;
   const int64_t c_chunk = 8 * jb.ncp;
   // S(a, m) of one chunk: lane l < 16 loads mode l with a vector load (a
   // scalar load would make every LDS wait in the loop wait for it too); the
   // wave takes its four modes with readlane
-  const float2* sbase = KR ? jb.S + jb.a * 16 + (lane & 15) : nullptr;
+  const __attribute__((address_space(1))) f2v* sbase =
+      KR ? gmem(reinterpret_cast<const f2v*>(jb.S)) + jb.a * 16 + (lane & 15) : nullptr;
   const int64_t s_chunk = jb.na * 16;
 
   // C values (and, in 3-D, the wave's four S values) of one chunk, loaded a
   // step before they are used
-  float4 cr[GC][2];
+  f4v cr[GC][2];
   float2 sl;
   auto gload = [&](int c) {
 #pragma unroll
     for (int g = 0; g < GC; ++g) {
-      const float4* p = cbase + c * c_chunk + 64 * g;
+      const __attribute__((address_space(1))) f4v* p = cbase + c * c_chunk + 64 * g;
       cr[g][0] = p[0];
       cr[g][1] = p[c_pair];
     }
-    if constexpr (KR) sl = sbase[c * s_chunk];
+    if constexpr (KR) {
+      // one 8-byte load, then plain float members for the readlanes below
+      // (bit-casting elements of a vector-typed sl lost its .y half)
+      const f2v t = sbase[c * s_chunk];
+      sl.x = t[0];
+      sl.y = t[1];
+    }
   };
   auto gstore = [&](const int buf) {
     float2 sv[4];
@@ -367,7 +376,7 @@ __device__ __forceinline__ void gemm_tile(GemmSmem<T, NS, JN>& sm, const TileJob
       uint32_t w[NS][4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const float4 v = cr[g][q >> 1];
+        const f4v v = cr[g][q >> 1];
         float re = (q & 1) ? v.z : v.x, im = (q & 1) ? v.w : v.y;
         if constexpr (KR) {
           const float r2 = sv[q].x * re - sv[q].y * im;
@@ -396,8 +405,8 @@ __device__ __forceinline__ void gemm_tile(GemmSmem<T, NS, JN>& sm, const TileJob
       for (int s = 0; s < NS; ++s)
 #pragma unroll
         for (int i = 0; i < 2; ++i)
-          af[set][ks][s][i] = *reinterpret_cast<const V8*>(abase + s * a_part + c * a_chunk +
-                                                           ks * a_ks + 128 * i);
+          af[set][ks][s][i] = *reinterpret_cast<const __attribute__((address_space(1))) V8*>(
+              abase + s * a_part + c * a_chunk + ks * a_ks + 128 * i);
   };
   V8 bf[2][NS][JN];  // [ks][part][column block]
   auto bread = [&](const int ks, const int buf) {
