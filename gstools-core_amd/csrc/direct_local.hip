@@ -905,8 +905,9 @@ __device__ __forceinline__ void sum_modes_vec(const ModeRec* __restrict__ modes,
 // k_direct_local, for 32 slots per wave (half a record group, two MFMA
 // groups), which leaves registers for D accumulator sets.  Wave w works on
 // slots 32 (w & 1) .. + 31 of record group w >> 1.
-constexpr int kVecChunk = 256;  // modes per (r0, amplitudes) table of a wave
+constexpr int kVecChunk = 512;  // modes per (r0, amplitudes) table of a wave pair
 constexpr int kVecRedLd = 33;   // LDS row of the cross-lane sums (doubles)
+static_assert(kVecChunk <= kModeChunk, "the mode tables are zero-padded by kModeChunk entries");
 template <int D, typename OutT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4)))
 k_direct_local_vec(const BinParams* __restrict__ prm, const double* __restrict__ centre,
@@ -915,16 +916,23 @@ k_direct_local_vec(const BinParams* __restrict__ prm, const double* __restrict__
                    const SmoothRec* __restrict__ sm, const ModeRec* __restrict__ rough,
                    const double* __restrict__ pos, int64_t ld, OutT* __restrict__ out,
                    int64_t out_ld, double scale) {
-  constexpr int kWaveLds = kVecChunk * 32 > 16 * kVecRedLd * 8 ? kVecChunk * 32
-                                                                 : 16 * kVecRedLd * 8;
-  __shared__ __attribute__((aligned(16))) char lds[4][kWaveLds];
+  // per wave pair (the two halves of one record group, one cell): the
+  // (r0, amplitude) table of one chunk of modes, each wave filling half of
+  // it, and afterwards the pair's cross-lane sums
+  constexpr int kPairLds = kVecChunk * 32 > 2 * 16 * kVecRedLd * 8 ? kVecChunk * 32
+                                                                    : 2 * 16 * kVecRedLd * 8;
+  __shared__ __attribute__((aligned(16))) char lds[2][kPairLds];
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int w = blockIdx.x * 4 + wv;
   const BinParams P = *prm;
-  if (w >= 2 * P.nwaves) return;
+  if (blockIdx.x * 4 >= 2 * P.nwaves) return;  // the whole block is past the list
+  // a pair past the list (in the last block) runs the loops on the last
+  // record group, for the block's barriers, and stores nothing
+  const bool live = w < 2 * P.nwaves;
   const int n_smooth = P.n_smooth, n_rough = P.n_rough;
-  const int rg = w >> 1;
+  const int rg = live ? (w >> 1) : P.nwaves - 1;
+  const int par = w & 1;
   double X[3];
   {
     const double* xc = centre + 4 * static_cast<int64_t>(wave_cell[rg]);
@@ -938,7 +946,7 @@ k_direct_local_vec(const BinParams* __restrict__ prm, const double* __restrict__
   bf8 a[2];
 #pragma unroll
   for (int g = 0; g < 2; ++g) a[g] = a_fragment<D>(rf + (16 * g + j) * 4, kg);
-  f4* rq = reinterpret_cast<f4*>(lds[wv]);
+  f4* rq = reinterpret_cast<f4*>(lds[wv >> 1]);
   f4* ram = rq + kVecChunk;
   float acc[D][2][4], tot[D][2][4];
 #pragma unroll
@@ -952,7 +960,7 @@ k_direct_local_vec(const BinParams* __restrict__ prm, const double* __restrict__
       }
   for (int c0 = 0; c0 < n_smooth; c0 += kVecChunk) {
 #pragma unroll 2
-    for (int r = 0; r < kVecChunk / 64; ++r) {
+    for (int r = par; r < kVecChunk / 64; r += 2) {  // this wave's half of the table
       const int i = c0 + 64 * r + lane;  // past n_smooth: zero padding
       const SmoothRec sr = sm[i];
       double t = __builtin_fma(sr.kap[0], X[0], sr.th);
@@ -963,6 +971,7 @@ k_direct_local_vec(const BinParams* __restrict__ prm, const double* __restrict__
       rq[64 * r + lane] = f4{r0, r0, r0, r0};
       ram[64 * r + lane] = f4{am.x, am.y, am.z, 0.0f};
     }
+    __syncthreads();  // both halves of the table written
     const int ntile = min(kVecChunk / 16, (n_smooth - c0 + 15) / 16);
     const uint4* bp = btpl + 4 * c0;
     const unsigned bo = 4u * j + kg;
@@ -1018,12 +1027,13 @@ k_direct_local_vec(const BinParams* __restrict__ prm, const double* __restrict__
           tot[c][g][v] += acc[c][g][v];
           acc[c][g][v] = 0.0f;
         }
+    __syncthreads();  // the pair's table is read; the next chunk (or the sums) overwrites it
   }
   // cross-lane sums, one component at a time: tot[c][g][v] is slot
   // 16 g + 4 (lane / 16) + v of the half, mode column j
-  const int p = lane < 32 ? __builtin_bit_cast(int, rf[lane * 4 + 3]) : -1;
+  const int p = live && lane < 32 ? __builtin_bit_cast(int, rf[lane * 4 + 3]) : -1;
   double comp[D];
-  double* red = reinterpret_cast<double*>(lds[wv]);
+  double* red = reinterpret_cast<double*>(lds[wv >> 1]) + par * 16 * kVecRedLd;
 #pragma unroll
   for (int c = 0; c < D; ++c) {
     __asm__ volatile("" ::: "memory");
