@@ -125,17 +125,6 @@ __device__ __forceinline__ void split_pair(f2v v, uint32_t (&o)[NS]) {
   }
 }
 
-// amp e^{2 pi i (kap_axis x + th)} of one mode at one coordinate (amp = 1,
-// th = 0 unless with_amp): FP64 phase in revolutions, exact reduction to
-// [-1/2, 1/2], then the FP32 cosine and sine instructions.
-__device__ __forceinline__ f2v table_value(const ModeRec& md, int axis, double x, bool with_amp) {
-  double t = md.kap[axis] * x;
-  if (with_amp) t += md.th;
-  const float r = static_cast<float>(t - rint(t));
-  const float amp = with_amp ? md.amp[0] : 1.0f;
-  return f2v{amp * __builtin_amdgcn_cosf(r), amp * __builtin_amdgcn_sinf(r)};
-}
-
 // Global-memory view of a pointer.  The ensemble passes its tables through
 // descriptors in device memory; without the address space the compiler
 // would use flat accesses, whose waits also cover the LDS counter.
@@ -144,11 +133,55 @@ __device__ __forceinline__ __attribute__((address_space(1))) T* gmem(T* p) {
   return (__attribute__((address_space(1))) T*)(p);  // address-space conversion
 }
 
+// Four table entries from their modes' data: v[e] = amp e^{2 pi i (kx x +
+// th)} of mode m0 + e (amp = 1, th = 0 unless with_amp), zero for modes past
+// M and for padding rows.  FP64 phase in revolutions, exact reduction to
+// [-1/2, 1/2], then the FP32 cosine and sine instructions.
+__device__ __forceinline__ void table_eval(const double (&kx)[4], const double (&th)[4],
+                                           const float (&am)[4], double x, bool with_amp,
+                                           bool valid, int m0, int M, f2v (&v)[4]) {
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    double t = kx[e] * x;
+    if (with_amp) t += th[e];
+    const float r = static_cast<float>(t - rint(t));
+    const float a = with_amp ? am[e] : 1.0f;
+    const f2v w = f2v{a * __builtin_amdgcn_cosf(r), a * __builtin_amdgcn_sinf(r)};
+    v[e] = (valid && m0 + e < M) ? w : f2v{0.0f, 0.0f};
+  }
+}
+
+// Stores of four entries (modes 16 c + 4 g .. + 3) of row `row` of the
+// role's table: role 0 -> S (3-D axis 0), 1 -> A (split), 2 -> C.  Each
+// role's store is one 16-byte vector per part, consecutive rows contiguous
+// for A and C.
+template <typename T, int NS>
+__device__ __forceinline__ void table_store(const f2v (&v)[4], int role, int64_t row, int64_t cc,
+                                            int g, int nch, int64_t na, int64_t arows,
+                                            int64_t ncp, float2* S, uint32_t* A, float2* C) {
+  if (role == 0) {
+    auto* d = gmem(reinterpret_cast<f4v*>(S + (cc * na + row) * 16 + 4 * g));
+    d[0] = f4v{v[0].x, v[0].y, v[1].x, v[1].y};
+    d[1] = f4v{v[2].x, v[2].y, v[3].x, v[3].y};
+  } else if (role == 1) {
+    uint32_t o[4][NS];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) split_pair<T, NS>(v[e], o[e]);
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      *gmem(reinterpret_cast<u32x4*>(A + ((((s * nch + cc) * 4) + g) * arows + row) * 4)) =
+          u32x4{o[0][s], o[1][s], o[2][s], o[3][s]};
+  } else {
+    auto* d = gmem(reinterpret_cast<f4v*>(C));
+    d[(8 * cc + 2 * g) * ncp + row] = f4v{v[0].x, v[0].y, v[1].x, v[1].y};
+    d[(8 * cc + 2 * g + 1) * ncp + row] = f4v{v[2].x, v[2].y, v[3].x, v[3].y};
+  }
+}
+
 // Four table entries: role 0 -> S (3-D axis 0), 1 -> A (split), 2 -> C; row
 // `row` of that table, modes 16 c + 4 g .. + 3 (g < 4).  The four modes'
-// data are loaded together (index clamped to M - 1, entries past M are
-// zero), then evaluated.  Each role's store is one 16-byte vector per part,
-// consecutive rows contiguous for A and C.
+// data are loaded together (index clamped to M - 1), then evaluated and
+// stored.
 template <typename T, int NS>
 __device__ __forceinline__ void table_quad(const ModeRec* __restrict__ modes, int M, int dim,
                                            int nch, const double* __restrict__ xs,
@@ -160,7 +193,6 @@ __device__ __forceinline__ void table_quad(const ModeRec* __restrict__ modes, in
                                            int role, int64_t row, int c, int g) {
   if (c >= nch || M <= 0) return;
   const int m0 = c * 16 + 4 * g;
-  const int64_t cc = c;
   int axis;
   bool amp, valid;
   double x = 0.0;
@@ -194,32 +226,8 @@ __device__ __forceinline__ void table_quad(const ModeRec* __restrict__ modes, in
     am[e] = gm[m].amp[0];
   }
   f2v v[4];
-#pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    double t = kx[e] * x;
-    if (amp) t += th[e];
-    const float r = static_cast<float>(t - rint(t));
-    const float a = amp ? am[e] : 1.0f;
-    const f2v w = f2v{a * __builtin_amdgcn_cosf(r), a * __builtin_amdgcn_sinf(r)};
-    v[e] = (valid && m0 + e < M) ? w : f2v{0.0f, 0.0f};
-  }
-  if (role == 0) {
-    auto* d = gmem(reinterpret_cast<f4v*>(S + (cc * na + row) * 16 + 4 * g));
-    d[0] = f4v{v[0].x, v[0].y, v[1].x, v[1].y};
-    d[1] = f4v{v[2].x, v[2].y, v[3].x, v[3].y};
-  } else if (role == 1) {
-    uint32_t o[4][NS];
-#pragma unroll
-    for (int e = 0; e < 4; ++e) split_pair<T, NS>(v[e], o[e]);
-#pragma unroll
-    for (int s = 0; s < NS; ++s)
-      *gmem(reinterpret_cast<u32x4*>(A + ((((s * nch + cc) * 4) + g) * arows + row) * 4)) =
-          u32x4{o[0][s], o[1][s], o[2][s], o[3][s]};
-  } else {
-    auto* d = gmem(reinterpret_cast<f4v*>(C));
-    d[(8 * cc + 2 * g) * ncp + row] = f4v{v[0].x, v[0].y, v[1].x, v[1].y};
-    d[(8 * cc + 2 * g + 1) * ncp + row] = f4v{v[2].x, v[2].y, v[3].x, v[3].y};
-  }
+  table_eval(kx, th, am, x, amp, valid, m0, M, v);
+  table_store<T, NS>(v, role, row, c, g, nch, na, arows, ncp, S, A, C);
 }
 
 // All tables of one call in one launch.  block = 64 rows x 16 modes of one
@@ -549,22 +557,82 @@ k_factored_gemm(GemmArgs g) {
 // Tables of all factored realisations: one block per task (realisation,
 // role, group of 64 rows); the block walks every chunk of its rows.
 // task_start has 3 nfact + 1 entries: tasks of fact[i] / role w start at
-// task_start[3 i + w].
+// task_start[3 i + w].  The descriptor and the row coordinate are read once
+// and the modes' data for the role's axis are staged in LDS, kTabStage modes
+// at a time, so the chunk loop's only global accesses are its stores (the
+// vector memory counter counts stores too: a global load in the loop would
+// wait for the stores before it).
+constexpr int kTabStage = 512;
 template <typename T, int NS>
 __global__ void __launch_bounds__(256)
 k_ens_tables(const EnsReal* __restrict__ d, const int* __restrict__ fact,
              const int64_t* __restrict__ task_start, int nfact) {
+  __shared__ double2 s_kt[kTabStage];  // (slope along the role's axis, phase offset)
+  __shared__ float s_am[kTabStage];    // amplitude
   const int64_t task = blockIdx.x;
   const int k = find_item(task_start, 3 * nfact, task);
-  const EnsReal& r = d[fact[k / 3]];
+  const EnsReal r = d[fact[k / 3]];
+  const int M = static_cast<int>(r.M);
+  if (M <= 0) return;  // the whole block: M is the realisation's
   const int role = k % 3;
-  const int64_t row = (task - task_start[k]) * 64 + (threadIdx.x & 63);
   const int nch = r.mp / 16;
-  const double* xa = r.dim == 3 ? r.ax1 : r.ax0;
-  const double* xc = r.dim == 3 ? r.ax2 : r.ax1;
-  for (int c = 0; c < nch; ++c)
-    table_quad<T, NS>(r.table, static_cast<int>(r.M), r.dim, nch, r.ax0, xa, xc, r.na, r.nrow,
-                      r.arows, r.ncols, r.ncp, r.S, r.A, r.C, role, row, c, threadIdx.x >> 6);
+  const int g = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  const int64_t row = (task - task_start[k]) * 64 + (threadIdx.x & 63);
+  int axis;
+  bool amp;
+  const double* xp;
+  int64_t nvalid, nrows;
+  if (role == 0) {
+    axis = 0;
+    amp = true;
+    xp = r.ax0;
+    nvalid = r.na;
+    nrows = r.na;
+  } else if (role == 1) {
+    axis = r.dim - 2;
+    amp = r.dim == 2;
+    xp = r.dim == 3 ? r.ax1 : r.ax0;
+    nvalid = r.nrow;
+    nrows = r.arows;
+  } else {
+    axis = r.dim - 1;
+    amp = false;
+    xp = r.dim == 3 ? r.ax2 : r.ax1;
+    nvalid = r.ncols;
+    nrows = r.ncp;
+  }
+  const bool active = row < nrows;  // rows of the last S group past na store nothing
+  const bool valid = row < nvalid;  // padding rows of A and C hold zeros
+  const double x = valid ? gmem(xp)[row] : 0.0;
+  const __attribute__((address_space(1))) ModeRec* gm = gmem(r.table);
+  float2* S = r.S;
+  uint32_t* A = r.A;
+  float2* C = r.C;
+  for (int b0 = 0; b0 < 16 * nch; b0 += kTabStage) {
+    __syncthreads();  // the previous stage is read
+    for (int i = threadIdx.x; i < kTabStage; i += 256) {
+      const int m = min(b0 + i, M - 1);
+      s_kt[i] = double2{gm[m].kap[axis], gm[m].th};
+      s_am[i] = gm[m].amp[0];
+    }
+    __syncthreads();
+    const int c1 = min(nch, (b0 + kTabStage) / 16);
+    for (int c = b0 / 16; c < c1; ++c) {
+      const int m0 = 16 * c + 4 * g;
+      double kx[4], th[4];
+      float am[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const double2 kt = s_kt[m0 - b0 + e];
+        kx[e] = kt.x;
+        th[e] = kt.y;
+        am[e] = s_am[m0 - b0 + e];
+      }
+      f2v v[4];
+      table_eval(kx, th, am, x, amp, valid, m0, M, v);
+      if (active) table_store<T, NS>(v, role, row, c, g, nch, r.na, r.arows, r.ncp, S, A, C);
+    }
+  }
 }
 
 // Persistent GEMM over the tiles of all factored realisations.  The tiles
