@@ -458,9 +458,9 @@ def main() -> int:
         "unit": "Tsin/s",
         "frac": round(achieved / (TRANS_PEAK / 1e12), 4),
         "traffic": traffic,
-        "kernel": "gsc_summate_dev: cell sort + k_direct_local<3,double>",
+        "kernel": "gsc_summate_dev: cell sort + k_direct_local32<3,double>",
         "work_per_launch": f"{n}*{m} point-modes, 1 v_sin_f32 + 1 v_fma_f32 each on the VALU; "
-                           "phases on v_mfma_f32_16x16x32_bf16 (256 point-modes per instruction)",
+                           "phases on v_mfma_f32_32x32x16_f16 (1024 point-modes per instruction)",
         "traffic_note": "bytes per step (whole device call) from FETCH_SIZE x2 + WRITE_SIZE x1 "
                         "(copy-calibrated); algorithmic 320 MB",
     }

@@ -5,23 +5,28 @@
 //   t = kap . x + th = (kap . X_c + th) + kap . xi,   xi = x - X_c.
 // r0(c, m) = (kap . X_c + th) - rint(...) is evaluated in FP64 once per
 // (wave, mode) and rounded to FP32; kap . xi is small (|xi| <= half a cell)
-// and is computed, together with + r0, on the matrix cores for 16 points x
-// 16 modes per v_mfma_f32_16x16x32_bf16:
+// and is computed, together with + r0, on the matrix cores as a product
 //   D[i][j] = sum_k A[i][k] B[k][j]
-// with xi and kap split into three bf16 parts each (x = h + m + l, 24
-// significant bits) and K holding, per dimension d, the six products of parts
-// whose orders sum to at most 2 (h h, h m, m h, h l, m m, l h) in slots
-// 8 d .. 8 d + 5: the eight slots a lane holds belong to one coordinate, so
-// a lane splits only that coordinate of its point.  r0 enters as the FP32
-// accumulator input C.  Every product of two bf16 parts is exact in FP32; the
-// dropped products are below 2^-24 of the phase.  The B fragments depend on
-// the mode only and are read from one table shared by all waves; a wave keeps
-// (r0, amplitude) of its cell for one chunk of modes in LDS.  The VALU is left
-// with one v_sin_f32 (argument in revolutions) and one FMA per point-mode; the
-// matrix instructions run only partly beside that VALU stream on gfx950 (about
-// half of their 16 cycles per 256 point-modes stay exposed).  FP32 partial
+// of split factors whose parts are exact in FP32 products, in two forms:
+// - scalar fields (k_direct_local32): 32 points x 32 modes per
+//   v_mfma_f32_32x32x16_f16, with power-of-two scaled xi and kap split into
+//   two fp16 parts each (22 significant bits; products hi hi, hi lo, lo hi
+//   per dimension) and r0 split into two fp16 parts as two more K slots;
+// - incompressible fields (k_direct_local_vec): 16 points x 16 modes per
+//   v_mfma_f32_16x16x32_bf16, with xi and kap split into three bf16 parts
+//   each (24 significant bits) and K holding, per dimension d, the six
+//   products of parts whose orders sum to at most 2 (h h, h m, m h, h l,
+//   m m, l h) in slots 8 d .. 8 d + 5; r0 enters as the FP32 accumulator C.
+// The dropped products are below 2^-22 (fp16) or 2^-24 (bf16) of the phase.
+// The mode-only parts of the B fragments are read from tables shared by all
+// waves; a wave keeps (r0, amplitude) of its cell for one chunk of modes in
+// LDS.  The VALU is left with one v_sin_f32 (argument in revolutions) and one
+// FMA per point-mode (D for the vector form); on gfx950 the matrix
+// instructions hardly overlap that VALU stream, which is why the scalar form
+// uses the instruction with the fewest matrix cycles per phase.  FP32 partial
 // sums over kModeChunk modes are added into a second FP32 set (one add per
-// chunk and point-mode column); the 16 columns of a point are summed in FP64.
+// chunk and point-mode column); the mode columns of a point are summed in
+// FP64.
 //
 // Modes whose local phase could exceed kRoughRev revolutions in some cell
 // (sum_k |kap_k| h_k / 2 > kRoughRev, e.g. the far tail of a heavy-tailed
@@ -84,6 +89,53 @@ __device__ __forceinline__ bf8 a_fragment(const float* rec, int kg) {
   return __builtin_bit_cast(bf8, v);
 }
 
+// ---- fp16 form of the phase product (32 x 32 x 16 matrix instructions) ----
+// Coordinates are scaled per dimension by 2^-e with 2^e > h / 2 >= 2^(e-1)
+// (h the cell width), so |u| = |xi| 2^-e <= 1, and kap by 2^e, so
+// sum_k |kap'_k| <= 2 kRoughRev; both scalings are exact.  Each factor is
+// split into two fp16 parts (x = hi + lo + O(2^-22 |x|)) and, per dimension,
+// the three products hi hi, hi lo, lo hi go into K slots 3 d .. 3 d + 2
+// (the dropped lo lo is below 2^-22 of the term).  Slots 10 and 11 (one
+// 32-bit word of the fragment) hold r0 = r0_hi + r0_lo against a constant 1
+// in A; slots 9 and 12..15 are zero.
+// A lane holds the eight slots 8 hl .. 8 hl + 7 (hl = lane / 32), so the
+// lower half-wave's B fragment (slots 0..7) depends on the mode only, and the
+// upper half's (slots 8..15) on the mode's r0 in the wave's cell as well.
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef float f16x __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ int half_cell_exp(double h) {
+  int e;
+  (void)frexp(0.5 * h, &e);
+  return e;
+}
+
+__device__ __forceinline__ void split2h(float x, uint16_t& hi, uint16_t& lo) {
+  const _Float16 h = static_cast<_Float16>(x);
+  hi = __builtin_bit_cast(uint16_t, h);
+  lo = __builtin_bit_cast(uint16_t, static_cast<_Float16>(x - static_cast<float>(h)));
+}
+
+// A fragment (32 points x 16 K slots, this lane's slots 8 hl .. 8 hl + 7) of
+// the point whose record starts at rec; usc[k] = 2^-e_k.
+template <int D>
+__device__ __forceinline__ h8 a_fragment32(const float* rec, const float (&usc)[3], int hl) {
+  uint16_t ph[3], pl[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) split2h(k < D ? rec[k] * usc[k] : 0.0f, ph[k], pl[k]);
+  constexpr uint16_t kOne = 0x3C00;  // fp16 1.0
+  uint16_t v[8];
+  if (hl == 0) {
+    v[0] = ph[0]; v[1] = ph[0]; v[2] = pl[0];
+    v[3] = ph[1]; v[4] = ph[1]; v[5] = pl[1];
+    v[6] = ph[2]; v[7] = ph[2];
+  } else {
+    v[0] = pl[2]; v[1] = 0; v[2] = kOne; v[3] = kOne;
+    v[4] = v[5] = v[6] = v[7] = 0;
+  }
+  return __builtin_bit_cast(h8, v);
+}
+
 constexpr int kCellPts = 2048;      // target points per cell
 constexpr int kMinCellPts = 64;     // the cell count never exceeds N / kMinCellPts
 constexpr double kCellRev = 0.75;   // cells keep sum_k |kap_k| h_k / 2 below this (revolutions)
@@ -96,12 +148,11 @@ constexpr int kSortThreads = 1024;  // threads of the histogram and scatter bloc
 constexpr int kSortUnroll = 4;      // points in flight per thread there
 constexpr int kColBlocks = 16;      // point blocks per group of the column scan
 constexpr int kColGroups = kMaxSortBlocks / kColBlocks;
-constexpr int kWavePts = 64;        // point slots per wave (4 MFMA groups of 16)
+constexpr int kWavePts = 64;        // point slots per record group (one wave)
 constexpr int kModeChunk = 512;     // modes per (r0, amplitude) table of a wave (32 tiles)
 constexpr double kRoughRev = 1.0;   // bound on |kap . xi| (revolutions) of the FP32 phase
 constexpr int kBoxBlocks = 1024;    // partial bounding boxes
 constexpr int kBoxUnroll = 4;       // points in flight per thread there
-constexpr int kRedLd = 65;          // LDS row of the final cross-lane sums (doubles)
 
 struct alignas(16) BinParams {
   double lo[3];
@@ -128,7 +179,8 @@ constexpr size_t al256(size_t b) { return (b + 255) & ~static_cast<size_t>(255);
 
 struct BinLayout {
   int64_t max_cells, max_waves, nblk, blk_pts;
-  size_t params, box, centre, bcount, gsum, counts, offsets, wave_cell, recs, btpl, amp, sm, rough,
+  size_t params, box, centre, bcount, gsum, counts, offsets, wave_cell, recs, btpl, btpl32, amp, sm,
+      rough,
       bytes;
 };
 
@@ -152,6 +204,7 @@ BinLayout bin_layout(int64_t M, int64_t N) {
   L.wave_cell = take(sizeof(int) * static_cast<size_t>(L.max_waves));
   L.recs = take(sizeof(float4) * static_cast<size_t>(L.max_waves) * kWavePts);
   L.btpl = take(sizeof(uint4) * 4 * (m + kModeChunk));
+  L.btpl32 = take(sizeof(uint4) * 2 * (m + kModeChunk));
   L.amp = take(sizeof(float4) * (m + kModeChunk));
   L.sm = take(sizeof(SmoothRec) * (m + kModeChunk));
   L.rough = take(sizeof(ModeRec) * m);
@@ -227,7 +280,8 @@ k_box_partial(const double* __restrict__ pos, int64_t ld, int64_t N, double* __r
 // lengths.
 template <int D>
 __device__ void split_modes(const ModeRec* __restrict__ modes, int M, const double* h,
-                            uint4* __restrict__ btpl, float4* __restrict__ amp,
+                            uint4* __restrict__ btpl, uint4* __restrict__ btpl32,
+                            float4* __restrict__ amp,
                             SmoothRec* __restrict__ sm, ModeRec* __restrict__ rough,
                             int* n_smooth, int* n_rough) {
   __shared__ int part[256];
@@ -277,6 +331,20 @@ __device__ void split_modes(const ModeRec* __restrict__ modes, int M, const doub
 #pragma unroll
         for (int q = 0; q < 4; ++q)
           btpl[4 * i + q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
+        // fp16 form: slots 0..7 (lower half-wave), then kap'_2 hi in slot 8
+        uint16_t kh[3], kl[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          float kf = d < D ? static_cast<float>(ldexp(r.kap[d], half_cell_exp(h[d]))) : 0.0f;
+          __asm__ volatile("" : "+v"(kf));  // FP32 rounding step kept (see k_direct_local32)
+          split2h(kf, kh[d], kl[d]);
+        }
+        auto pk = [](uint16_t a, uint16_t b) {
+          return static_cast<uint32_t>(a) | (static_cast<uint32_t>(b) << 16);
+        };
+        btpl32[2 * i] = make_uint4(pk(kh[0], kl[0]), pk(kh[0], kh[1]), pk(kl[1], kh[1]),
+                                   pk(kh[2], kl[2]));
+        btpl32[2 * i + 1] = make_uint4(pk(kh[2], 0), 0u, 0u, 0u);
         amp[i] = make_float4(r.amp[0], r.amp[1], D == 3 ? r.amp[2] : 0.0f, 0.0f);
         sm[i] = SmoothRec{{r.kap[0], r.kap[1], r.kap[2]}, r.th};
       } else {
@@ -299,6 +367,8 @@ __device__ void split_modes(const ModeRec* __restrict__ modes, int M, const doub
   // smooth one: zero phase data, zero amplitude and finite B fragments there
   for (int i = threadIdx.x; i < 4 * kModeChunk; i += 256)
     btpl[4 * ns_s + i] = make_uint4(0u, 0u, 0u, 0u);
+  for (int i = threadIdx.x; i < 2 * kModeChunk; i += 256)
+    btpl32[2 * ns_s + i] = make_uint4(0u, 0u, 0u, 0u);
   for (int i = threadIdx.x; i < kModeChunk; i += 256) {
     amp[ns_s + i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     sm[ns_s + i] = SmoothRec{{0.0, 0.0, 0.0}, 0.0};
@@ -315,8 +385,8 @@ template <int D>
 __global__ void __launch_bounds__(256)
 k_bin_params(const double* __restrict__ box, int64_t N, int64_t max_cells,
              const ModeRec* __restrict__ modes, int M, BinParams* __restrict__ prm,
-             uint4* __restrict__ btpl, float4* __restrict__ amp, SmoothRec* __restrict__ sm,
-             ModeRec* __restrict__ rough) {
+             uint4* __restrict__ btpl, uint4* __restrict__ btpl32, float4* __restrict__ amp,
+             SmoothRec* __restrict__ sm, ModeRec* __restrict__ rough) {
   __shared__ BinParams Ps;
   __shared__ double red[2 * D][256];
   __shared__ int qh[kQBins];
@@ -434,7 +504,7 @@ k_bin_params(const double* __restrict__ box, int64_t N, int64_t max_cells,
   }
   __syncthreads();
   // the modes' smooth / rough split depends on the cell width
-  split_modes<D>(modes, M, Ps.h, btpl, amp, sm, rough, &Ps.n_smooth, &Ps.n_rough);
+  split_modes<D>(modes, M, Ps.h, btpl, btpl32, amp, sm, rough, &Ps.n_smooth, &Ps.n_rough);
   __syncthreads();
   if (threadIdx.x == 0) *prm = Ps;
 }
@@ -729,19 +799,30 @@ k_bin_scatter(const double* __restrict__ pos, int64_t ld, int64_t N, int64_t blk
 }
 
 // ---- main kernel -------------------------------------------------------------
-// One wave per kWavePts slots of one cell; four independent waves per block,
-// each with its own LDS region (no block barriers).
+// 32 x 32 form of the local kernel: one v_mfma_f32_32x32x16_f16 gives the
+// phases of 32 points x 32 modes (the fp16 product above), about half the
+// matrix-core time per point-mode of four 16 x 16 x 32 bf16 tiles; the VALU
+// work (one sine and one FMA per point-mode) is the same.  One wave per
+// record group (kWavePts points, two row groups of 32), as k_direct_local,
+// with its own LDS table per chunk of kModeChunk modes: per mode the word
+// r0_hi | r0_lo << 16, which the upper half-wave ORs into the B fragment read
+// from the per-mode table (slots 10 and 11), and the amplitude, which every
+// lane takes for its mode column (8-byte records: 32 distinct ones per read,
+// no bank conflicts).  The two row groups' MFMAs of tile q + 1 are issued
+// right after the sines of their tile q, so each result set has a whole
+// group's VALU work to complete in.  Output element v of lane l in group g is
+// point 32 g + 8 (v / 4) + 4 (l / 32) + v % 4, mode column l % 32.
+constexpr int kRed32Ld = 33;  // LDS row of the cross-lane sums (floats)
 template <int D, typename OutT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4)))
-k_direct_local(const BinParams* __restrict__ prm, const double* __restrict__ centre,
-               const int* __restrict__ wave_cell,
-               const float4* __restrict__ recs, const uint4* __restrict__ btpl,
-               const float4* __restrict__ ampt, const SmoothRec* __restrict__ sm,
-               const ModeRec* __restrict__ rough, const double* __restrict__ pos, int64_t ld,
-               OutT* __restrict__ out, double scale) {
-  // per wave: (r0, amplitude) of one chunk of modes, and afterwards the
-  // 16 x 64 partial sums (row stride kRedLd doubles)
-  constexpr int kWaveLds = kModeChunk * 20 > 16 * kRedLd * 8 ? kModeChunk * 20 : 16 * kRedLd * 8;
+k_direct_local32(const BinParams* __restrict__ prm, const double* __restrict__ centre,
+                 const int* __restrict__ wave_cell, const float4* __restrict__ recs,
+                 const uint4* __restrict__ btpl32, const float4* __restrict__ ampt,
+                 const SmoothRec* __restrict__ sm, const ModeRec* __restrict__ rough,
+                 const double* __restrict__ pos, int64_t ld, OutT* __restrict__ out,
+                 double scale) {
+  constexpr int kWaveLds = kModeChunk * 8 > 64 * kRed32Ld * 4 ? kModeChunk * 8
+                                                              : 64 * kRed32Ld * 4;
   __shared__ __attribute__((aligned(16))) char lds[4][kWaveLds];
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -756,111 +837,117 @@ k_direct_local(const BinParams* __restrict__ prm, const double* __restrict__ cen
     X[1] = xc[1];
     X[2] = xc[2];
   }
-  // A operands: lane l holds slots 8 (l / 16) .. + 7 of point l % 16 of group g
-  const int kg = lane >> 4, j = lane & 15;
+  const int hl = lane >> 5, lr = lane & 31;
   const float* rf = reinterpret_cast<const float*>(recs + static_cast<int64_t>(w) * kWavePts);
-  bf8 a[4];
+  float usc[3];
 #pragma unroll
-  for (int g = 0; g < 4; ++g) a[g] = a_fragment<D>(rf + (16 * g + j) * 4, kg);
-  // per-chunk tables in the wave's LDS region: the MFMA C operand of every
-  // mode (r0 replicated over the four rows a lane holds) and its amplitude
-  f4* rq = reinterpret_cast<f4*>(lds[wv]);
-  float* ram = reinterpret_cast<float*>(rq + kModeChunk);
-  float acc[4][4], tot[4][4];
+  for (int k = 0; k < 3; ++k)
+    usc[k] = k < D ? static_cast<float>(ldexp(1.0, -half_cell_exp(P.h[k]))) : 0.0f;
+  h8 a[2];
 #pragma unroll
-  for (int g = 0; g < 4; ++g)
+  for (int g = 0; g < 2; ++g) a[g] = a_fragment32<D>(rf + 4 * (32 * g + lr), usc, hl);
+  uint2* wt = reinterpret_cast<uint2*>(lds[wv]);
+  const unsigned mu = hl ? ~0u : 0u;  // upper half-wave: r0 slots from the table
+  f16x zero;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
+  for (int v = 0; v < 16; ++v) zero[v] = 0.0f;
+  float acc[2][16], tot[2][16];
+#pragma unroll
+  for (int g = 0; g < 2; ++g)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
       acc[g][v] = 0.0f;
       tot[g][v] = 0.0f;
     }
   for (int c0 = 0; c0 < n_smooth; c0 += kModeChunk) {
-    // r0 of modes c0 .. c0 + kModeChunk - 1 for this cell: the FP64 phase at
-    // the cell centre, reduced exactly (zero phase and amplitude past n_smooth);
-    // four records in flight keep the register peak below the main loop's
-#pragma unroll 4
+    // the wave's table: r0 of its cell (FP64 phase at the centre, reduced
+    // exactly) split into two fp16 parts, and the amplitude; zero phase data
+    // and amplitude past n_smooth.  The wave reads only what it wrote, in
+    // order, so no barrier is needed.
+    __asm__ volatile("" ::: "memory");
+#pragma unroll 2
     for (int r = 0; r < kModeChunk / 64; ++r) {
-      const int i = c0 + 64 * r + lane;  // past n_smooth: zero padding
+      const int i = c0 + 64 * r + lane;
       const SmoothRec sr = sm[i];
       double t = __builtin_fma(sr.kap[0], X[0], sr.th);
       t = __builtin_fma(sr.kap[1], X[1], t);
       if constexpr (D == 3) t = __builtin_fma(sr.kap[2], X[2], t);
-      const float r0 = static_cast<float>(t - rint(t));
-      rq[64 * r + lane] = f4{r0, r0, r0, r0};
-      ram[64 * r + lane] = ampt[i].x;
+      // the empty asm keeps the FP32 rounding step: without it the compiler
+      // folds FP64 -> FP32 -> fp16 into a long software FP64 -> fp16 sequence
+      float r0 = static_cast<float>(t - rint(t));
+      __asm__ volatile("" : "+v"(r0));
+      uint16_t rh, rl;
+      split2h(r0, rh, rl);
+      wt[64 * r + lane] = make_uint2(static_cast<uint32_t>(rh) | (static_cast<uint32_t>(rl) << 16),
+                                     __builtin_bit_cast(uint32_t, ampt[i].x));
     }
-    // software pipeline: the phases of tile q + 1 are issued before the
-    // sines of tile q, so the MFMA results are ready when they are read; the
-    // loop is unrolled by two so the two result sets keep their registers.
-    // B fragments are requested two tiles ahead of their MFMAs; the reads
-    // run at most 47 modes past n_smooth, inside the zeroed padding.
-    const int ntile = min(kModeChunk / 16, (n_smooth - c0 + 15) / 16);
-    // lane's k-range of mode c0 + j: a uniform base that steps by whole
-    // tiles plus a fixed 32-bit lane offset
-    const uint4* bp = btpl + 4 * c0;
-    const unsigned bo = 4u * j + kg;
-    const f4* rp = rq + j;
-    const float* mp = ram + j;
-    uint4 ba = bp[bo], bb = bp[bo + 64];
+    __asm__ volatile("" ::: "memory");
+    // tiles of 32 modes; B fragments one tile ahead of their MFMAs (reads at
+    // most 63 modes past n_smooth, inside the zeroed padding); table records
+    // of the next tile clamped to the chunk
+    const int ntile = min(kModeChunk / 32, (n_smooth - c0 + 31) / 32);
+    const uint4* bp = btpl32 + 2 * c0;
+    const unsigned bo = 2u * lr + hl;
+    auto fragment = [&](const uint4& bq, const uint2& wq) {
+      uint4 b = bq;
+      b.y |= wq.x & mu;
+      return __builtin_bit_cast(h8, b);
+    };
+    uint4 bq = bp[bo];
+    uint2 wq = wt[lr];
+    float am = __builtin_bit_cast(float, wq.y);
+    f16x t0, t1;
+    {
+      const h8 b = fragment(bq, wq);
+      t0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[0], b, zero, 0, 0, 0);
+      t1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[1], b, zero, 0, 0, 0);
+    }
+    bq = bp[bo + 64];
+    wq = wt[min(32, kModeChunk - 32) + lr];
     bp += 128;
-    f4 ta[4], tb[4];
-    float ama, amb;
-    auto issue = [&](f4 (&t)[4], float& am, uint4& bq) {
-      const f4 c = *rp;
-      am = *mp;
-      rp += 16;
-      mp += 16;
-#pragma unroll
-      for (int g = 0; g < 4; ++g)
-        t[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[g], __builtin_bit_cast(bf8, bq), c, 0, 0,
-                                                       0);
+    for (int q = 0; q + 1 < ntile; ++q) {
+      const h8 b = fragment(bq, wq);
+      const float am_next = __builtin_bit_cast(float, wq.y);
       bq = bp[bo];
       bp += 64;
-    };
-    auto consume = [&](const f4 (&t)[4], float am) {
+      wq = wt[min(32 * (q + 2), kModeChunk - 32) + lr];
 #pragma unroll
-      for (int g = 0; g < 4; ++g)
+      for (int v = 0; v < 16; ++v)
+        acc[0][v] = __builtin_fmaf(am, __builtin_amdgcn_sinf(t0[v]), acc[0][v]);
+      t0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[0], b, zero, 0, 0, 0);
 #pragma unroll
-        for (int v = 0; v < 4; ++v)
-          acc[g][v] = __builtin_fmaf(am, __builtin_amdgcn_sinf(t[g][v]), acc[g][v]);
-    };
-    // (the loop body issues unconditionally, so every wait for a B fragment
-    // leaves the younger prefetch in flight)
-    issue(ta, ama, ba);
-    int q = 0;
-    for (; q + 2 < ntile; q += 2) {
-      issue(tb, amb, bb);
-      consume(ta, ama);
-      issue(ta, ama, ba);
-      consume(tb, amb);
-    }
-    if (q + 1 < ntile) {
-      issue(tb, amb, bb);
-      consume(ta, ama);
-      consume(tb, amb);
-    } else {
-      consume(ta, ama);
+      for (int v = 0; v < 16; ++v)
+        acc[1][v] = __builtin_fmaf(am, __builtin_amdgcn_sinf(t1[v]), acc[1][v]);
+      t1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[1], b, zero, 0, 0, 0);
+      am = am_next;
     }
 #pragma unroll
-    for (int g = 0; g < 4; ++g)
+    for (int v = 0; v < 16; ++v) {
+      acc[0][v] = __builtin_fmaf(am, __builtin_amdgcn_sinf(t0[v]), acc[0][v]);
+      acc[1][v] = __builtin_fmaf(am, __builtin_amdgcn_sinf(t1[v]), acc[1][v]);
+    }
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
+    for (int g = 0; g < 2; ++g)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
         tot[g][v] += acc[g][v];
         acc[g][v] = 0.0f;
       }
   }
-  // cross-lane sum: tot[g][v] is slot 16 g + 4 (lane / 16) + v, mode column j.
-  // The table and the sums share the wave's LDS region under different
-  // types; the compiler barrier keeps the last table reads before these writes.
+  // cross-lane sums over the 32 mode columns, in FP64; the table and the
+  // sums share the wave's LDS region, the compiler barrier keeps the last
+  // table reads before these writes
   __asm__ volatile("" ::: "memory");
-  double* red = reinterpret_cast<double*>(lds[wv]);
+  float* red = reinterpret_cast<float*>(lds[wv]);
 #pragma unroll
-  for (int g = 0; g < 4; ++g)
+  for (int g = 0; g < 2; ++g)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) red[j * kRedLd + 16 * g + 4 * kg + v] = static_cast<double>(tot[g][v]);
+    for (int v = 0; v < 16; ++v)
+      red[(32 * g + 8 * (v >> 2) + 4 * hl + (v & 3)) * kRed32Ld + lr] = tot[g][v];
+  __asm__ volatile("" ::: "memory");
   double sum = 0.0;
-#pragma unroll
-  for (int jj = 0; jj < 16; ++jj) sum += red[jj * kRedLd + lane];
+#pragma unroll 8
+  for (int c = 0; c < 32; ++c) sum += static_cast<double>(red[lane * kRed32Ld + c]);
   const int p = __builtin_bit_cast(int, rf[lane * 4 + 3]);
   if (p < 0) return;
   if (n_rough > 0) {
@@ -1092,6 +1179,7 @@ hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table
   float4* recs = at<float4>(bin_work, L.recs);
 
   uint4* btpl = at<uint4>(bin_work, L.btpl);
+  uint4* btpl32 = at<uint4>(bin_work, L.btpl32);
   float4* ampt = at<float4>(bin_work, L.amp);
   SmoothRec* sm = at<SmoothRec>(bin_work, L.sm);
   ModeRec* rough = at<ModeRec>(bin_work, L.rough);
@@ -1107,8 +1195,8 @@ hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table
 #define GSC_BIN(D)                                                                               \
   do {                                                                                           \
     k_box_partial<D><<<kBoxBlocks, 256, 0, s>>>(pos, pos_ld, N, box);                            \
-    k_bin_params<D><<<1, 256, 0, s>>>(box, N, L.max_cells, modes, Mi, prm, btpl, ampt, sm,       \
-                                      rough);                                                    \
+    k_bin_params<D><<<1, 256, 0, s>>>(box, N, L.max_cells, modes, Mi, prm, btpl, btpl32, ampt,   \
+                                      sm, rough);                                                \
     k_bin_hist<D><<<nblk, kSortThreads, 0, s>>>(pos, pos_ld, N, L.blk_pts, prm, bcount);         \
     k_col_sum<<<dim3(cb, ngroups), 256, 0, s>>>(prm, nblk, bcount, gsum);                        \
     k_col_groups<<<cb, 256, 0, s>>>(prm, ngroups, gsum, counts);                                 \
@@ -1127,13 +1215,13 @@ hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table
                                                        static_cast<double*>(out), out_ld,        \
                                                        scale);                                   \
     else if (f32)                                                                                \
-      k_direct_local<D, float><<<mb, 256, 0, s>>>(prm, centre, wave_cell, recs, btpl, ampt, sm,  \
-                                                  rough, pos, pos_ld, static_cast<float*>(out),  \
-                                                  scale);                                        \
+      k_direct_local32<D, float><<<mb, 256, 0, s>>>(prm, centre, wave_cell, recs, btpl32, ampt,  \
+                                                    sm, rough, pos, pos_ld,                      \
+                                                    static_cast<float*>(out), scale);            \
     else                                                                                         \
-      k_direct_local<D, double><<<mb, 256, 0, s>>>(prm, centre, wave_cell, recs, btpl, ampt, sm, \
-                                                   rough, pos, pos_ld, static_cast<double*>(out),\
-                                                   scale);                                       \
+      k_direct_local32<D, double><<<mb, 256, 0, s>>>(prm, centre, wave_cell, recs, btpl32, ampt, \
+                                                     sm, rough, pos, pos_ld,                     \
+                                                     static_cast<double*>(out), scale);          \
   } while (0)
   if (dim == 2) GSC_BIN(2);
   else GSC_BIN(3);

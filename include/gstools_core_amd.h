@@ -79,8 +79,10 @@ typedef enum gsc_accum {
 /* How the direct (per-point) kernels evaluate the phases k . x.
  * LOCAL sorts the points into spatial cells and splits every phase at the
  * cell centre X: (k . X + theta) in FP64 once per cell and mode, k . (x - X)
- * from three-part bf16 splits on the matrix cores (v_mfma_f32_16x16x32_bf16,
- * 256 point-modes per instruction); modes whose local phase could exceed one
+ * on the matrix cores, from two-part fp16 splits for scalar fields
+ * (v_mfma_f32_32x32x16_f16, 1024 point-modes per instruction) and three-part
+ * bf16 splits for incompressible fields (v_mfma_f32_16x16x32_bf16, 256
+ * point-modes per instruction); modes whose local phase could exceed one
  * revolution keep the FP64 phase.  It serves all three summators in 2-D and
  * 3-D (N < 2^30); in other dimensions LOCAL runs the VALU kernel.
  * On gfx950 matrix instructions do not overlap with the VALU sine work (the
