@@ -153,6 +153,7 @@ constexpr int kModeChunk = 512;     // modes per (r0, amplitude) table of a wave
 constexpr double kRoughRev = 1.0;   // bound on |kap . xi| (revolutions) of the FP32 phase
 constexpr int kBoxBlocks = 1024;    // partial bounding boxes
 constexpr int kBoxUnroll = 4;       // points in flight per thread there
+constexpr size_t kR0TableCap = size_t(256) << 20;  // largest (cell, mode) r0 table
 
 struct alignas(16) BinParams {
   double lo[3];
@@ -180,8 +181,8 @@ constexpr size_t al256(size_t b) { return (b + 255) & ~static_cast<size_t>(255);
 struct BinLayout {
   int64_t max_cells, max_waves, nblk, blk_pts;
   size_t params, box, centre, bcount, gsum, counts, offsets, wave_cell, recs, btpl, btpl32, amp, sm,
-      rough,
-      bytes;
+      rough, r0w, bytes;
+  int64_t r0ld;  // row length of the (cell, mode) r0 table; 0: no table
 };
 
 BinLayout bin_layout(int64_t M, int64_t N) {
@@ -208,6 +209,12 @@ BinLayout bin_layout(int64_t M, int64_t N) {
   L.amp = take(sizeof(float4) * (m + kModeChunk));
   L.sm = take(sizeof(SmoothRec) * (m + kModeChunk));
   L.rough = take(sizeof(ModeRec) * m);
+  // r0 words of every (cell, smooth mode), when the table stays below
+  // kR0TableCap bytes; otherwise every wave evaluates its cell's r0 itself
+  const int64_t r0ld = static_cast<int64_t>(m) + kModeChunk;
+  const size_t r0bytes = sizeof(uint32_t) * static_cast<size_t>(L.max_cells * r0ld);
+  L.r0ld = r0bytes <= kR0TableCap ? r0ld : 0;
+  L.r0w = L.r0ld > 0 ? take(r0bytes) : 0;
   L.bytes = off;
   return L;
 }
@@ -798,6 +805,35 @@ k_bin_scatter(const double* __restrict__ pos, int64_t ld, int64_t N, int64_t blk
   }
 }
 
+// r0 word (r0_hi | r0_lo << 16, the fp16 parts of the reduced FP64 phase at
+// the cell centre) of every (cell, smooth mode) pair, modes up to kModeChunk
+// past n_smooth (zero phase data there): computed once per cell here instead
+// of once per wave of the cell in k_direct_local32.  One block per cell.
+template <int D>
+__global__ void __launch_bounds__(256)
+k_cell_r0(const BinParams* __restrict__ prm, const double* __restrict__ centre,
+          const SmoothRec* __restrict__ sm, uint32_t* __restrict__ r0w, int64_t r0ld) {
+  const BinParams P = *prm;
+  const int cell = blockIdx.x;
+  if (cell >= P.ncells) return;
+  const double* xc = centre + 4 * static_cast<int64_t>(cell);
+  const double X0 = xc[0], X1 = xc[1], X2 = xc[2];
+  uint32_t* row = r0w + cell * r0ld;
+  const int end = P.n_smooth + kModeChunk;
+#pragma unroll 2
+  for (int i = threadIdx.x; i < end; i += 256) {
+    const SmoothRec sr = sm[i];
+    double t = __builtin_fma(sr.kap[0], X0, sr.th);
+    t = __builtin_fma(sr.kap[1], X1, t);
+    if constexpr (D == 3) t = __builtin_fma(sr.kap[2], X2, t);
+    float r0 = static_cast<float>(t - rint(t));
+    __asm__ volatile("" : "+v"(r0));  // FP32 rounding step kept (see k_direct_local32)
+    uint16_t rh, rl;
+    split2h(r0, rh, rl);
+    row[i] = static_cast<uint32_t>(rh) | (static_cast<uint32_t>(rl) << 16);
+  }
+}
+
 // ---- main kernel -------------------------------------------------------------
 // 32 x 32 form of the local kernel: one v_mfma_f32_32x32x16_f16 gives the
 // phases of 32 points x 32 modes (the fp16 product above), about half the
@@ -818,7 +854,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))
 k_direct_local32(const BinParams* __restrict__ prm, const double* __restrict__ centre,
                  const int* __restrict__ wave_cell, const float4* __restrict__ recs,
                  const uint4* __restrict__ btpl32, const float4* __restrict__ ampt,
-                 const SmoothRec* __restrict__ sm, const ModeRec* __restrict__ rough,
+                 const SmoothRec* __restrict__ sm, const uint32_t* __restrict__ r0w,
+                 int64_t r0ld, const ModeRec* __restrict__ rough,
                  const double* __restrict__ pos, int64_t ld, OutT* __restrict__ out,
                  double scale) {
   constexpr int kWaveLds = kModeChunk * 8 > 64 * kRed32Ld * 4 ? kModeChunk * 8
@@ -830,13 +867,15 @@ k_direct_local32(const BinParams* __restrict__ prm, const double* __restrict__ c
   const BinParams P = *prm;
   if (w >= P.nwaves) return;
   const int n_smooth = P.n_smooth, n_rough = P.n_rough;
+  const int cell = wave_cell[w];
   double X[3];
   {
-    const double* xc = centre + 4 * static_cast<int64_t>(wave_cell[w]);
+    const double* xc = centre + 4 * static_cast<int64_t>(cell);
     X[0] = xc[0];
     X[1] = xc[1];
     X[2] = xc[2];
   }
+  const uint32_t* r0row = r0w != nullptr ? r0w + cell * r0ld : nullptr;
   const int hl = lane >> 5, lr = lane & 31;
   const float* rf = reinterpret_cast<const float*>(recs + static_cast<int64_t>(w) * kWavePts);
   float usc[3];
@@ -865,6 +904,13 @@ k_direct_local32(const BinParams* __restrict__ prm, const double* __restrict__ c
     // and amplitude past n_smooth.  The wave reads only what it wrote, in
     // order, so no barrier is needed.
     __asm__ volatile("" ::: "memory");
+    if (r0row != nullptr) {  // precomputed per cell: a copy
+#pragma unroll 4
+      for (int r = 0; r < kModeChunk / 64; ++r) {
+        const int i = c0 + 64 * r + lane;
+        wt[64 * r + lane] = make_uint2(r0row[i], __builtin_bit_cast(uint32_t, ampt[i].x));
+      }
+    } else {
 #pragma unroll 2
     for (int r = 0; r < kModeChunk / 64; ++r) {
       const int i = c0 + 64 * r + lane;
@@ -880,6 +926,7 @@ k_direct_local32(const BinParams* __restrict__ prm, const double* __restrict__ c
       split2h(r0, rh, rl);
       wt[64 * r + lane] = make_uint2(static_cast<uint32_t>(rh) | (static_cast<uint32_t>(rl) << 16),
                                      __builtin_bit_cast(uint32_t, ampt[i].x));
+    }
     }
     __asm__ volatile("" ::: "memory");
     // tiles of 32 modes; B fragments one tile ahead of their MFMAs (reads at
@@ -1183,6 +1230,7 @@ hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table
   float4* ampt = at<float4>(bin_work, L.amp);
   SmoothRec* sm = at<SmoothRec>(bin_work, L.sm);
   ModeRec* rough = at<ModeRec>(bin_work, L.rough);
+  uint32_t* r0w = L.r0ld > 0 ? at<uint32_t>(bin_work, L.r0w) : nullptr;
   const ModeRec* modes = static_cast<const ModeRec*>(table);
   const int nblk = static_cast<int>(L.nblk);
   const int ngroups = (nblk + kColBlocks - 1) / kColBlocks;
@@ -1205,6 +1253,9 @@ hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table
     k_bin_waves<D><<<wb, 256, 0, s>>>(prm, counts, offsets, centre, wave_cell, recs);            \
     k_bin_scatter<D><<<nblk, kSortThreads, 0, s>>>(pos, pos_ld, N, L.blk_pts, prm, bcount,       \
                                                    offsets, recs);                               \
+    if (!vec && r0w != nullptr)                                                                  \
+      k_cell_r0<D><<<static_cast<unsigned>(L.max_cells), 256, 0, s>>>(prm, centre, sm, r0w,      \
+                                                                      L.r0ld);                   \
     if (vec && f32)                                                                              \
       k_direct_local_vec<D, float><<<vb, 256, 0, s>>>(prm, centre, wave_cell, recs, btpl, ampt,  \
                                                       sm, rough, pos, pos_ld,                    \
@@ -1216,11 +1267,11 @@ hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table
                                                        scale);                                   \
     else if (f32)                                                                                \
       k_direct_local32<D, float><<<mb, 256, 0, s>>>(prm, centre, wave_cell, recs, btpl32, ampt,  \
-                                                    sm, rough, pos, pos_ld,                      \
+                                                    sm, r0w, L.r0ld, rough, pos, pos_ld,         \
                                                     static_cast<float*>(out), scale);            \
     else                                                                                         \
       k_direct_local32<D, double><<<mb, 256, 0, s>>>(prm, centre, wave_cell, recs, btpl32, ampt, \
-                                                     sm, rough, pos, pos_ld,                     \
+                                                     sm, r0w, L.r0ld, rough, pos, pos_ld,        \
                                                      static_cast<double*>(out), scale);          \
   } while (0)
   if (dim == 2) GSC_BIN(2);

@@ -275,6 +275,22 @@ def test_direct_variants_agree_at_scale():
     assert err <= 5e-6
 
 
+def test_local_without_cell_r0_table():
+    # 16384 cells x (4000 + 512) modes of r0 words exceed the 256 MB table
+    # cap, so every wave evaluates its cell's r0 itself; 1000 modes keep the
+    # precomputed table.  Both against the FP64 VALU kernel.
+    rng = np.random.default_rng(77)
+    pos = rng.uniform(0, 500, size=(3, 1_100_000))
+    for m in (4000, 1000):
+        cov = rng.normal(0, 0.03, size=(3, m))
+        z1, z2 = rng.normal(size=m), rng.normal(size=m)
+        a = gstools_core.summate(cov, z1, z2, pos, variant="local")
+        b = gstools_core.summate(cov, z1, z2, pos, variant="valu")
+        err = _err(a, b) / np.sqrt(m)
+        print(f"local vs valu, m={m}: {err:.3e} sigma")
+        assert err <= 1e-5
+
+
 # -- the cell-local (binned) kernel ---------------------------------------------
 
 def _rand_modes(rng, d, m, s=0.05):
