@@ -149,7 +149,7 @@ constexpr int kSortUnroll = 4;      // points in flight per thread there
 constexpr int kColBlocks = 16;      // point blocks per group of the column scan
 constexpr int kColGroups = kMaxSortBlocks / kColBlocks;
 constexpr int kWavePts = 64;        // point slots per record group (one wave)
-constexpr int kModeChunk = 512;     // modes per (r0, amplitude) table of a wave (32 tiles)
+constexpr int kModeChunk = 1024;    // modes per (r0, amplitude) table of a wave (32 tiles)
 constexpr double kRoughRev = 1.0;   // bound on |kap . xi| (revolutions) of the FP32 phase
 constexpr int kBoxBlocks = 1024;    // partial bounding boxes
 constexpr int kBoxUnroll = 4;       // points in flight per thread there

@@ -276,7 +276,7 @@ def test_direct_variants_agree_at_scale():
 
 
 def test_local_without_cell_r0_table():
-    # 16384 cells x (4000 + 512) modes of r0 words exceed the 256 MB table
+    # 16384 cells x (4000 + 1024) modes of r0 words exceed the 256 MB table
     # cap, so every wave evaluates its cell's r0 itself; 1000 modes keep the
     # precomputed table.  Both against the FP64 VALU kernel.
     rng = np.random.default_rng(77)
