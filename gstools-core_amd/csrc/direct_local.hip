@@ -104,10 +104,14 @@ __device__ __forceinline__ bf8 a_fragment(const float* rec, int kg) {
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef float f16x __attribute__((ext_vector_type(16)));
 
+// e with 2^e > h / 2 >= 2^(e-1), clamped to [-126, 126] so that 2^-e stays
+// a finite normal FP32 number (kap' = kap 2^e <= kap h <= 2 for smooth modes
+// needs no bound).  The clamp is only reached by cells narrower than 2^-126,
+// whose offsets xi the FP32 point records cannot hold anyway.
 __device__ __forceinline__ int half_cell_exp(double h) {
   int e;
   (void)frexp(0.5 * h, &e);
-  return e;
+  return min(max(e, -126), 126);
 }
 
 __device__ __forceinline__ void split2h(float x, uint16_t& hi, uint16_t& lo) {

@@ -275,6 +275,23 @@ def test_direct_variants_agree_at_scale():
     assert err <= 5e-6
 
 
+@pytest.mark.parametrize("scale", [1e-35, 1e30])
+def test_local_extreme_coordinate_scales(scale):
+    # the same phases with positions scaled by `scale` and wavevectors by
+    # 1 / scale: the power-of-two scaling of the fp16 phase product takes
+    # exponents near -110 and +100 (the FP32 point records hold offsets
+    # down to about 1e-37); both against the FP64 VALU kernel
+    rng = np.random.default_rng(91)
+    m, n = 500, 50_000
+    cov = rng.normal(0, 0.03, size=(3, m)) / scale
+    z1, z2 = rng.normal(size=m), rng.normal(size=m)
+    pos = rng.uniform(0, 400, size=(3, n)) * scale
+    a = gstools_core.summate(cov, z1, z2, pos, variant="local")
+    b = gstools_core.summate(cov, z1, z2, pos, variant="valu")
+    assert np.all(np.isfinite(a))
+    assert _err(a, b) / np.sqrt(m) <= 1e-5
+
+
 def test_local_without_cell_r0_table():
     # 16384 cells x (4000 + 1024) modes of r0 words exceed the 256 MB table
     # cap, so every wave evaluates its cell's r0 itself; 1000 modes keep the
