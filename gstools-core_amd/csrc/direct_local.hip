@@ -76,12 +76,13 @@ __device__ __forceinline__ void split3(float x, uint16_t (&p)[3]) {
 
 // A fragment (16 points x 32 K slots, this lane's 8 slots) of the point
 // whose record starts at rec: lane group kg = lane / 16 holds slots
-// 8 kg .. 8 kg + 7, the split parts of coordinate kg (zero for kg >= D).
+// 8 kg .. 8 kg + 7, the split parts of coordinate kg scaled by sc = 2^-e_kg
+// (the power-of-two cell scaling of the fp16 form below; zero for kg >= D).
 template <int D>
-__device__ __forceinline__ bf8 a_fragment(const float* rec, int kg) {
+__device__ __forceinline__ bf8 a_fragment(const float* rec, int kg, float sc) {
   const float x = rec[kg];  // kg = 3: the index field, not used
   uint16_t p[3];
-  split3(kg < D ? x : 0.0f, p);
+  split3(kg < D ? x * sc : 0.0f, p);
   uint16_t v[8];
 #pragma unroll
   for (int t = 0; t < 6; ++t) v[t] = p[kXPart[t]];
@@ -332,7 +333,9 @@ __device__ void split_modes(const ModeRec* __restrict__ modes, int M, const doub
 #pragma unroll
         for (int d = 0; d < D; ++d) {
           uint16_t kp[3];
-          split3(static_cast<float>(r.kap[d]), kp);
+          float kf = static_cast<float>(ldexp(r.kap[d], half_cell_exp(h[d])));
+          __asm__ volatile("" : "+v"(kf));  // FP32 rounding step kept
+          split3(kf, kp);
 #pragma unroll
           for (int t = 0; t < 6; ++t) {
             const int k = 8 * d + t;
@@ -1082,8 +1085,9 @@ k_direct_local_vec(const BinParams* __restrict__ prm, const double* __restrict__
   const float* rf =
       reinterpret_cast<const float*>(recs + static_cast<int64_t>(rg) * kWavePts + 32 * (w & 1));
   bf8 a[2];
+  const float sck = kg < D ? static_cast<float>(ldexp(1.0, -half_cell_exp(P.h[kg]))) : 0.0f;
 #pragma unroll
-  for (int g = 0; g < 2; ++g) a[g] = a_fragment<D>(rf + (16 * g + j) * 4, kg);
+  for (int g = 0; g < 2; ++g) a[g] = a_fragment<D>(rf + (16 * g + j) * 4, kg, sck);
   f4* rq = reinterpret_cast<f4*>(lds[wv >> 1]);
   f4* ram = rq + kVecChunk;
   float acc[D][2][4], tot[D][2][4];

@@ -290,6 +290,10 @@ def test_local_extreme_coordinate_scales(scale):
     b = gstools_core.summate(cov, z1, z2, pos, variant="valu")
     assert np.all(np.isfinite(a))
     assert _err(a, b) / np.sqrt(m) <= 1e-5
+    ai = gstools_core.summate_incompr(cov, z1, z2, pos, variant="local")
+    bi = gstools_core.summate_incompr(cov, z1, z2, pos, variant="valu")
+    assert np.all(np.isfinite(ai))
+    assert _err(ai, bi) / np.sqrt(m) <= 1e-5
 
 
 def test_local_without_cell_r0_table():
