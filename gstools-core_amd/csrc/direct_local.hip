@@ -1085,7 +1085,9 @@ k_direct_local_vec(const BinParams* __restrict__ prm, const double* __restrict__
   const float* rf =
       reinterpret_cast<const float*>(recs + static_cast<int64_t>(rg) * kWavePts + 32 * (w & 1));
   bf8 a[2];
-  const float sck = kg < D ? static_cast<float>(ldexp(1.0, -half_cell_exp(P.h[kg]))) : 0.0f;
+  // (a select chain, not P.h[kg]: a lane-dependent index would put P in scratch)
+  const double hk = kg == 0 ? P.h[0] : (kg == 1 ? P.h[1] : P.h[2]);
+  const float sck = kg < D ? static_cast<float>(ldexp(1.0, -half_cell_exp(hk))) : 0.0f;
 #pragma unroll
   for (int g = 0; g < 2; ++g) a[g] = a_fragment<D>(rf + (16 * g + j) * 4, kg, sck);
   f4* rq = reinterpret_cast<f4*>(lds[wv >> 1]);
