@@ -16,8 +16,9 @@ no CPU fallback.  Deliberate differences from the reference:
   * keyword-only extensions: out_dtype (np.float64 / np.float32), scale
     (multiplies the result, e.g. sqrt(var / M)), device (HIP ordinal),
     variant ("local": points sorted into spatial cells, FP64 phase at each
-    cell centre and split-bf16 matrix-core phases inside the cell, the
-    default in 2-D and 3-D; "valu": FP64 phases on the vector ALUs;
+    cell centre and split-fp16 (scalar fields) or split-bf16 (incompressible
+    fields) matrix-core phases inside the cell, the default in 2-D and 3-D;
+    "valu": FP64 phases on the vector ALUs;
     "mfma": FP64 phases in matrix-core tiles, d <= 3).
 num_threads is accepted with the reference's meaning (worker threads); here
 it bounds the host threads that stage strided inputs.
