@@ -298,6 +298,51 @@ def incompr_leg(steps: int, warmup: int, n: int = 4_000_000, m: int = 1000) -> d
     return res
 
 
+def fourier_leg(steps: int, warmup: int, n: int = 10_000_000, side: int = 10) -> dict:
+    """summate_fourier (field.rs:219-249) on the 3-D unstructured positions of
+    the direct workload (N = 1e7 uniform in [0,500]^3) with M = side^3 modes
+    on a regular k-grid (spacing 2 pi / 200, centred on 0) weighted by a
+    Gaussian spectrum (l = 20, sum of sf^2 = 1): the cell-local kernel and the
+    FP64-phase VALU kernel, device-resident, FP64 out."""
+    import numpy as np
+    import torch
+
+    from gstools_core import device, workloads
+
+    w = workloads.c2(n=n)
+    d, m = 3, side ** 3
+    dk = 2.0 * np.pi / 200.0
+    ax = (np.arange(side) - (side - 1) / 2.0) * dk
+    kg = np.stack(np.meshgrid(ax, ax, ax, indexing="ij")).reshape(3, m)
+    sf = np.exp(-((kg ** 2).sum(axis=0)) * 20.0 ** 2 / 8.0)
+    sf = sf / np.sqrt((sf ** 2).sum())
+    rng = np.random.default_rng(20240607)
+    z1, z2 = rng.normal(size=m), rng.normal(size=m)
+    dev = torch.device("cuda:0")
+    t_sf = torch.from_numpy(sf).to(dev)
+    t_k = torch.from_numpy(np.ascontiguousarray(kg)).to(dev)
+    t_z1, t_z2 = torch.from_numpy(z1).to(dev), torch.from_numpy(z2).to(dev)
+    pos = torch.from_numpy(np.ascontiguousarray(w.pos)).to(dev)
+    out = torch.empty(n, dtype=torch.float64, device=dev)
+    nb = max(device.workspace_bytes(d, m, n, v) for v in ("local", "valu"))
+    work = torch.empty(nb, dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream()
+    res = {"workload": f"3-D unstructured summate_fourier, N={n} uniform in [0,500]^3, "
+                       f"M={m} modes on a {side}^3 k-grid (spacing 2 pi/200), Gaussian spectrum "
+                       "(l=20)"}
+    for vname in ("local", "valu"):
+        def step(vname=vname):
+            device.summate_fourier_dev(d, m, n, t_sf.data_ptr(), t_k.data_ptr(), t_z1.data_ptr(),
+                                       t_z2.data_ptr(), pos.data_ptr(), n, out.data_ptr(),
+                                       work.data_ptr(), nb, stream=stream.cuda_stream,
+                                       variant=vname)
+
+        _, tot = time_steps(step, steps, warmup, stream)
+        ms = tot / steps
+        res[f"{vname}_phase"] = {"ms_per_step": ms, "point_modes_per_s": n * m / (ms * 1e-3)}
+    return res
+
+
 def krige_leg(steps: int, warmup: int, n: int = 1000, N: int = 1_000_000) -> dict:
     """Kriging calculators (krige.rs:24-118) on device-resident inputs: n
     conditions, N targets, FP64.  field+error: 2 n^2 N flops on the FP64
@@ -354,8 +399,9 @@ def main() -> int:
     ap.add_argument("--m", type=int, default=1000)
     ap.add_argument("--no-grid", action="store_true", help="skip the structured-grid leg")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble leg")
-    ap.add_argument("--legs", default="direct,incompr,grid,c5,ensemble,krige",
-                    help="comma list of legs: direct, incompr, grid, c5, ensemble, krige, copy "
+    ap.add_argument("--legs", default="direct,incompr,fourier,grid,c5,ensemble,krige",
+                    help="comma list of legs: direct, incompr, fourier, grid, c5, ensemble, "
+                         "krige, copy "
                          "(a 1 GiB device copy, the reference for HBM byte counters)")
     args = ap.parse_args()
     legs = set(args.legs.split(","))
@@ -388,6 +434,8 @@ def main() -> int:
         res = {}
         if "incompr" in legs:
             res["incompr"] = incompr_leg(max(3, args.steps // 2), 1)
+        if "fourier" in legs:
+            res["fourier"] = fourier_leg(max(3, args.steps // 2), 1)
         if "grid" in legs:
             res["structured_grid"] = grid_leg(args.steps, args.warmup)
         if "c5" in legs:
@@ -470,6 +518,11 @@ def main() -> int:
         log("incompressible leg (N = 4e6)")
         inc_res = incompr_leg(max(3, args.steps // 2), 1)
 
+    fourier_res = None
+    if "fourier" in legs:
+        log("fourier leg (N = 1e7, 10^3 k-grid)")
+        fourier_res = fourier_leg(max(3, args.steps // 2), 1)
+
     grid_res = None
     if "grid" in legs:
         log("grid leg (256^3, factored vs direct)")
@@ -542,6 +595,7 @@ def main() -> int:
         "direct_variants": direct_variants,
         "host_arrays": host_arrays,
         "incompr": inc_res,
+        "fourier": fourier_res,
         "structured_grid": grid_res,
         "heavy_tail_grid": c5_res,
         "krige": krige_res,
