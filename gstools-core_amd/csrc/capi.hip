@@ -773,6 +773,7 @@ int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes
     if (e.npts == 0) continue;
     if (e.path == GSC_PATH_FACTORED) {
       const gsc::FactoredPlan fp = gsc::plan_factored(dim, M, sh, o->precision);
+      e.trans = fp.trans;
       plans[(size_t)r] = fp;
       e.mp = fp.mp;
       e.na = fp.na;

@@ -91,10 +91,17 @@ FactoredPlan plan_factored(int dim, int64_t M, const int64_t* shape, int precisi
   p.nch = p.mp / 16;
   p.kp = 2 * static_cast<int64_t>(p.mp);
   if (dim == 2) {
+    // rows are padded to kBlockRows and columns to up to kMaxTileCols: take
+    // axis 1 as the row (A) axis when that pads less; the GEMM then stores
+    // the C-order output transposed (row stride 1, column stride nrow)
+    auto work = [](int64_t r, int64_t c) {
+      return (r + kBlockRows - 1) / kBlockRows * ((c + kMaxTileCols - 1) / kMaxTileCols);
+    };
+    p.trans = work(shape[1], shape[0]) < work(shape[0], shape[1]) ? 1 : 0;
     p.na = 1;
-    p.nrow = shape[0];
+    p.nrow = shape[p.trans ? 1 : 0];
     p.nb_out = 0;
-    p.ncols = shape[1];
+    p.ncols = shape[p.trans ? 0 : 1];
   } else {
     p.na = shape[0];
     p.nrow = shape[1];
@@ -156,13 +163,15 @@ hipError_t launch_factored(const FactoredPlan& p, int64_t M, const double* const
   // tables (the mode table was written by launch_prep): one launch
   const int64_t nmax = std::max(std::max(kr ? p.na : 0, p.arows), p.ncp);
   const dim3 tg(static_cast<unsigned>((nmax + 63) / 64), static_cast<unsigned>(p.nch), 3);
-  TableArgs ta{modes,     Mi,     p.dim,   p.nch,   axes[0], axes[kr ? 1 : 0], axes[p.dim - 1],
-               p.na,      p.nrow, p.arows, p.ncols, p.ncp,   S,               A,
-               C};
+  const double* xa = axes[kr ? 1 : (p.trans ? 1 : 0)];
+  const double* xc = axes[kr ? 2 : (p.trans ? 0 : 1)];
+  TableArgs ta{modes, Mi,      p.dim,   p.nch, axes[0], xa, xc, p.na, p.nrow,
+               p.arows, p.ncols, p.ncp, S,     A,       C,  p.trans};
   hipError_t e = by_format<TablesF>(p.bf16, p.ns, ta, tg, s);
   if (e != hipSuccess) return e;
   GemmArgs ga{A,       p.arows, p.nch, S,     p.na,  C,  p.ncp, p.nrow, p.nb_out, out,
-              out_dtype != 1, p.ncols, p.ncols, scale, kr, accum};
+              out_dtype != 1, p.trans ? 1 : p.ncols, p.ncols, p.trans ? p.nrow : 1, scale, kr,
+              accum};
   const int64_t tile_cols = gemm_tile_cols(p.ns, accum);
   const dim3 grid(static_cast<unsigned>(p.row_tiles),
                   static_cast<unsigned>((p.ncols + tile_cols - 1) / tile_cols));

@@ -51,6 +51,7 @@ struct FactoredPlan {
   int64_t ncols;     // last-axis length
   int64_t ncp;       // ncols padded to kMaxTileCols
   int64_t row_tiles; // 3-D: na * arows / kBlockRows; 2-D: arows / kBlockRows
+  int trans;         // 2-D with axis 1 as the A axis (it pads less); the output is stored transposed
   size_t off_s, off_a, off_c, bytes;
 };
 
@@ -85,6 +86,7 @@ struct EnsReal {
   int dim;
   int path;           // GSC_PATH_DIRECT or GSC_PATH_FACTORED
   int mp;             // factored: modes padded to 32
+  int trans;          // factored 2-D: rows are y and columns x (the padding is smaller)
   int64_t na, nrow, arows, nb_out, ncols, ncp, row_tiles, col_tiles;  // factored geometry
   int64_t s0, s1, s2, npts;                     // grid shape
   double scale;
@@ -131,6 +133,7 @@ struct TableArgs {
   float2* S;
   uint32_t* A;
   float2* C;
+  int trans;  // 2-D: A from axis 1, C from axis 0
 };
 struct GemmArgs {
   const uint32_t* A;
@@ -145,6 +148,7 @@ struct GemmArgs {
   void* out;
   bool out_f64;
   int64_t ldo, ncols;
+  int64_t ldc;  // column stride of the output (1, or nrow for a transposed 2-D plan)
   double scale;
   bool kr;
   int accum;

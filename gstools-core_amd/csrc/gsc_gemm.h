@@ -190,7 +190,7 @@ __device__ __forceinline__ void table_quad(const ModeRec* __restrict__ modes, in
                                            int64_t nrow, int64_t arows, int64_t ncols,
                                            int64_t ncp, float2* __restrict__ S,
                                            uint32_t* __restrict__ A, float2* __restrict__ C,
-                                           int role, int64_t row, int c, int g) {
+                                           int role, int64_t row, int c, int g, int trans) {
   if (c >= nch || M <= 0) return;
   const int m0 = c * 16 + 4 * g;
   int axis;
@@ -202,15 +202,15 @@ __device__ __forceinline__ void table_quad(const ModeRec* __restrict__ modes, in
     amp = true;
     valid = true;
     x = gmem(xs)[row];
-  } else if (role == 1) {
+  } else if (role == 1) {  // a transposed 2-D plan takes its rows from axis 1
     if (row >= arows) return;
-    axis = dim - 2;
+    axis = trans ? 1 : dim - 2;
     amp = dim == 2;
     valid = row < nrow;
     if (valid) x = gmem(xa)[row];
   } else {
     if (row >= ncp) return;
-    axis = dim - 1;
+    axis = trans ? 0 : dim - 1;
     amp = false;
     valid = row < ncols;
     if (valid) x = gmem(xc)[row];
@@ -238,10 +238,11 @@ __global__ void __launch_bounds__(256)
 k_tables(const ModeRec* __restrict__ modes, int M, int dim, int nch,
          const double* __restrict__ xs, const double* __restrict__ xa,
          const double* __restrict__ xc, int64_t na, int64_t nrow, int64_t arows, int64_t ncols,
-         int64_t ncp, float2* __restrict__ S, uint32_t* __restrict__ A, float2* __restrict__ C) {
+         int64_t ncp, float2* __restrict__ S, uint32_t* __restrict__ A, float2* __restrict__ C,
+         int trans) {
   table_quad<T, NS>(modes, M, dim, nch, xs, xa, xc, na, nrow, arows, ncols, ncp, S, A, C,
                     blockIdx.z, static_cast<int64_t>(blockIdx.x) * 64 + (threadIdx.x & 63),
-                    blockIdx.y, threadIdx.x >> 6);
+                    blockIdx.y, threadIdx.x >> 6, trans);
 }
 
 // ---------------------------------------------------------------------------
@@ -277,11 +278,12 @@ struct TileJob {
   bool out_f64;
   int64_t ldo, out_row0, nvalid, ncols;
   double scale;
+  int64_t ldc;  // column stride of the output (1, or the row count of a transposed store)
 };
 
 // Output of one wave's 64 x 32 JN tile (rows past nvalid and columns past
 // ncols are not stored).
-template <int ACC, int JN, typename OutT, typename A2>
+template <int ACC, int JN, bool TR, typename OutT, typename A2>
 __device__ __forceinline__ void gemm_store(OutT* __restrict__ out, const TileJob& jb,
                                            const f32x16_t (&acc)[2][JN], const A2& acc2,
                                            int wave, int hl, int lr) {
@@ -295,7 +297,8 @@ __device__ __forceinline__ void gemm_store(OutT* __restrict__ out, const TileJob
     for (int j = 0; j < JN; ++j) {
       const int64_t col = jb.col0 + 32 * j + lr;
       if (col >= jb.ncols) continue;
-      __attribute__((address_space(1))) OutT* const p = gmem(out) + (jb.out_row0 + r0) * ldo + col;
+      __attribute__((address_space(1))) OutT* const p =
+          gmem(out) + (jb.out_row0 + r0) * ldo + (TR ? col * jb.ldc : col);
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         const int rr = 8 * (v >> 2) + (v & 3);
@@ -308,13 +311,16 @@ __device__ __forceinline__ void gemm_store(OutT* __restrict__ out, const TileJob
           if constexpr (sizeof(OutT) == 8) val = static_cast<double>(a) * jb.scale;
           else val = a * scale32;
         }
-        __builtin_nontemporal_store(val, p + rr * ldo);
+        // a transposed store writes 4-byte pieces of many lines: ordinary
+        // stores, so that L2 assembles each line before it is written
+        if constexpr (TR) p[rr * ldo] = val;
+        else __builtin_nontemporal_store(val, p + rr * ldo);
       }
     }
   }
 }
 
-template <typename T, int NS, bool KR, int ACC, int JN>
+template <typename T, int NS, bool KR, int ACC, int JN, bool TR = false>
 __device__ __forceinline__ void gemm_tile(GemmSmem<T, NS, JN>& sm, const TileJob& jb) {
   using V8 = typename Fmt<T>::v8;
   using A2 = typename AccT<ACC>::type;
@@ -522,8 +528,10 @@ __device__ __forceinline__ void gemm_tile(GemmSmem<T, NS, JN>& sm, const TileJob
   // One pointer per column block; the row steps 8 (v/4) + v%4 are uniform
   // multiples of ldo.  FP32 output is scaled in FP32 (the sums are FP32
   // already unless ACC_F64).
-  if (jb.out_f64) gemm_store<ACC, JN>(static_cast<double*>(jb.out), jb, acc, acc2, wave, hl, lr);
-  else gemm_store<ACC, JN>(static_cast<float*>(jb.out), jb, acc, acc2, wave, hl, lr);
+  if (jb.out_f64)
+    gemm_store<ACC, JN, TR>(static_cast<double*>(jb.out), jb, acc, acc2, wave, hl, lr);
+  else
+    gemm_store<ACC, JN, TR>(static_cast<float*>(jb.out), jb, acc, acc2, wave, hl, lr);
 }
 
 // Tile job of row tile rt, column tile ct of one contraction.
@@ -533,16 +541,16 @@ __device__ __forceinline__ TileJob make_job(const uint32_t* A, int64_t arows, in
                                             int64_t ncp, int64_t nrow, int64_t nb_out,
                                             void* out, bool out_f64, int64_t ldo,
                                             int64_t ncols, double scale, int64_t rt,
-                                            int64_t ct) {
+                                            int64_t ct, int64_t ldc = 1) {
   const int64_t nbt = arows / kBlockRows;
   const int64_t a = KR ? rt / nbt : 0;
   const int64_t arow0 = (rt - a * nbt) * kBlockRows;
   return TileJob{A,     arows, nch,  S,   na,      a,           C,
                  ncp,   arow0, ct * 32 * JN, out, out_f64, ldo,
-                 KR ? a * nb_out + arow0 : arow0, nrow - arow0, ncols, scale};
+                 KR ? a * nb_out + arow0 : arow0, nrow - arow0, ncols, scale, ldc};
 }
 
-template <typename T, int NS, bool KR, int ACC>
+template <typename T, int NS, bool KR, int ACC, bool TR>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(gemm_waves<ACC>(), gemm_waves<ACC>())))
 k_factored_gemm(GemmArgs g) {
@@ -550,8 +558,8 @@ k_factored_gemm(GemmArgs g) {
   __shared__ __attribute__((aligned(16))) GemmSmem<T, NS, JN> sm;
   const TileJob jb = make_job<KR, JN>(g.A, g.arows, g.nch, g.S, g.na, g.C, g.ncp, g.nrow,
                                       g.nb_out, g.out, g.out_f64, g.ldo, g.ncols, g.scale,
-                                      blockIdx.x, blockIdx.y);
-  gemm_tile<T, NS, KR, ACC, JN>(sm, jb);
+                                      blockIdx.x, blockIdx.y, g.ldc);
+  gemm_tile<T, NS, KR, ACC, JN, TR>(sm, jb);
 }
 
 // Tables of all factored realisations: one block per task (realisation,
@@ -588,16 +596,16 @@ k_ens_tables(const EnsReal* __restrict__ d, const int* __restrict__ fact,
     xp = r.ax0;
     nvalid = r.na;
     nrows = r.na;
-  } else if (role == 1) {
-    axis = r.dim - 2;
+  } else if (role == 1) {  // a transposed 2-D realisation takes its rows from y
+    axis = r.trans ? 1 : r.dim - 2;
     amp = r.dim == 2;
-    xp = r.dim == 3 ? r.ax1 : r.ax0;
+    xp = r.dim == 3 ? r.ax1 : (r.trans ? r.ax1 : r.ax0);
     nvalid = r.nrow;
     nrows = r.arows;
   } else {
-    axis = r.dim - 1;
+    axis = r.trans ? 0 : r.dim - 1;
     amp = false;
-    xp = r.dim == 3 ? r.ax2 : r.ax1;
+    xp = r.dim == 3 ? r.ax2 : (r.trans ? r.ax0 : r.ax1);
     nvalid = r.ncols;
     nrows = r.ncp;
   }
@@ -664,10 +672,13 @@ k_ens_gemm(const EnsReal* __restrict__ d, const int* __restrict__ fact,
       const EnsReal& r = d[fact[i]];
       const int64_t local = t - tile_start[i];
       const int64_t rt = local / r.col_tiles, ct = local - rt * r.col_tiles;
+      // a transposed realisation (2-D): rows are y, columns x of the C-order
+      // (x, y) output, so the row stride is 1 and the column stride nrow
       const TileJob jb = make_job<KR, JN>(r.A, r.arows, r.mp / 16, r.S, r.na, r.C, r.ncp, r.nrow,
-                                          r.nb_out, r.out, out_f64, r.ncols, r.ncols, r.scale,
-                                          rt, ct);
-      gemm_tile<T, NS, KR, ACC, JN>(sm, jb);
+                                          r.nb_out, r.out, out_f64, r.trans ? 1 : r.ncols,
+                                          r.ncols, r.scale, rt, ct, r.trans ? r.nrow : 1);
+      if (!KR && r.trans) gemm_tile<T, NS, KR, ACC, JN, true>(sm, jb);
+      else gemm_tile<T, NS, KR, ACC, JN, false>(sm, jb);
     }
   }
 }
@@ -677,14 +688,19 @@ k_ens_gemm(const EnsReal* __restrict__ d, const int* __restrict__ fact,
 template <typename T, int NS>
 hipError_t tables_launch(const TableArgs& a, dim3 grid, hipStream_t s) {
   gemm::k_tables<T, NS><<<grid, 256, 0, s>>>(a.modes, a.M, a.dim, a.nch, a.xs, a.xa, a.xc, a.na,
-                                              a.nrow, a.arows, a.ncols, a.ncp, a.S, a.A, a.C);
+                                              a.nrow, a.arows, a.ncols, a.ncp, a.S, a.A, a.C,
+                                              a.trans);
   return hipGetLastError();
 }
 
 template <typename T, int NS>
 hipError_t factored_gemm_launch(const GemmArgs& a, dim3 grid, hipStream_t s) {
   using namespace gemm;
-#define GSC_G(KR, ACC) k_factored_gemm<T, NS, KR, ACC><<<grid, 256, 0, s>>>(a)
+#define GSC_G(KR, ACC)                                                  \
+  do {                                                                  \
+    if (!KR && a.ldc != 1) k_factored_gemm<T, NS, KR, ACC, true><<<grid, 256, 0, s>>>(a); \
+    else k_factored_gemm<T, NS, KR, ACC, false><<<grid, 256, 0, s>>>(a);                  \
+  } while (0)
   if (a.kr) {
     if (a.accum == ACC_F32X2) GSC_G(true, ACC_F32X2);
     else if (a.accum == ACC_F32) GSC_G(true, ACC_F32);
