@@ -31,8 +31,10 @@ def _modes(d, m, ls, seed):
     return rng.normal(0, 1.0 / ls, size=(d, m)), rng.normal(size=m), rng.normal(size=m)
 
 
+# (3, 300) and (300, 700) take axis 1 as the GEMM row axis (it pads less to
+# 256-row tiles) and are stored transposed; (700, 300) keeps axis 0
 @pytest.mark.parametrize("shape", [(37, 50), (130, 129), (5, 7, 200), (33, 17, 129), (16, 16, 16),
-                                   (64, 1, 40), (3, 300)])
+                                   (64, 1, 40), (3, 300), (300, 700), (700, 300)])
 @pytest.mark.parametrize("precision", ["fp16x3", "bf16x3", "bf16x6", "fp16x1", "bf16x1"])
 def test_factored_small_grids(shape, precision):
     d = len(shape)
