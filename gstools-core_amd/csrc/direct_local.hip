@@ -813,8 +813,8 @@ k_bin_scatter(const double* __restrict__ pos, int64_t ld, int64_t N, int64_t blk
 }
 
 // r0 word (r0_hi | r0_lo << 16, the fp16 parts of the reduced FP64 phase at
-// the cell centre) of every (cell, smooth mode) pair, modes up to kModeChunk
-// past n_smooth (zero phase data there): computed once per cell here instead
+// the cell centre) of every (cell, smooth mode) pair, up to the end of the
+// last chunk (zero phase data past n_smooth): computed once per cell here instead
 // of once per wave of the cell in k_direct_local32.  One block per cell.
 template <int D>
 __global__ void __launch_bounds__(256)
@@ -826,7 +826,8 @@ k_cell_r0(const BinParams* __restrict__ prm, const double* __restrict__ centre,
   const double* xc = centre + 4 * static_cast<int64_t>(cell);
   const double X0 = xc[0], X1 = xc[1], X2 = xc[2];
   uint32_t* row = r0w + cell * r0ld;
-  const int end = P.n_smooth + kModeChunk;
+  // the chunks k_direct_local32 copies: n_smooth rounded up to whole chunks
+  const int end = (P.n_smooth + kModeChunk - 1) / kModeChunk * kModeChunk;
 #pragma unroll 2
   for (int i = threadIdx.x; i < end; i += 256) {
     const SmoothRec sr = sm[i];
