@@ -296,10 +296,9 @@ __device__ void split_modes(const ModeRec* __restrict__ modes, int M, const doub
                             float4* __restrict__ amp,
                             SmoothRec* __restrict__ sm, ModeRec* __restrict__ rough,
                             int* n_smooth, int* n_rough) {
-  __shared__ int part[256];
-  __shared__ int ns_s, nr_s;
-  if (threadIdx.x == 0) ns_s = nr_s = 0;
-  __syncthreads();
+  __shared__ int wcnt[4];  // smooth modes per wave of the current 256
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int ns_s = 0, nr_s = 0;   // list lengths so far (the same in every thread)
   for (int base = 0; base < M; base += 256) {
     const int m = base + threadIdx.x;
     ModeRec r{};
@@ -313,16 +312,19 @@ __device__ void split_modes(const ModeRec* __restrict__ modes, int M, const doub
 #pragma unroll
       for (int c = 0; c < D; ++c) smooth = smooth && isfinite(static_cast<double>(r.amp[c]));
     }
-    part[threadIdx.x] = smooth ? 1 : 0;
+    // order-keeping split: a ballot per wave, then the waves' counts
+    const unsigned long long bal = __ballot(smooth);
+    if (lane == 0) wcnt[wv] = __popcll(bal);
     __syncthreads();
-    for (int s = 1; s < 256; s <<= 1) {
-      const int v = threadIdx.x >= s ? part[threadIdx.x - s] : 0;
-      __syncthreads();
-      part[threadIdx.x] += v;
-      __syncthreads();
+    int before = __popcll(bal & ((1ull << lane) - 1));  // smooth modes before m in block
+    int total = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = wcnt[q];
+      if (q < wv) before += c;
+      total += c;
     }
     const int ns0 = ns_s, nr0 = nr_s;
-    const int before = part[threadIdx.x] - (smooth ? 1 : 0);  // smooth modes before m in block
     if (m < M) {
       if (smooth) {
         const int i = ns0 + before;
@@ -365,13 +367,9 @@ __device__ void split_modes(const ModeRec* __restrict__ modes, int M, const doub
         rough[nr0 + (threadIdx.x - before)] = r;
       }
     }
-    __syncthreads();
-    if (threadIdx.x == 255) {
-      const int blk = min(256, M - base);
-      ns_s = ns0 + part[255];
-      nr_s = nr0 + blk - part[255];
-    }
-    __syncthreads();
+    ns_s = ns0 + total;
+    nr_s = nr0 + min(256, M - base) - total;
+    __syncthreads();  // wcnt is read by every thread before the next 256 overwrite it
   }
   if (threadIdx.x == 0) {
     *n_smooth = ns_s;
