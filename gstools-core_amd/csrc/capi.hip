@@ -47,8 +47,9 @@ int fail(int code, const char* fmt, ...) {
 // growing device buffer, guarded by a mutex (calls on one device serialise).
 struct DeviceCtx {
   std::mutex mu;
-  bool checked = false;
+  std::once_flag arch_once;  // the gfx950 check runs once per device, from any thread
   bool ok = false;
+  char arch[64] = {0};
   hipStream_t stream = nullptr;
   void* buf = nullptr;
   size_t cap = 0;
@@ -77,18 +78,21 @@ int resolve_device(const gsc_opts* o, int* dev) {
 }
 
 // Checks once per device that it is a gfx950 (the only code object built).
+// Safe to call from concurrent first calls on different threads.
 int check_arch(int dev) {
   DeviceCtx& c = g_dev[dev];
-  if (!c.checked) {
+  std::call_once(c.arch_once, [&c, dev] {
     hipDeviceProp_t p;
-    if (hipGetDeviceProperties(&p, dev) != hipSuccess)
-      return fail(GSC_ERR_NODEVICE, "cannot query device %d", dev);
+    if (hipGetDeviceProperties(&p, dev) != hipSuccess) {
+      std::snprintf(c.arch, sizeof(c.arch), "(unknown)");
+      return;
+    }
+    std::snprintf(c.arch, sizeof(c.arch), "%s", p.gcnArchName);
     c.ok = std::strncmp(p.gcnArchName, "gfx950", 6) == 0;
-    c.checked = true;
-    if (!c.ok) return fail(GSC_ERR_NODEVICE, "device %d is %s, this library is built for gfx950",
-                           dev, p.gcnArchName);
-  }
-  return c.ok ? GSC_OK : fail(GSC_ERR_NODEVICE, "device %d is not a gfx950", dev);
+  });
+  return c.ok ? GSC_OK
+              : fail(GSC_ERR_NODEVICE, "device %d is %s, this library is built for gfx950", dev,
+                     c.arch);
 }
 
 int validate_common(int dim, int64_t M, int64_t N, const double* cov, const double* z1,
@@ -332,8 +336,11 @@ int64_t grid_points(int dim, const int64_t* shape) {
 
 // Path choice: the factored contraction needs a 2-D or 3-D grid; below a
 // 16-point last axis or 16 rows its GEMM block tiles are mostly padding.
-int choose_grid_path(int dim, const int64_t* shape, const gsc_opts* o) {
-  if (dim == 1) return GSC_PATH_DIRECT;
+// Without modes the field is zero: the direct kernels write those zeros (the
+// factored tables would hold nothing, so that path is never taken for M = 0,
+// whatever was asked).
+int choose_grid_path(int dim, int64_t M, const int64_t* shape, const gsc_opts* o) {
+  if (dim == 1 || M == 0) return GSC_PATH_DIRECT;
   if (o->path == GSC_PATH_DIRECT || o->path == GSC_PATH_FACTORED) return o->path;
   const int64_t cols = shape[dim - 1];
   const int64_t rows = grid_points(dim, shape) / (cols > 0 ? cols : 1);
@@ -577,7 +584,7 @@ int gsc_grid_path(int dim, int64_t M, const int64_t* shape, const gsc_opts* opts
   gsc_opts tmp;
   const gsc_opts* o = opts_or_default(opts, &tmp);
   if (validate_grid(dim, M, shape, shape) || validate_precision(o)) return -1;
-  const int path = choose_grid_path(dim, shape, o);
+  const int path = choose_grid_path(dim, M, shape, o);
   if (path == GSC_PATH_FACTORED && dim < 2) return -1;
   return path;
 }
@@ -586,7 +593,7 @@ size_t gsc_grid_workspace_bytes(int dim, int64_t M, const int64_t* shape, const 
   gsc_opts tmp;
   const gsc_opts* o = opts_or_default(opts, &tmp);
   if (validate_grid(dim, M, shape, shape) || validate_precision(o)) return 0;
-  return grid_work_bytes(dim, M, shape, o, choose_grid_path(dim, shape, o));
+  return grid_work_bytes(dim, M, shape, o, choose_grid_path(dim, M, shape, o));
 }
 
 int gsc_summate_grid(int dim, int64_t M, const double* cov, int64_t cov_sd, int64_t cov_sm,
@@ -598,7 +605,7 @@ int gsc_summate_grid(int dim, int64_t M, const double* cov, int64_t cov_sd, int6
   int rc = validate_grid(dim, M, shape, axes);
   if (rc || (rc = validate_opts(o)) || (rc = validate_precision(o))) return rc;
   if (M > 0 && (!cov || !z1 || !z2)) return fail(GSC_ERR_INVALID, "null mode array");
-  const int path = choose_grid_path(dim, shape, o);
+  const int path = choose_grid_path(dim, M, shape, o);
   if (path == GSC_PATH_FACTORED && dim < 2)
     return fail(GSC_ERR_INVALID, "the factored path needs a 2-D or 3-D grid");
   const int64_t N = grid_points(dim, shape);
@@ -680,7 +687,7 @@ int gsc_summate_grid_dev(int dim, int64_t M, const double* cov, const double* z1
   int rc = validate_grid(dim, M, shape, axes);
   if (rc || (rc = validate_opts(o)) || (rc = validate_precision(o))) return rc;
   if (M > 0 && (!cov || !z1 || !z2)) return fail(GSC_ERR_INVALID, "null mode array");
-  const int path = choose_grid_path(dim, shape, o);
+  const int path = choose_grid_path(dim, M, shape, o);
   if (path == GSC_PATH_FACTORED && dim < 2)
     return fail(GSC_ERR_INVALID, "the factored path needs a 2-D or 3-D grid");
   const size_t need = grid_work_bytes(dim, M, shape, o, path);
@@ -763,7 +770,7 @@ int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes
     e.scale = scales ? scales[r] : o->scale;
     gsc_opts po = *o;
     po.path = paths ? paths[r] : GSC_PATH_AUTO;
-    e.path = choose_grid_path(dim, sh, &po);
+    e.path = choose_grid_path(dim, M, sh, &po);
     m_off += M;
     for (int i = 0; i < dim; ++i) a_off += sh[i];
     o_off += e.npts;

@@ -38,3 +38,18 @@ def sigma_tol(m: int, rel: float = 1e-5) -> float:
     rel * sqrt(M) on the unscaled one.
     """
     return rel * float(np.sqrt(max(m, 1)))
+
+
+@pytest.fixture(scope="session")
+def c5_subset():
+    """The heavy-tailed 512^3 grid (M = 4000) and the FP64 oracle on a seeded
+    10^6-point random subset of it (about 4e9 point-modes on the host)."""
+    import oracle
+    from gstools_core import workloads
+
+    w = workloads.c5()
+    md = w.modes
+    idx = workloads.subset_indices(w.n_points, 1_000_000, seed=5)
+    pts = workloads.grid_points(w.axes, idx)
+    ref = oracle.summate(md.cov_samples, md.z1, md.z2, pts)
+    return w, idx, pts, ref

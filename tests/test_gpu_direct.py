@@ -71,7 +71,7 @@ def test_c2_subset_against_oracle():
     w = workloads.c2()
     md = w.modes
     gpu = gstools_core.summate(md.cov_samples, md.z1, md.z2, w.pos)
-    idx = workloads.subset_indices(w.n_points, 100_000, seed=2)
+    idx = workloads.subset_indices(w.n_points, 1_000_000, seed=2)
     ref = oracle.summate(md.cov_samples, md.z1, md.z2, w.pos[:, idx])
     err = _err(gpu[idx], ref) * md.scale
     print(f"c2 max|err|/sigma = {err:.3e} on {idx.size} points")
@@ -85,25 +85,23 @@ def test_c2_incompr_subset_against_oracle():
     md = w.modes
     gpu = gstools_core.summate_incompr(md.cov_samples, md.z1, md.z2, w.pos)
     assert gpu.shape == (3, 4_000_000)
-    idx = workloads.subset_indices(w.n_points, 20_000, seed=3)
+    idx = workloads.subset_indices(w.n_points, 1_000_000, seed=3)
     ref = oracle.summate_incompr(md.cov_samples, md.z1, md.z2, w.pos[:, idx])
     err = _err(gpu[:, idx], ref) * md.scale
     print(f"c2i max|err|/sigma = {err:.3e} on {idx.size} points")
     assert err <= 1e-5
 
 
-def test_heavy_tailed_large_phase():
-    # exponential-model modes of the 512^3 heavy-tailed case on scattered
+def test_heavy_tailed_large_phase(c5_subset):
+    # exponential-model modes of the 512^3 heavy-tailed case on 10^6 scattered
     # points of that grid: |k . x| reaches ~1e5 rad; FP64 range reduction
-    w = workloads.c5(side=512, m=4000)
+    w, idx, pts, ref = c5_subset
     md = w.modes
-    idx = workloads.subset_indices(w.n_points, 20_000, seed=5)
-    pts = workloads.grid_points(w.axes, idx)
-    gpu = gstools_core.summate(md.cov_samples, md.z1, md.z2, pts)
-    ref = oracle.summate(md.cov_samples, md.z1, md.z2, pts)
-    err = _err(gpu, ref) * md.scale
-    print(f"c5 direct max|err|/sigma = {err:.3e}")
-    assert err <= 1e-5
+    for variant in ("local", "valu"):
+        gpu = gstools_core.summate(md.cov_samples, md.z1, md.z2, pts, variant=variant)
+        err = _err(gpu, ref) * md.scale
+        print(f"c5 point list ({variant}) max|err|/sigma = {err:.3e} on {idx.size} points")
+        assert err <= 1e-5
 
 
 def test_fourier_random_against_oracle():
@@ -258,21 +256,46 @@ def test_direct_variants_against_oracle(variant):
 
 
 def test_direct_variants_agree_at_scale():
-    w = workloads.c2(n=2_000_000)
+    # every one of the 1e7 points of the C2 workload: the cell-local kernel
+    # (fp16 split local phases) against the FP64-phase VALU kernel; the FP64
+    # matrix-core tiles against the VALU kernel on the first 2e6
+    w = workloads.c2()
     md = w.modes
-    a = gstools_core.summate(md.cov_samples, md.z1, md.z2, w.pos, variant="mfma")
     b = gstools_core.summate(md.cov_samples, md.z1, md.z2, w.pos, variant="valu")
     c = gstools_core.summate(md.cov_samples, md.z1, md.z2, w.pos, variant="local")
-    assert _err(a, b) * md.scale <= 2e-6
     err = _err(c, b) * md.scale
-    print(f"local vs valu at 2e6 points: {err:.3e} sigma")
+    print(f"local vs valu at all {w.n_points} points: {err:.3e} sigma")
     assert err <= 5e-6
-    wi = workloads.c2(n=1_000_000)
+    head = np.ascontiguousarray(w.pos[:, :2_000_000])
+    a = gstools_core.summate(md.cov_samples, md.z1, md.z2, head, variant="mfma")
+    assert _err(a, b[:2_000_000]) * md.scale <= 2e-6
+    wi = workloads.c2(n=4_000_000)
     bi = gstools_core.summate_incompr(md.cov_samples, md.z1, md.z2, wi.pos, variant="valu")
     ci = gstools_core.summate_incompr(md.cov_samples, md.z1, md.z2, wi.pos, variant="local")
     err = _err(ci, bi) * md.scale
-    print(f"incompr local vs valu at 1e6 points: {err:.3e} sigma")
+    print(f"incompr local vs valu at all 4e6 points: {err:.3e} sigma")
     assert err <= 5e-6
+
+
+@pytest.mark.parametrize("num_threads", [1, 3, 16])
+def test_num_threads_on_strided_host_inputs(num_threads):
+    # num_threads (field.rs:48-51, lib.rs:41) sizes the host pool that gathers
+    # strided inputs; every pool size gives the default call's result bit for bit
+    w = workloads.c2(n=1_000_000)
+    md = w.modes
+    pos_f = np.asfortranarray(w.pos)          # (3, N) with point stride 3
+    pos_r = w.pos[:, ::-1]                    # negative point stride
+    base = gstools_core.summate(md.cov_samples, md.z1, md.z2, w.pos)
+    g = gstools_core.summate(md.cov_samples, md.z1, md.z2, pos_f, num_threads=num_threads)
+    np.testing.assert_array_equal(g, base)
+    g = gstools_core.summate(md.cov_samples, md.z1, md.z2, pos_r, num_threads=num_threads)
+    np.testing.assert_array_equal(g, base[::-1])
+    gi = gstools_core.summate_incompr(md.cov_samples, md.z1, md.z2, pos_f,
+                                      num_threads=num_threads)
+    np.testing.assert_array_equal(gi, gstools_core.summate_incompr(md.cov_samples, md.z1, md.z2,
+                                                                   w.pos))
+    ref = oracle.summate(md.cov_samples, md.z1, md.z2, np.ascontiguousarray(w.pos[:, :20_000]))
+    assert _err(g[::-1][:20_000], ref) * md.scale <= 1e-5
 
 
 @pytest.mark.parametrize("scale", [1e-35, 1e30])

@@ -97,7 +97,7 @@ def test_summate_structured_is_the_grid_entry_point():
 def test_c3_grid_256_factored_and_direct():
     w = workloads.c3()
     md = w.modes
-    idx = workloads.subset_indices(w.n_points, 100_000, seed=3)
+    idx = workloads.subset_indices(w.n_points, 1_000_000, seed=3)
     pts = workloads.grid_points(w.axes, idx)
     ref = oracle.summate(md.cov_samples, md.z1, md.z2, pts)
     for prec in ("fp16x3", "bf16x6", "bf16x3"):
@@ -114,17 +114,19 @@ def test_c3_grid_256_factored_and_direct():
     assert err <= 1e-5
 
 
-def test_c5_grid_512_heavy_tailed():
-    w = workloads.c5()
+def test_c5_grid_512_heavy_tailed(c5_subset):
+    # every split format the benchmark times, on the seeded 10^6-point subset;
+    # bf16x3 is reported against its own (looser) bound: its 16-bit
+    # significand does not reach 1e-5 sigma
+    w, idx, _, ref = c5_subset
     md = w.modes
-    idx = workloads.subset_indices(w.n_points, 20_000, seed=5)
-    pts = workloads.grid_points(w.axes, idx)
-    ref = oracle.summate(md.cov_samples, md.z1, md.z2, pts)
-    for acc, tol in (("f64", 1e-5), ("f32x2", 1e-5), ("f32", 2e-5)):
+    cases = (("fp16x3", "f64", 1e-5), ("fp16x3", "f32x2", 1e-5), ("fp16x3", "f32", 2e-5),
+             ("bf16x6", "f32x2", 1e-5), ("bf16x3", "f32x2", TOL["bf16x3"]))
+    for prec, acc, tol in cases:
         g = grid.summate_grid(md.cov_samples, md.z1, md.z2, w.axes, path="factored",
-                              precision="fp16x3", out_dtype=np.float32, accum=acc)
+                              precision=prec, out_dtype=np.float32, accum=acc)
         err = _scaled_err(g.ravel()[idx], ref, md.m)
-        print(f"c5 factored fp16x3 accum {acc}: max|err|/sigma = {err:.3e}")
+        print(f"c5 factored {prec} accum {acc}: max|err|/sigma = {err:.3e} on {idx.size} points")
         assert err <= tol
         del g
     g = grid.summate_grid(md.cov_samples, md.z1, md.z2, w.axes, path="direct",
@@ -132,6 +134,20 @@ def test_c5_grid_512_heavy_tailed():
     err = _scaled_err(g.ravel()[idx], ref, md.m)
     print(f"c5 direct: max|err|/sigma = {err:.3e}")
     assert err <= 1e-5
+
+
+def test_factored_zero_modes_after_a_nonempty_call():
+    # M = 0 is a zero field on every path; an earlier call must leave nothing
+    # behind in the library's reused device buffers
+    cov, z1, z2 = _modes(2, 300, 4.0, 9)
+    axes = [np.arange(64.0), np.arange(64.0)]
+    grid.summate_grid(cov, z1, z2, axes, path="factored")
+    for path in ("factored", "auto", "direct"):
+        r = grid.summate_grid(np.zeros((2, 0)), np.zeros(0), np.zeros(0), axes, path=path)
+        np.testing.assert_array_equal(r, np.zeros((64, 64)))
+    r = grid.summate_grid(np.zeros((3, 0)), np.zeros(0), np.zeros(0), [np.arange(40.0)] * 3,
+                          path="factored", out_dtype=np.float32)
+    np.testing.assert_array_equal(r, np.zeros((40, 40, 40), np.float32))
 
 
 def test_c4_realisations_sample():
