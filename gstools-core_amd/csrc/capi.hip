@@ -315,6 +315,44 @@ int dev_common(int kind, int dim, int64_t M, int64_t N, const double* sf, const 
   return GSC_OK;
 }
 
+// ---- the no-allocation entry point ------------------------------------------------
+// gsc_summate_dev_noalloc runs the kernels of gsc_summate_dev on this array
+// instead of a caller workspace.  Every device the library runs on holds its
+// own copy until the process ends.  In host code `scratch` is only a stand-in
+// name: its device address on each device comes from hipGetSymbolAddress,
+// looked up once per device (with that device current) and kept in
+// g_scratch; kernels only ever receive that address.
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[GSC_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+struct ScratchSlot {
+  std::once_flag once;
+  void* ptr = nullptr;
+  hipError_t err = hipSuccess;
+};
+ScratchSlot g_scratch[kMaxDevices];
+
+int scratch_address(int dev, void** ptr) {
+  ScratchSlot& sl = g_scratch[dev];
+  std::call_once(sl.once, [&sl, dev] {
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess) cur = -1;
+    if (cur != dev && (sl.err = hipSetDevice(dev)) != hipSuccess) return;
+    void* p = nullptr;
+    sl.err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (sl.err == hipSuccess) sl.ptr = p;
+    if (cur >= 0 && cur != dev) (void)hipSetDevice(cur);
+  });
+  if (sl.err != hipSuccess || !sl.ptr)
+    return fail(GSC_ERR_HIP, "device scratch of device %d is unavailable: %s", dev,
+                hipGetErrorString(sl.err != hipSuccess ? sl.err : hipErrorInvalidSymbol));
+  *ptr = sl.ptr;
+  return GSC_OK;
+}
+
 // ---- structured grids -------------------------------------------------------
 
 int validate_grid(int dim, int64_t M, const int64_t* shape, const void* axes) {
@@ -562,6 +600,28 @@ int gsc_summate_dev(int dim, int64_t M, int64_t N, const double* cov, const doub
                     size_t work_bytes, const gsc_opts* opts, void* stream) {
   return dev_common(gsc::KIND_SCALAR, dim, M, N, nullptr, cov, z1, z2, pos, pos_ld, out, 0, work,
                     work_bytes, opts, stream);
+}
+
+int gsc_summate_dev_noalloc(int dim, int64_t M, int64_t N, const double* cov, const double* z1,
+                            const double* z2, const double* pos, int64_t pos_ld, void* out,
+                            const gsc_opts* opts, void* stream) {
+  gsc_opts tmp;
+  const gsc_opts* o = opts_or_default(opts, &tmp);
+  int rc = validate_common(dim, M, N, cov, z1, z2, pos, out);
+  if (rc || (rc = validate_opts(o))) return rc;
+  const size_t need = direct_work_bytes(gsc::KIND_SCALAR, dim, M, N, o);
+  if (need > (size_t)GSC_NOALLOC_SCRATCH_BYTES)
+    return fail(GSC_ERR_WORKSPACE,
+                "this call needs %zu bytes of device scratch, above the %zu bytes the "
+                "no-allocation entry point holds (use gsc_summate_dev with a workspace)",
+                need, (size_t)GSC_NOALLOC_SCRATCH_BYTES);
+  if (N == 0) return GSC_OK;
+  int dev = 0;
+  if ((rc = resolve_device(o, &dev)) || (rc = check_arch(dev))) return rc;
+  void* work = nullptr;
+  if ((rc = scratch_address(dev, &work))) return rc;
+  return dev_common(gsc::KIND_SCALAR, dim, M, N, nullptr, cov, z1, z2, pos, pos_ld, out, 0, work,
+                    (size_t)GSC_NOALLOC_SCRATCH_BYTES, o, stream);
 }
 
 int gsc_summate_incompr_dev(int dim, int64_t M, int64_t N, const double* cov, const double* z1,

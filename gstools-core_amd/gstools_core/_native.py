@@ -69,6 +69,8 @@ SIGNATURES = {
                                    _i64, _vp, _i64, _i64, _vp, _i64, _optp]),
     "gsc_summate_dev": (_int, [_int, _i64, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _sz, _optp,
                                _vp]),
+    "gsc_summate_dev_noalloc": (_int, [_int, _i64, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _optp,
+                                       _vp]),
     "gsc_summate_incompr_dev": (_int, [_int, _i64, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _i64, _vp,
                                        _sz, _optp, _vp]),
     "gsc_summate_fourier_dev": (_int, [_int, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp,

@@ -184,6 +184,24 @@ int gsc_summate_incompr_dev(int dim, int64_t M, int64_t N, const double* cov, co
                             const double* z2, const double* pos, int64_t pos_ld, void* out,
                             int64_t out_ld, void* work, size_t work_bytes, const gsc_opts* opts,
                             void* stream);
+
+/*
+ * gsc_summate_dev without a workspace argument: the same kernels and the
+ * same result, with the scratch taken from a device array of
+ * GSC_NOALLOC_SCRATCH_BYTES that the library holds on every device it runs on
+ * until the process ends.  It calls no hipMalloc / hipFree, so callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid allocation) can use it.  No counterpart in the
+ * reference.  Calls whose workspace (gsc_workspace_bytes) exceeds the array
+ * return GSC_ERR_WORKSPACE.  The array belongs to the device opts->device (-1:
+ * the current device), which must be the device of `stream`.  All calls on a
+ * device share its one array and leave their data in it: calls that overlap
+ * on one device (different streams or threads without ordering) race.
+ */
+#define GSC_NOALLOC_SCRATCH_BYTES ((size_t)512 << 20)
+int gsc_summate_dev_noalloc(int dim, int64_t M, int64_t N, const double* cov, const double* z1,
+                            const double* z2, const double* pos, int64_t pos_ld, void* out,
+                            const gsc_opts* opts, void* stream);
 int gsc_summate_fourier_dev(int dim, int64_t M, int64_t N, const double* sf, const double* modes,
                             const double* z1, const double* z2, const double* pos, int64_t pos_ld,
                             void* out, void* work, size_t work_bytes, const gsc_opts* opts,
