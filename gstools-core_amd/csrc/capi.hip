@@ -385,22 +385,67 @@ int choose_grid_path(int dim, int64_t M, const int64_t* shape, const gsc_opts* o
   return (cols >= 16 && rows >= 16) ? GSC_PATH_FACTORED : GSC_PATH_DIRECT;
 }
 
-size_t grid_work_bytes(int dim, int64_t M, const int64_t* shape, const gsc_opts* o, int path) {
-  if (path == GSC_PATH_FACTORED) return gsc::plan_factored(dim, M, shape, o->precision).bytes;
-  return gsc::mode_table_bytes(dim, M);
+// Field kinds of the grid entry points (gsc_field) and their mode kinds.
+int field_kind(int field) {
+  return field == GSC_FIELD_INCOMPR ? gsc::KIND_INCOMPR
+                                    : (field == GSC_FIELD_FOURIER ? gsc::KIND_FOURIER
+                                                                  : gsc::KIND_SCALAR);
+}
+int field_comps(int field, int dim) { return field == GSC_FIELD_INCOMPR ? dim : 1; }
+
+// Workspace of a grid call.  Factored: the plan (mode table and per-axis
+// tables; the incompressible field has dim component blocks of the last-axis
+// table).  Direct: the mode table; the incompressible field also expands the
+// grid into a (dim, N) point list and runs the point-list kernel on it.
+size_t grid_work_bytes(int field, int dim, int64_t M, const int64_t* shape, const gsc_opts* o,
+                       int path) {
+  if (path == GSC_PATH_FACTORED)
+    return gsc::plan_factored(dim, M, shape, o->precision, field_comps(field, dim)).bytes;
+  if (field != GSC_FIELD_INCOMPR) return gsc::mode_table_bytes(dim, M);
+  const int64_t N = grid_points(dim, shape);
+  return align256(sizeof(double) * (size_t)dim * (size_t)N) +
+         direct_work_bytes(gsc::KIND_INCOMPR, dim, M, N, o);
 }
 
-int grid_launch(int dim, int64_t M, const double* cov, const double* z1, const double* z2,
-                const double* const* axes, const int64_t* shape, void* out, void* work,
-                const gsc_opts* o, int path, hipStream_t s) {
-  GSC_HIP(gsc::launch_prep(gsc::KIND_SCALAR, dim, M, cov, z1, z2, nullptr, work, s));
+// `work` starts with the mode table in every case.
+int grid_launch(int field, int dim, int64_t M, const double* sf, const double* cov,
+                const double* z1, const double* z2, const double* const* axes,
+                const int64_t* shape, void* out, void* work, const gsc_opts* o, int path,
+                hipStream_t s) {
   if (path == GSC_PATH_FACTORED) {
-    const gsc::FactoredPlan p = gsc::plan_factored(dim, M, shape, o->precision);
+    // the factored tables take the amplitude A (sf A for the Fourier method)
+    // from amp[0]; the projector of the incompressible field is applied to the
+    // last-axis table
+    const int kind = field == GSC_FIELD_FOURIER ? gsc::KIND_FOURIER : gsc::KIND_SCALAR;
+    GSC_HIP(gsc::launch_prep(kind, dim, M, cov, z1, z2, sf, work, s));
+    const gsc::FactoredPlan p =
+        gsc::plan_factored(dim, M, shape, o->precision, field_comps(field, dim));
     GSC_HIP(gsc::launch_factored(p, M, axes, work, out, o->out_dtype, o->scale, o->accum, s));
-  } else {
+    return GSC_OK;
+  }
+  if (field != GSC_FIELD_INCOMPR) {
+    GSC_HIP(gsc::launch_prep(field_kind(field), dim, M, cov, z1, z2, sf, work, s));
     GSC_HIP(gsc::launch_direct_grid(dim, M, shape, axes, work, out, o->out_dtype, o->scale,
                                     o->direct_variant, s));
+    return GSC_OK;
   }
+  const int64_t N = grid_points(dim, shape);
+  const size_t b_pos = align256(sizeof(double) * (size_t)dim * (size_t)N);
+  double* pos = static_cast<double*>(work);
+  void* dwork = static_cast<char*>(work) + b_pos;
+  GSC_HIP(gsc::launch_expand_grid(dim, shape, axes, pos, s));
+  GSC_HIP(gsc::launch_prep(gsc::KIND_INCOMPR, dim, M, cov, z1, z2, nullptr, dwork, s));
+  GSC_HIP(launch_direct(gsc::KIND_INCOMPR, dim, M, N, dwork, pos, N, out, N, o, s));
+  return GSC_OK;
+}
+
+int validate_field(int field, int dim, int64_t M, const double* sf) {
+  if (field < GSC_FIELD_SCALAR || field > GSC_FIELD_FOURIER)
+    return fail(GSC_ERR_INVALID, "unknown field kind %d", field);
+  if (field == GSC_FIELD_INCOMPR && dim != 2 && dim != 3)
+    return fail(GSC_ERR_DIM, "Only two- and three-dimensional problems are supported.");
+  if (field == GSC_FIELD_FOURIER && M > 0 && !sf)
+    return fail(GSC_ERR_INVALID, "null spectrum_factor array");
   return GSC_OK;
 }
 
@@ -649,21 +694,32 @@ int gsc_grid_path(int dim, int64_t M, const int64_t* shape, const gsc_opts* opts
   return path;
 }
 
-size_t gsc_grid_workspace_bytes(int dim, int64_t M, const int64_t* shape, const gsc_opts* opts) {
+size_t gsc_grid_field_workspace_bytes(int field, int dim, int64_t M, const int64_t* shape,
+                                      const gsc_opts* opts) {
   gsc_opts tmp;
   const gsc_opts* o = opts_or_default(opts, &tmp);
-  if (validate_grid(dim, M, shape, shape) || validate_precision(o)) return 0;
-  return grid_work_bytes(dim, M, shape, o, choose_grid_path(dim, M, shape, o));
+  if (validate_grid(dim, M, shape, shape) || validate_precision(o) ||
+      field < GSC_FIELD_SCALAR || field > GSC_FIELD_FOURIER ||
+      (field == GSC_FIELD_INCOMPR && dim != 2 && dim != 3))
+    return 0;
+  return grid_work_bytes(field, dim, M, shape, o, choose_grid_path(dim, M, shape, o));
 }
 
-int gsc_summate_grid(int dim, int64_t M, const double* cov, int64_t cov_sd, int64_t cov_sm,
-                     const double* z1, int64_t z1_s, const double* z2, int64_t z2_s,
-                     const double* const* axes, const int64_t* axis_s, const int64_t* shape,
-                     void* out, const gsc_opts* opts) {
+size_t gsc_grid_workspace_bytes(int dim, int64_t M, const int64_t* shape, const gsc_opts* opts) {
+  return gsc_grid_field_workspace_bytes(GSC_FIELD_SCALAR, dim, M, shape, opts);
+}
+
+int gsc_summate_grid_field(int field, int dim, int64_t M, const double* sf, int64_t sf_s,
+                           const double* cov, int64_t cov_sd, int64_t cov_sm, const double* z1,
+                           int64_t z1_s, const double* z2, int64_t z2_s,
+                           const double* const* axes, const int64_t* axis_s,
+                           const int64_t* shape, void* out, const gsc_opts* opts) {
   gsc_opts tmp;
   const gsc_opts* o = opts_or_default(opts, &tmp);
   int rc = validate_grid(dim, M, shape, axes);
-  if (rc || (rc = validate_opts(o)) || (rc = validate_precision(o))) return rc;
+  if (rc || (rc = validate_opts(o)) || (rc = validate_precision(o)) ||
+      (rc = validate_field(field, dim, M, sf)))
+    return rc;
   if (M > 0 && (!cov || !z1 || !z2)) return fail(GSC_ERR_INVALID, "null mode array");
   const int path = choose_grid_path(dim, M, shape, o);
   if (path == GSC_PATH_FACTORED && dim < 2)
@@ -674,13 +730,15 @@ int gsc_summate_grid(int dim, int64_t M, const double* cov, int64_t cov_sd, int6
   for (int i = 0; i < dim; ++i)
     if (!axes[i]) return fail(GSC_ERR_INVALID, "null axis %d", i);
   const size_t elt = o->out_dtype == GSC_F32 ? 4 : 8;
+  const int ncomp = field_comps(field, dim);
   try {
     const int64_t Mm = std::max<int64_t>(M, 1);
-    std::vector<double> modes((size_t)((dim + 2) * Mm), 0.0);
+    std::vector<double> modes((size_t)((dim + 3) * Mm), 0.0);
     if (M > 0) {
       gather2(cov, dim, M, cov_sd, cov_sm, modes.data(), 1);
       gather1(z1, M, z1_s, modes.data() + dim * M);
       gather1(z2, M, z2_s, modes.data() + (dim + 1) * M);
+      if (field == GSC_FIELD_FOURIER) gather1(sf, M, sf_s, modes.data() + (dim + 2) * M);
     }
     int64_t ax_total = 0;
     for (int i = 0; i < dim; ++i) ax_total += shape[i];
@@ -697,10 +755,10 @@ int gsc_summate_grid(int dim, int64_t M, const double* cov, int64_t cov_sd, int6
     GSC_HIP(hipSetDevice(dev));
     if ((rc = check_arch(dev))) return rc;
     if (!c.stream) GSC_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
-    const size_t b_work = align256(grid_work_bytes(dim, M, shape, o, path));
+    const size_t b_work = align256(grid_work_bytes(field, dim, M, shape, o, path));
     const size_t b_modes = align256(sizeof(double) * modes.size());
     const size_t b_ax = align256(sizeof(double) * ax.size());
-    const size_t b_out = align256(elt * N);
+    const size_t b_out = align256(elt * (size_t)ncomp * (size_t)N);
     const size_t need = b_work + b_modes + b_ax + b_out;
     if (need > c.cap) {
       if (c.buf) GSC_HIP(hipFree(c.buf));
@@ -725,10 +783,12 @@ int gsc_summate_grid(int dim, int64_t M, const double* cov, int64_t cov_sd, int6
       d_axes[i] = d_ax + off;
       off += shape[i];
     }
-    if ((rc = grid_launch(dim, M, d_modes, d_modes + dim * M, d_modes + (dim + 1) * M, d_axes,
-                          shape, d_out, d_work, o, path, c.stream)))
+    if ((rc = grid_launch(field, dim, M, d_modes + (dim + 2) * M, d_modes, d_modes + dim * M,
+                          d_modes + (dim + 1) * M, d_axes, shape, d_out, d_work, o, path,
+                          c.stream)))
       return rc;
-    GSC_HIP(hipMemcpyAsync(out, d_out, elt * N, hipMemcpyDeviceToHost, c.stream));
+    GSC_HIP(hipMemcpyAsync(out, d_out, elt * (size_t)ncomp * (size_t)N, hipMemcpyDeviceToHost,
+                           c.stream));
     GSC_HIP(hipStreamSynchronize(c.stream));
   } catch (const std::bad_alloc&) {
     return fail(GSC_ERR_ALLOC, "host allocation failed");
@@ -738,19 +798,30 @@ int gsc_summate_grid(int dim, int64_t M, const double* cov, int64_t cov_sd, int6
   return GSC_OK;
 }
 
-int gsc_summate_grid_dev(int dim, int64_t M, const double* cov, const double* z1,
-                         const double* z2, const double* const* axes, const int64_t* shape,
-                         void* out, void* work, size_t work_bytes, const gsc_opts* opts,
-                         void* stream) {
+int gsc_summate_grid(int dim, int64_t M, const double* cov, int64_t cov_sd, int64_t cov_sm,
+                     const double* z1, int64_t z1_s, const double* z2, int64_t z2_s,
+                     const double* const* axes, const int64_t* axis_s, const int64_t* shape,
+                     void* out, const gsc_opts* opts) {
+  return gsc_summate_grid_field(GSC_FIELD_SCALAR, dim, M, nullptr, 0, cov, cov_sd, cov_sm, z1,
+                                z1_s, z2, z2_s, axes, axis_s, shape, out, opts);
+}
+
+int gsc_summate_grid_field_dev(int field, int dim, int64_t M, const double* sf,
+                               const double* cov, const double* z1, const double* z2,
+                               const double* const* axes, const int64_t* shape, void* out,
+                               void* work, size_t work_bytes, const gsc_opts* opts,
+                               void* stream) {
   gsc_opts tmp;
   const gsc_opts* o = opts_or_default(opts, &tmp);
   int rc = validate_grid(dim, M, shape, axes);
-  if (rc || (rc = validate_opts(o)) || (rc = validate_precision(o))) return rc;
+  if (rc || (rc = validate_opts(o)) || (rc = validate_precision(o)) ||
+      (rc = validate_field(field, dim, M, sf)))
+    return rc;
   if (M > 0 && (!cov || !z1 || !z2)) return fail(GSC_ERR_INVALID, "null mode array");
   const int path = choose_grid_path(dim, M, shape, o);
   if (path == GSC_PATH_FACTORED && dim < 2)
     return fail(GSC_ERR_INVALID, "the factored path needs a 2-D or 3-D grid");
-  const size_t need = grid_work_bytes(dim, M, shape, o, path);
+  const size_t need = grid_work_bytes(field, dim, M, shape, o, path);
   if (!work || work_bytes < need)
     return fail(GSC_ERR_WORKSPACE, "workspace of %zu bytes is below the %zu required", work_bytes,
                 need);
@@ -758,8 +829,16 @@ int gsc_summate_grid_dev(int dim, int64_t M, const double* cov, const double* z1
   if (!out) return fail(GSC_ERR_INVALID, "null output");
   for (int i = 0; i < dim; ++i)
     if (!axes[i]) return fail(GSC_ERR_INVALID, "null axis %d", i);
-  return grid_launch(dim, M, cov, z1, z2, axes, shape, out, work, o, path,
+  return grid_launch(field, dim, M, sf, cov, z1, z2, axes, shape, out, work, o, path,
                      static_cast<hipStream_t>(stream));
+}
+
+int gsc_summate_grid_dev(int dim, int64_t M, const double* cov, const double* z1,
+                         const double* z2, const double* const* axes, const int64_t* shape,
+                         void* out, void* work, size_t work_bytes, const gsc_opts* opts,
+                         void* stream) {
+  return gsc_summate_grid_field_dev(GSC_FIELD_SCALAR, dim, M, nullptr, cov, z1, z2, axes, shape,
+                                    out, work, work_bytes, opts, stream);
 }
 
 }  // extern "C"

@@ -80,11 +80,45 @@ k_ens_direct(const EnsReal* __restrict__ d, const int* __restrict__ dir,
   }
 }
 
+template <int D>
+__global__ void __launch_bounds__(256)
+k_expand_grid(const double* __restrict__ ax0, const double* __restrict__ ax1,
+              const double* __restrict__ ax2, int64_t s1, int64_t s2, int64_t N,
+              double* __restrict__ pos) {
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (j >= N) return;
+  if constexpr (D == 1) {
+    pos[j] = ax0[j];
+  } else if constexpr (D == 2) {
+    pos[j] = ax0[j / s1];
+    pos[N + j] = ax1[j % s1];
+  } else {
+    const int64_t i0 = j / (s1 * s2), r = j - i0 * (s1 * s2);
+    pos[j] = ax0[i0];
+    pos[N + j] = ax1[r / s2];
+    pos[2 * N + j] = ax2[r % s2];
+  }
+}
+
 }  // namespace
 
-FactoredPlan plan_factored(int dim, int64_t M, const int64_t* shape, int precision) {
+hipError_t launch_expand_grid(int dim, const int64_t* shape, const double* const* axes,
+                              double* pos, hipStream_t s) {
+  int64_t N = 1;
+  for (int i = 0; i < dim; ++i) N *= shape[i];
+  if (N <= 0) return hipSuccess;
+  const int64_t s1 = dim > 1 ? shape[1] : 1, s2 = dim > 2 ? shape[2] : 1;
+  const unsigned g = static_cast<unsigned>((N + 255) / 256);
+  if (dim == 1) k_expand_grid<1><<<g, 256, 0, s>>>(axes[0], nullptr, nullptr, s1, s2, N, pos);
+  else if (dim == 2) k_expand_grid<2><<<g, 256, 0, s>>>(axes[0], axes[1], nullptr, s1, s2, N, pos);
+  else k_expand_grid<3><<<g, 256, 0, s>>>(axes[0], axes[1], axes[2], s1, s2, N, pos);
+  return hipGetLastError();
+}
+
+FactoredPlan plan_factored(int dim, int64_t M, const int64_t* shape, int precision, int ncomp) {
   FactoredPlan p{};
   p.dim = dim;
+  p.ncomp = ncomp;
   p.ns = precision_splits(precision);
   p.bf16 = precision_is_bf16(precision);
   p.mp = static_cast<int>(((M > 0 ? M : 1) + 31) / 32 * 32);
@@ -110,6 +144,7 @@ FactoredPlan plan_factored(int dim, int64_t M, const int64_t* shape, int precisi
   }
   p.arows = (p.nrow + kBlockRows - 1) / kBlockRows * kBlockRows;
   p.ncp = (p.ncols + kMaxTileCols - 1) / kMaxTileCols * kMaxTileCols;
+  p.ncpt = p.ncp * ncomp;
   p.row_tiles = (dim == 3 ? p.na : 1) * (p.arows / kBlockRows);
   auto al = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
   size_t off = al(mode_table_bytes(3, M));
@@ -118,7 +153,7 @@ FactoredPlan plan_factored(int dim, int64_t M, const int64_t* shape, int precisi
   p.off_a = off;
   off += al(sizeof(uint32_t) * 16 * static_cast<size_t>(p.ns) * p.nch * p.arows);
   p.off_c = off;
-  off += al(sizeof(float2) * static_cast<size_t>(p.ncp) * p.mp);
+  off += al(sizeof(float2) * static_cast<size_t>(p.ncpt) * p.mp);
   p.bytes = off;
   return p;
 }
@@ -161,20 +196,23 @@ hipError_t launch_factored(const FactoredPlan& p, int64_t M, const double* const
   const int Mi = static_cast<int>(M);
   const bool kr = p.dim == 3;
   // tables (the mode table was written by launch_prep): one launch
-  const int64_t nmax = std::max(std::max(kr ? p.na : 0, p.arows), p.ncp);
+  const int64_t nmax = std::max(std::max(kr ? p.na : 0, p.arows), p.ncpt);
   const dim3 tg(static_cast<unsigned>((nmax + 63) / 64), static_cast<unsigned>(p.nch), 3);
   const double* xa = axes[kr ? 1 : (p.trans ? 1 : 0)];
   const double* xc = axes[kr ? 2 : (p.trans ? 0 : 1)];
   TableArgs ta{modes, Mi,      p.dim,   p.nch, axes[0], xa, xc, p.na, p.nrow,
-               p.arows, p.ncols, p.ncp, S,     A,       C,  p.trans};
+               p.arows, p.ncols, p.ncp, S,     A,       C,  p.trans, p.ncomp};
   hipError_t e = by_format<TablesF>(p.bf16, p.ns, ta, tg, s);
   if (e != hipSuccess) return e;
+  // output: (ncomp, grid) dense; a transposed 2-D plan stores rows with
+  // stride 1 and columns with stride nrow
+  const int64_t npts = (kr ? p.na : 1) * p.nrow * p.ncols;
   GemmArgs ga{A,       p.arows, p.nch, S,     p.na,  C,  p.ncp, p.nrow, p.nb_out, out,
               out_dtype != 1, p.trans ? 1 : p.ncols, p.ncols, p.trans ? p.nrow : 1, scale, kr,
-              accum};
+              accum, p.ncpt, npts};
   const int64_t tile_cols = gemm_tile_cols(p.ns, accum);
-  const dim3 grid(static_cast<unsigned>(p.row_tiles),
-                  static_cast<unsigned>((p.ncols + tile_cols - 1) / tile_cols));
+  const int64_t tpc = (p.ncols + tile_cols - 1) / tile_cols;
+  const dim3 grid(static_cast<unsigned>(p.row_tiles), static_cast<unsigned>(tpc * p.ncomp));
   return by_format<GemmF>(p.bf16, p.ns, ga, grid, s);
 }
 

@@ -27,7 +27,11 @@ inline bool precision_is_bf16(int prec) { return prec == 2 || prec == 3 || prec 
 //   S: axis 0 of a 3-D grid (amplitude and theta folded in): na x mp float2
 //   A: the A axis, split into ns parts: ns x nch x arows x 16 packed pairs
 //      (2-D: amplitude and theta folded in); rows >= the axis length are zero
-//   C: last axis, ncp x mp float2; columns >= ncols are zero
+//   C: last axis, ncpt x mp float2; columns >= ncols of each component are
+//      zero.  The incompressible field (ncomp = dim) has one block of ncp
+//      columns per component i, its entries multiplied by the projector
+//      p_i(k_m) = delta_i0 - k_i k_0 / |k|^2, so the GEMM's columns run over
+//      (component, last-axis index) and its output over (component, grid point).
 // Workspace layout (bytes from the start of the caller's workspace):
 //   [0, off_s)       ModeRec table (M records)
 //   [off_s, off_a)   S (3-D only)
@@ -49,7 +53,9 @@ struct FactoredPlan {
   int64_t arows;     // nrow padded to kBlockRows
   int64_t nb_out;    // 3-D: output rows per axis-0 index (= nrow); 2-D: 0
   int64_t ncols;     // last-axis length
-  int64_t ncp;       // ncols padded to kMaxTileCols
+  int64_t ncp;       // ncols padded to kMaxTileCols (one field component)
+  int ncomp;         // field components: 1, or dim for the incompressible field
+  int64_t ncpt;      // ncomp * ncp: columns of the C table and of the GEMM
   int64_t row_tiles; // 3-D: na * arows / kBlockRows; 2-D: arows / kBlockRows
   int trans;         // 2-D with axis 1 as the A axis (it pads less); the output is stored transposed
   size_t off_s, off_a, off_c, bytes;
@@ -64,7 +70,8 @@ constexpr int gemm_tile_cols(int ns, int accum) {
   return 128;
 }
 
-FactoredPlan plan_factored(int dim, int64_t M, const int64_t* shape, int precision);
+FactoredPlan plan_factored(int dim, int64_t M, const int64_t* shape, int precision,
+                           int ncomp = 1);
 
 // One realisation of an ensemble (device-resident descriptor).
 struct EnsReal {
@@ -134,6 +141,7 @@ struct TableArgs {
   uint32_t* A;
   float2* C;
   int trans;  // 2-D: A from axis 1, C from axis 0
+  int ncomp;  // C components (1, or dim with the projector folded in)
 };
 struct GemmArgs {
   const uint32_t* A;
@@ -152,6 +160,8 @@ struct GemmArgs {
   double scale;
   bool kr;
   int accum;
+  int64_t ncpt;        // columns of the C table: ncomp blocks of ncp
+  int64_t comp_stride; // output elements between field components
 };
 
 // Expects the mode table at the start of `work` (launch_prep, KIND_SCALAR).
@@ -160,6 +170,11 @@ struct GemmArgs {
 hipError_t launch_factored(const FactoredPlan& p, int64_t M, const double* const* axes,
                            void* work, void* out, int out_dtype, double scale, int accum,
                            hipStream_t stream);
+
+// The grid's points as a dense (dim, N) point list, C order: pos[i * N + j]
+// is coordinate i of flat grid index j.
+hipError_t launch_expand_grid(int dim, const int64_t* shape, const double* const* axes,
+                              double* pos, hipStream_t stream);
 
 // Direct kernel on a grid: point coordinates come from the axes (device
 // pointers) by the C-order flat index; nothing is materialised.

@@ -158,7 +158,7 @@ __device__ __forceinline__ void table_eval(const double (&kx)[4], const double (
 template <typename T, int NS>
 __device__ __forceinline__ void table_store(const f2v (&v)[4], int role, int64_t row, int64_t cc,
                                             int g, int nch, int64_t na, int64_t arows,
-                                            int64_t ncp, float2* S, uint32_t* A, float2* C) {
+                                            int64_t ncpt, float2* S, uint32_t* A, float2* C) {
   if (role == 0) {
     auto* d = gmem(reinterpret_cast<f4v*>(S + (cc * na + row) * 16 + 4 * g));
     d[0] = f4v{v[0].x, v[0].y, v[1].x, v[1].y};
@@ -173,15 +173,17 @@ __device__ __forceinline__ void table_store(const f2v (&v)[4], int role, int64_t
           u32x4{o[0][s], o[1][s], o[2][s], o[3][s]};
   } else {
     auto* d = gmem(reinterpret_cast<f4v*>(C));
-    d[(8 * cc + 2 * g) * ncp + row] = f4v{v[0].x, v[0].y, v[1].x, v[1].y};
-    d[(8 * cc + 2 * g + 1) * ncp + row] = f4v{v[2].x, v[2].y, v[3].x, v[3].y};
+    d[(8 * cc + 2 * g) * ncpt + row] = f4v{v[0].x, v[0].y, v[1].x, v[1].y};
+    d[(8 * cc + 2 * g + 1) * ncpt + row] = f4v{v[2].x, v[2].y, v[3].x, v[3].y};
   }
 }
 
 // Four table entries: role 0 -> S (3-D axis 0), 1 -> A (split), 2 -> C; row
 // `row` of that table, modes 16 c + 4 g .. + 3 (g < 4).  The four modes'
 // data are loaded together (index clamped to M - 1), then evaluated and
-// stored.
+// stored.  C has ncomp blocks of ncp rows; with ncomp > 1 block i holds the
+// entries times the projector p_i(k) = delta_i0 - k_i k_0 / |k|^2 (FP64 from
+// the mode's kap = k / 2 pi; k = 0 gives NaN, as field.rs:138).
 template <typename T, int NS>
 __device__ __forceinline__ void table_quad(const ModeRec* __restrict__ modes, int M, int dim,
                                            int nch, const double* __restrict__ xs,
@@ -190,10 +192,11 @@ __device__ __forceinline__ void table_quad(const ModeRec* __restrict__ modes, in
                                            int64_t nrow, int64_t arows, int64_t ncols,
                                            int64_t ncp, float2* __restrict__ S,
                                            uint32_t* __restrict__ A, float2* __restrict__ C,
-                                           int role, int64_t row, int c, int g, int trans) {
+                                           int role, int64_t row, int c, int g, int trans,
+                                           int ncomp) {
   if (c >= nch || M <= 0) return;
   const int m0 = c * 16 + 4 * g;
-  int axis;
+  int axis, comp = 0;
   bool amp, valid;
   double x = 0.0;
   if (role == 0) {
@@ -209,25 +212,37 @@ __device__ __forceinline__ void table_quad(const ModeRec* __restrict__ modes, in
     valid = row < nrow;
     if (valid) x = gmem(xa)[row];
   } else {
-    if (row >= ncp) return;
+    if (row >= ncp * ncomp) return;
+    comp = static_cast<int>(row / ncp);
+    const int64_t col = row - comp * ncp;
     axis = trans ? 0 : dim - 1;
     amp = false;
-    valid = row < ncols;
-    if (valid) x = gmem(xc)[row];
+    valid = col < ncols;
+    if (valid) x = gmem(xc)[col];
   }
   const __attribute__((address_space(1))) ModeRec* gm = gmem(modes);
   double kx[4], th[4];
-  float am[4];
+  float am[4], pr[4];
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
     const int m = min(m0 + e, M - 1);
     kx[e] = gm[m].kap[axis];
     th[e] = gm[m].th;
     am[e] = gm[m].amp[0];
+    pr[e] = 1.0f;
+    if (ncomp > 1 && role == 2) {
+      const double k0 = gm[m].kap[0], k1 = gm[m].kap[1], k2 = gm[m].kap[2];
+      const double kc = comp == 0 ? k0 : (comp == 1 ? k1 : k2);
+      pr[e] = static_cast<float>((comp == 0 ? 1.0 : 0.0) - (kc * k0) / (k0 * k0 + k1 * k1 + k2 * k2));
+    }
   }
   f2v v[4];
   table_eval(kx, th, am, x, amp, valid, m0, M, v);
-  table_store<T, NS>(v, role, row, c, g, nch, na, arows, ncp, S, A, C);
+  if (ncomp > 1 && role == 2) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) v[e] = v[e] * pr[e];
+  }
+  table_store<T, NS>(v, role, row, c, g, nch, na, arows, ncp * ncomp, S, A, C);
 }
 
 // All tables of one call in one launch.  block = 64 rows x 16 modes of one
@@ -239,10 +254,10 @@ k_tables(const ModeRec* __restrict__ modes, int M, int dim, int nch,
          const double* __restrict__ xs, const double* __restrict__ xa,
          const double* __restrict__ xc, int64_t na, int64_t nrow, int64_t arows, int64_t ncols,
          int64_t ncp, float2* __restrict__ S, uint32_t* __restrict__ A, float2* __restrict__ C,
-         int trans) {
+         int trans, int ncomp) {
   table_quad<T, NS>(modes, M, dim, nch, xs, xa, xc, na, nrow, arows, ncols, ncp, S, A, C,
                     blockIdx.z, static_cast<int64_t>(blockIdx.x) * 64 + (threadIdx.x & 63),
-                    blockIdx.y, threadIdx.x >> 6, trans);
+                    blockIdx.y, threadIdx.x >> 6, trans, ncomp);
 }
 
 // ---------------------------------------------------------------------------
@@ -272,9 +287,11 @@ struct TileJob {
   const float2* S;  // 3-D: S table and the block's index a
   int64_t na, a;
   const float2* C;
-  int64_t ncp;
-  int64_t arow0, col0;
-  void* out;
+  int64_t ncp;      // row length of the C table (all components)
+  int64_t arow0;
+  int64_t ccol0;    // first C-table column of the tile
+  int64_t col0;     // its output column (within the tile's field component)
+  void* out;        // the component's output
   bool out_f64;
   int64_t ldo, out_row0, nvalid, ncols;
   double scale;
@@ -345,7 +362,7 @@ __device__ __forceinline__ void gemm_tile(GemmSmem<T, NS, JN>& sm, const TileJob
   // mode pair 8 c + 2 wave + h of column col0 + gcol + 64 g: float4 index
   // (8 c + 2 wave + h) ncp + col0 + gcol + 64 g
   const __attribute__((address_space(1))) f4v* cbase =
-      gmem(reinterpret_cast<const f4v*>(jb.C)) + 2 * wave * jb.ncp + jb.col0 + gcol;
+      gmem(reinterpret_cast<const f4v*>(jb.C)) + 2 * wave * jb.ncp + jb.ccol0 + gcol;
   const int64_t c_pair = jb.ncp;
   const int64_t c_chunk = 8 * jb.ncp;
   // S(a, m) of one chunk: lane l < 16 loads mode l with a vector load (a
@@ -534,20 +551,43 @@ __device__ __forceinline__ void gemm_tile(GemmSmem<T, NS, JN>& sm, const TileJob
     gemm_store<ACC, JN, TR>(static_cast<float*>(jb.out), jb, acc, acc2, wave, hl, lr);
 }
 
-// Tile job of row tile rt, column tile ct of one contraction.
+// Tile job of row tile rt, column tile ct of one contraction.  Column tiles
+// run component-major: tiles_per_comp tiles of 32 JN output columns per field
+// component, component i's C-table columns starting at i ncp and its output
+// at out + i comp_stride.
 template <bool KR, int JN>
 __device__ __forceinline__ TileJob make_job(const uint32_t* A, int64_t arows, int nch,
                                             const float2* S, int64_t na, const float2* C,
-                                            int64_t ncp, int64_t nrow, int64_t nb_out,
+                                            int64_t ncpt, int64_t nrow, int64_t nb_out,
                                             void* out, bool out_f64, int64_t ldo,
                                             int64_t ncols, double scale, int64_t rt,
-                                            int64_t ct, int64_t ldc = 1) {
+                                            int64_t ct, int64_t ldc, int64_t tiles_per_comp,
+                                            int64_t ncp, int64_t comp_stride) {
   const int64_t nbt = arows / kBlockRows;
   const int64_t a = KR ? rt / nbt : 0;
   const int64_t arow0 = (rt - a * nbt) * kBlockRows;
-  return TileJob{A,     arows, nch,  S,   na,      a,           C,
-                 ncp,   arow0, ct * 32 * JN, out, out_f64, ldo,
-                 KR ? a * nb_out + arow0 : arow0, nrow - arow0, ncols, scale, ldc};
+  const int64_t comp = ct / tiles_per_comp;
+  TileJob jb;
+  jb.A = A;
+  jb.arows = arows;
+  jb.nch = nch;
+  jb.S = S;
+  jb.na = na;
+  jb.a = a;
+  jb.C = C;
+  jb.ncp = ncpt;
+  jb.arow0 = arow0;
+  jb.col0 = (ct - comp * tiles_per_comp) * 32 * JN;
+  jb.ccol0 = comp * ncp + jb.col0;
+  jb.out = static_cast<char*>(out) + comp * comp_stride * (out_f64 ? 8 : 4);
+  jb.out_f64 = out_f64;
+  jb.ldo = ldo;
+  jb.out_row0 = KR ? a * nb_out + arow0 : arow0;
+  jb.nvalid = nrow - arow0;
+  jb.ncols = ncols;
+  jb.scale = scale;
+  jb.ldc = ldc;
+  return jb;
 }
 
 template <typename T, int NS, bool KR, int ACC, bool TR>
@@ -556,9 +596,10 @@ __attribute__((amdgpu_waves_per_eu(gemm_waves<ACC>(), gemm_waves<ACC>())))
 k_factored_gemm(GemmArgs g) {
   constexpr int JN = cols_blocks<NS, ACC>();
   __shared__ __attribute__((aligned(16))) GemmSmem<T, NS, JN> sm;
-  const TileJob jb = make_job<KR, JN>(g.A, g.arows, g.nch, g.S, g.na, g.C, g.ncp, g.nrow,
+  const int64_t tpc = (g.ncols + 32 * JN - 1) / (32 * JN);
+  const TileJob jb = make_job<KR, JN>(g.A, g.arows, g.nch, g.S, g.na, g.C, g.ncpt, g.nrow,
                                       g.nb_out, g.out, g.out_f64, g.ldo, g.ncols, g.scale,
-                                      blockIdx.x, blockIdx.y, g.ldc);
+                                      blockIdx.x, blockIdx.y, g.ldc, tpc, g.ncp, g.comp_stride);
   gemm_tile<T, NS, KR, ACC, JN, TR>(sm, jb);
 }
 
@@ -676,7 +717,8 @@ k_ens_gemm(const EnsReal* __restrict__ d, const int* __restrict__ fact,
       // (x, y) output, so the row stride is 1 and the column stride nrow
       const TileJob jb = make_job<KR, JN>(r.A, r.arows, r.mp / 16, r.S, r.na, r.C, r.ncp, r.nrow,
                                           r.nb_out, r.out, out_f64, r.trans ? 1 : r.ncols,
-                                          r.ncols, r.scale, rt, ct, r.trans ? r.nrow : 1);
+                                          r.ncols, r.scale, rt, ct, r.trans ? r.nrow : 1,
+                                          r.col_tiles, r.ncp, 0);
       if (!KR && r.trans) gemm_tile<T, NS, KR, ACC, JN, true>(sm, jb);
       else gemm_tile<T, NS, KR, ACC, JN, false>(sm, jb);
     }
@@ -689,7 +731,7 @@ template <typename T, int NS>
 hipError_t tables_launch(const TableArgs& a, dim3 grid, hipStream_t s) {
   gemm::k_tables<T, NS><<<grid, 256, 0, s>>>(a.modes, a.M, a.dim, a.nch, a.xs, a.xa, a.xc, a.na,
                                               a.nrow, a.arows, a.ncols, a.ncp, a.S, a.A, a.C,
-                                              a.trans);
+                                              a.trans, a.ncomp);
   return hipGetLastError();
 }
 

@@ -16,9 +16,11 @@
 // Box-Muller, n = sqrt(-2 ln(1 - u_even)) (cos, sin)(2 pi u_odd), all FP64:
 //   g_d = n_d (d < dim), z1 = n_4, z2 = n_5, w = n_6.
 // Wave vectors:
-//   Gaussian model:     k = g / l                (k ~ N(0, l^-2 I))
-//   exponential model:  k = g / (l |w|)          (isotropic multivariate t, 1 dof;
-//                       |w| is kept >= DBL_MIN, so k stays finite)
+//   Gaussian model:     k = g / l                (k ~ N(0, l^-2 I): the spectral
+//                       law of the covariance exp(-r^2 / (2 l^2)))
+//   exponential model:  k = g / (l |w|)          (isotropic multivariate t, 1 dof:
+//                       the spectral law of exp(-r / l); |w| is kept >= DBL_MIN,
+//                       so k stays finite)
 #include "gsc_device.h"
 #include "gsc_internal.h"
 

@@ -23,8 +23,9 @@ no CPU fallback.  Deliberate differences from the reference:
 num_threads is accepted with the reference's meaning (worker threads); here
 it bounds the host threads that stage strided inputs.
 
-Extensions: summate_structured (the field on a structured grid, here or
-in gstools_core.grid with its workspace form), gstools_core.ensemble.Ensemble
+Extensions: summate_structured, summate_incompr_structured and
+summate_fourier_structured (the three fields on a structured grid, here or
+in gstools_core.grid with their workspace forms), gstools_core.ensemble.Ensemble
 (many realisations in one graph-captured plan), gstools_core.randmeth
 (RandMeth / IncomprRandMeth caller glue), and the kriging calculators
 calc_field_krige_and_variance / calc_field_krige (krige.rs:24-118).
@@ -40,6 +41,7 @@ from . import _native
 __version__ = "1.1.0"
 
 __all__ = ["summate", "summate_incompr", "summate_fourier", "summate_structured",
+           "summate_incompr_structured", "summate_fourier_structured",
            "calc_field_krige_and_variance", "calc_field_krige", "__version__"]
 
 
@@ -246,3 +248,23 @@ def summate_structured(cov_samples, z1, z2, axes, num_threads=None, **options):
     from .grid import summate_grid
 
     return summate_grid(cov_samples, z1, z2, axes, num_threads, **options)
+
+
+def summate_incompr_structured(cov_samples, z1, z2, axes, num_threads=None, **options):
+    """``summate_incompr`` on the structured grid axes[0] x ... x axes[d-1]
+    (d in 2..3) without expanding it: shape (d, len(axes[0]), ...), C order.
+    Options as for ``summate_structured``."""
+    from .grid import summate_incompr_grid
+
+    return summate_incompr_grid(cov_samples, z1, z2, axes, num_threads, **options)
+
+
+def summate_fourier_structured(spectrum_factor, modes, z1, z2, axes, num_threads=None,
+                               **options):
+    """``summate_fourier`` on the structured grid axes[0] x ... x axes[d-1]
+    without expanding it: the field in C order, shape (len(axes[0]), ...).
+    Options as for ``summate_structured``."""
+    from .grid import summate_fourier_grid
+
+    return summate_fourier_grid(spectrum_factor, modes, z1, z2, axes, num_threads, **options)
+

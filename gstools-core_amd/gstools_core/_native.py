@@ -23,6 +23,10 @@ GSC_ERR_WORKSPACE = 6
 GSC_F64 = 0
 GSC_F32 = 1
 
+GSC_FIELD_SCALAR = 0
+GSC_FIELD_INCOMPR = 1
+GSC_FIELD_FOURIER = 2
+
 GSC_PATH_AUTO = 0
 GSC_PATH_DIRECT = 1
 GSC_PATH_FACTORED = 2
@@ -81,6 +85,11 @@ SIGNATURES = {
                                 _i64p, _vp, _optp]),
     "gsc_summate_grid_dev": (_int, [_int, _i64, _vp, _vp, _vp, _vpp, _i64p, _vp, _vp, _sz, _optp,
                                     _vp]),
+    "gsc_grid_field_workspace_bytes": (_sz, [_int, _int, _i64, _i64p, _optp]),
+    "gsc_summate_grid_field": (_int, [_int, _int, _i64, _vp, _i64, _vp, _i64, _i64, _vp, _i64, _vp,
+                                      _i64, _vpp, _i64p, _i64p, _vp, _optp]),
+    "gsc_summate_grid_field_dev": (_int, [_int, _int, _i64, _vp, _vp, _vp, _vp, _vpp, _i64p, _vp,
+                                          _vp, _sz, _optp, _vp]),
     "gsc_ensemble_create": (_int, [ctypes.POINTER(_vp), _int, _int, _i64p, _i64p,
                                    ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double),
                                    _vp, _vp, _vp, _vp, _vp, _optp]),

@@ -16,9 +16,18 @@ the summation kernels, and structured meshes go through the grid entry point
 noise is drawn on the host from numpy's Philox generator keyed (seed, stream +
 1), so a generator with a fixed seed is reproducible.
 
+Covariance models, as GSTools parametrises them (len_scale l, var s^2):
+  "gaussian"     C(r) = s^2 exp(-pi r^2 / (4 l^2))  (GSTools' Gaussian with its
+                 default rescale sqrt(pi) / 2): k ~ N(0, (pi / 2) l^-2 I), i.e.
+                 the sampler's Gaussian law with length l sqrt(2 / pi);
+  "exponential"  C(r) = s^2 exp(-r / l): k = g / (l |w|), the sampler's
+                 multivariate t law with one degree of freedom.
+rescale=False takes the sampler's laws as they are (Gaussian: C(r) =
+s^2 exp(-r^2 / (2 l^2)), the law the benchmark workloads state).
+
 What is not GSTools': the random streams (GSTools seeds numpy RandomState its
-own way), and the covariance models (only the two spectral laws of the
-sampler).  The distributions, the scaling and the e_1 term are the same.
+own way), and the covariance models (only these two).  The distributions, the
+scaling and the e_1 term are the same.
 """
 from __future__ import annotations
 
@@ -36,12 +45,14 @@ class RandMeth:
     model: "gaussian" or "exponential"; dim: 1..3; var, len_scale, nugget: the
     covariance model's variance, length scale and nugget; mode_no: number of
     modes M; seed: Philox key (None: drawn from numpy's entropy); stream:
-    second key word (independent streams for one seed).
+    second key word (independent streams for one seed); rescale: GSTools'
+    length-scale convention of the model (see the module docstring).
     """
 
     def __init__(self, model: str = "gaussian", dim: int = 2, *, var: float = 1.0,
                  len_scale: float = 1.0, nugget: float = 0.0, mode_no: int = 1000,
-                 seed: Optional[int] = None, stream: int = 0, device: int = -1):
+                 seed: Optional[int] = None, stream: int = 0, device: int = -1,
+                 rescale: bool = True):
         if model not in MODELS:
             raise ValueError(f"model must be one of {sorted(MODELS)}")
         if dim < 1 or dim > 3:
@@ -57,7 +68,16 @@ class RandMeth:
         self.mode_no = int(mode_no)
         self.stream = int(stream)
         self.device = device
+        self.rescale = bool(rescale)
         self.reset_seed(seed)
+
+    @property
+    def sampler_len_scale(self) -> float:
+        """Length scale handed to the sampler: l sqrt(2 / pi) for GSTools'
+        Gaussian model (k ~ N(0, (pi / 2) l^-2 I)), else l."""
+        if self.rescale and self.model == "gaussian":
+            return self.len_scale * float(np.sqrt(2.0 / np.pi))
+        return self.len_scale
 
     def reset_seed(self, seed: Optional[int] = None) -> None:
         """Draws new modes for `seed` (None: a fresh random seed)."""
@@ -65,7 +85,7 @@ class RandMeth:
             seed = int(np.random.SeedSequence().generate_state(1, np.uint64)[0])
         self.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
         self.cov_samples, self.z_1, self.z_2 = sample_modes(
-            self.model, self.dim, self.mode_no, self.len_scale, self.seed, self.stream,
+            self.model, self.dim, self.mode_no, self.sampler_len_scale, self.seed, self.stream,
             self.device)
 
     @property
@@ -128,23 +148,20 @@ class IncomprRandMeth(RandMeth):
     def __call__(self, pos, mesh_type: str = "unstructured", add_nugget: bool = True,
                  **summate_kw) -> np.ndarray:
         """(dim, N) at the points pos, or (dim, n_0, .., n_{dim-1}) on a
-        structured mesh (the mesh is expanded: the incompressible field has no
-        factored form here)."""
+        structured mesh (the grid entry point: the factored path folds the
+        projector into the last axis' table)."""
         from . import summate_incompr
         if mesh_type == "structured":
-            axes = self._axes(pos)
-            shape = tuple(len(a) for a in axes)
-            pts = np.stack([g.ravel() for g in np.meshgrid(*axes, indexing="ij")])
+            field = _grid.summate_incompr_grid(self.cov_samples, self.z_1, self.z_2,
+                                               self._axes(pos), scale=self.mean_u * self.scale,
+                                               device=self.device, **summate_kw)
         elif mesh_type == "unstructured":
-            pts, shape = self._points(pos), None
+            field = summate_incompr(self.cov_samples, self.z_1, self.z_2, self._points(pos),
+                                    scale=self.mean_u * self.scale, device=self.device,
+                                    **summate_kw)
         else:
             raise ValueError("mesh_type must be 'structured' or 'unstructured'")
-        field = summate_incompr(self.cov_samples, self.z_1, self.z_2, pts,
-                                scale=self.mean_u * self.scale, device=self.device,
-                                **summate_kw)
         field[0] += self.mean_u
         if add_nugget:
             field = field + self.get_nugget(field.shape)
-        if shape is not None:
-            field = field.reshape((self.dim,) + shape)
         return field

@@ -5,9 +5,13 @@ density and z1, z2 ~ N(0, 1) on the host, then calls ``summate``
 (gen_benchmark_inputs.py:17-29).  ``sample_modes`` draws the same quantities
 on the GPU from a seed (gsc_sample_modes):
 
-* ``"gaussian"``:    k ~ N(0, len_scale^-2 I);
+* ``"gaussian"``:    k ~ N(0, len_scale^-2 I), the spectral law of the
+  covariance exp(-r^2 / (2 len_scale^2)) (GSTools' Gaussian model, whose
+  default rescale gives exp(-pi r^2 / (4 l^2)), corresponds to
+  len_scale = l sqrt(2 / pi); gstools_core.randmeth applies that);
 * ``"exponential"``: k = g / (len_scale |w|), g ~ N(0, I), w ~ N(0, 1)
-  (isotropic multivariate t with one degree of freedom).
+  (isotropic multivariate t with one degree of freedom), the spectral law of
+  the covariance exp(-r / len_scale).
 
 The random bits are Philox4x64-10 keyed (seed, stream), identical word for word
 to ``numpy.random.Philox(key=np.array([seed, stream], np.uint64)).random_raw()``; mode m uses words

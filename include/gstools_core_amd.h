@@ -239,6 +239,39 @@ int gsc_summate_grid_dev(int dim, int64_t M, const double* cov, const double* z1
                          void* stream);
 
 /*
+ * Structured grids for every field of the reference's summators:
+ *   GSC_FIELD_SCALAR   field::summator (field.rs:37-65), as gsc_summate_grid;
+ *   GSC_FIELD_INCOMPR  field::summator_incompr (field.rs:97-182), dim 2 or 3;
+ *                      `out` holds dim component grids back to back
+ *                      (out[i * prod(shape) + j], j the C-order grid index);
+ *   GSC_FIELD_FOURIER  field::summator_fourier (field.rs:219-249) with the
+ *                      spectrum factors sf (M values, stride sf_s; `cov` holds
+ *                      the modes).
+ * Each equals its point-list summator on the expanded grid.  On the factored
+ * path the incompressible field folds the projector delta_i0 - k_i k_0 / |k|^2
+ * into the last axis' table, so one contraction with dim times the columns
+ * gives all components.  No counterpart in the reference (its callers expand
+ * structured meshes into point lists).  sf is ignored unless FOURIER.
+ */
+typedef enum gsc_field {
+  GSC_FIELD_SCALAR = 0,
+  GSC_FIELD_INCOMPR = 1,
+  GSC_FIELD_FOURIER = 2
+} gsc_field;
+size_t gsc_grid_field_workspace_bytes(int field, int dim, int64_t M, const int64_t* shape,
+                                      const gsc_opts* opts);
+int gsc_summate_grid_field(int field, int dim, int64_t M, const double* sf, int64_t sf_s,
+                           const double* cov, int64_t cov_sd, int64_t cov_sm,
+                           const double* z1, int64_t z1_s, const double* z2, int64_t z2_s,
+                           const double* const* axes, const int64_t* axis_s,
+                           const int64_t* shape, void* out, const gsc_opts* opts);
+int gsc_summate_grid_field_dev(int field, int dim, int64_t M, const double* sf,
+                               const double* cov, const double* z1, const double* z2,
+                               const double* const* axes, const int64_t* shape, void* out,
+                               void* work, size_t work_bytes, const gsc_opts* opts,
+                               void* stream);
+
+/*
  * Ensembles of realisations on structured grids (no counterpart in the
  * reference, whose callers loop over realisations in Python and call summate
  * once per field).  `count` realisations of one dimension dim in {2, 3};

@@ -6,8 +6,9 @@ tests can check the device draws value by value.  Only ``tests/`` may import it.
 The reference has no sampler: GSTools' RandMeth (the caller, referenced at
 gen_benchmark_inputs.py:17-29 and not part of /root/reference) draws the modes
 on the host.  The distributions follow the benchmark's rules: k ~ N(0, l^-2 I)
-(Gaussian model) and k = g / (l |w|) (exponential model, isotropic
-multivariate t with one degree of freedom), z1, z2 ~ N(0, 1).
+(Gaussian model, the spectral law of the covariance exp(-r^2 / (2 l^2))) and
+k = g / (l |w|) (exponential model, isotropic multivariate t with one degree
+of freedom, the spectral law of exp(-r / l)), z1, z2 ~ N(0, 1).
 
 What pins each part:
 * the random words: ``numpy.random.Philox`` is Philox4x64-10 (Random123); the

@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 import oracle
+from conftest import sigma_tol
 import gstools_core
 from gstools_core import grid, workloads
 
@@ -160,3 +161,92 @@ def test_c4_realisations_sample():
         err = _scaled_err(g.ravel()[idx], ref, md.m)
         print(f"{w.name} {g.shape}: {err:.3e}")
         assert err <= 1e-5
+
+
+# -- the incompressible and Fourier fields on grids ---------------------------------
+
+@pytest.mark.parametrize("shape", [(37, 50), (300, 40), (3, 300), (5, 7, 200), (33, 17, 129),
+                                   (16, 16, 16)])
+@pytest.mark.parametrize("precision", ["fp16x3", "bf16x6"])
+def test_incompr_grid_factored(shape, precision):
+    d = len(shape)
+    cov, z1, z2 = _modes(d, 150, 4.0, 2000 + sum(shape))
+    axes = [np.arange(n) * 0.75 - 3.0 for n in shape]
+    g = grid.summate_incompr_grid(cov, z1, z2, axes, path="factored", precision=precision)
+    assert g.shape == (d,) + shape
+    ref = oracle.summate_incompr(cov, z1, z2, oracle.grid_points(axes))
+    err = _scaled_err(g.reshape(d, -1), ref, 150)
+    print(f"incompr {shape} {precision}: {err:.3e}")
+    assert err <= TOL[precision]
+
+
+@pytest.mark.parametrize("shape", [(37, 50), (5, 7, 200), (2, 3, 4)])
+@pytest.mark.parametrize("variant", ["valu", "local"])
+def test_incompr_grid_direct(shape, variant):
+    d = len(shape)
+    cov, z1, z2 = _modes(d, 200, 5.0, 3000 + sum(shape))
+    axes = [np.linspace(-10, 40, n) for n in shape]
+    g = grid.summate_incompr_grid(cov, z1, z2, axes, path="direct", variant=variant,
+                                  out_dtype=np.float32, scale=0.5)
+    assert g.shape == (d,) + shape and g.dtype == np.float32
+    pts = oracle.grid_points(axes)
+    p = gstools_core.summate_incompr(cov, z1, z2, pts, variant=variant, scale=0.5,
+                                     out_dtype=np.float32)
+    np.testing.assert_array_equal(g.reshape(d, -1), p)  # same kernels on the expanded grid
+    ref = oracle.summate_incompr(cov, z1, z2, pts)
+    assert _scaled_err(g.reshape(d, -1) / 0.5, ref, 200) <= 1e-5
+
+
+def test_incompr_grid_is_divergence_free_and_k0_nan():
+    # the factored path keeps sum_i d_i u_i = 0 (central differences on a fine
+    # mesh) and, like the reference (field.rs:138), gives NaN for k = 0
+    cov, z1, z2 = _modes(3, 300, 3.0, 41)
+    h = 0.05
+    axes = [np.arange(32) * h + 5.0 for _ in range(3)]
+    g = grid.summate_incompr_grid(cov, z1, z2, axes, path="factored")
+    div = ((g[0, 2:, 1:-1, 1:-1] - g[0, :-2, 1:-1, 1:-1]) +
+           (g[1, 1:-1, 2:, 1:-1] - g[1, 1:-1, :-2, 1:-1]) +
+           (g[2, 1:-1, 1:-1, 2:] - g[2, 1:-1, 1:-1, :-2])) / (2 * h)
+    grad = np.abs(g[0, 2:] - g[0, :-2]).max() / (2 * h)
+    assert np.abs(div).max() < 2e-3 * grad
+    cov0 = cov.copy()
+    cov0[:, 7] = 0.0
+    g0 = grid.summate_incompr_grid(cov0, z1, z2, [np.arange(20.0)] * 2 + [np.arange(20.0)],
+                                   path="factored")
+    assert np.isnan(g0).all()
+
+
+@pytest.mark.parametrize("shape", [(37, 50), (3, 300), (5, 7, 200), (1000,)])
+def test_fourier_grid(shape):
+    d = len(shape)
+    rng = np.random.default_rng(sum(shape))
+    side = 6
+    ax = (np.arange(side) - (side - 1) / 2.0) * (2 * np.pi / 60.0)
+    kg = np.stack(np.meshgrid(*([ax] * d), indexing="ij")).reshape(d, -1)
+    m = kg.shape[1]
+    sf = np.exp(-(kg ** 2).sum(axis=0) * 25.0)
+    z1, z2 = rng.normal(size=m), rng.normal(size=m)
+    axes = [np.arange(n) * 0.9 for n in shape]
+    pts = oracle.grid_points(axes)
+    ref = oracle.summate_fourier(sf, kg, z1, z2, pts)
+    for path in ("auto", "direct", "factored") if d > 1 else ("auto", "direct"):
+        g = gstools_core.summate_fourier_structured(sf, kg, z1, z2, axes, path=path)
+        assert g.shape == shape
+        err = float(np.abs(g.ravel() - ref).max())
+        print(f"fourier {shape} {path}: {err / sf.max() / np.sqrt(m):.3e}")
+        assert err <= sigma_tol(m) * sf.max()
+
+
+def test_c3_incompr_grid_256():
+    # the 256^3 configuration as an incompressible field: factored (projector
+    # folded into the last axis' table) against the FP64 oracle on 10^6 points
+    w = workloads.c3()
+    md = w.modes
+    idx = workloads.subset_indices(w.n_points, 1_000_000, seed=13)
+    pts = workloads.grid_points(w.axes, idx)
+    ref = oracle.summate_incompr(md.cov_samples, md.z1, md.z2, pts)
+    g = grid.summate_incompr_grid(md.cov_samples, md.z1, md.z2, w.axes, path="factored",
+                                  precision="fp16x3", out_dtype=np.float32)
+    err = _scaled_err(g.reshape(3, -1)[:, idx], ref, md.m)
+    print(f"c3 incompr factored fp16x3: max|err|/sigma = {err:.3e} on {idx.size} points")
+    assert err <= 1e-5

@@ -106,3 +106,40 @@ def test_bad_arguments():
         rm([np.arange(3.0)], mesh_type="structured")
     with pytest.raises(ValueError):
         rm(np.zeros((2, 5)), mesh_type="hexagonal")
+
+
+@pytest.mark.parametrize("model,rescale", [("gaussian", True), ("gaussian", False),
+                                           ("exponential", True)])
+def test_covariance_of_the_drawn_modes(model, rescale):
+    # E[cos(k . r)] over the spectral law is the model's correlation at lag r:
+    # GSTools' Gaussian exp(-pi r^2 / (4 l^2)) (rescale), the sampler's
+    # exp(-r^2 / (2 l^2)) (no rescale), the exponential exp(-r / l); averaged
+    # over M = 20000 drawn modes, along each axis
+    ls = 10.0
+    rm = RandMeth(model, 3, len_scale=ls, mode_no=20_000, seed=77, rescale=rescale)
+    for r in (0.25 * ls, ls, 2.0 * ls):
+        if model == "exponential":
+            want = np.exp(-r / ls)
+        elif rescale:
+            want = np.exp(-np.pi * r * r / (4.0 * ls * ls))
+        else:
+            want = np.exp(-r * r / (2.0 * ls * ls))
+        for axis in range(3):
+            got = float(np.mean(np.cos(rm.cov_samples[axis] * r)))
+            assert abs(got - want) < 0.03, (model, rescale, r, axis, got, want)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_incompressible_structured_mesh_uses_the_grid_path(dim):
+    rm = IncomprRandMeth("gaussian", dim, mean_velocity=1.5, len_scale=6.0, mode_no=400,
+                         seed=90 + dim)
+    axes = [np.arange(n) * 0.7 for n in ((50, 37) if dim == 2 else (20, 24, 33))]
+    f = rm(axes, mesh_type="structured")
+    assert f.shape == (dim,) + tuple(len(a) for a in axes)
+    g = grid.summate_incompr_grid(rm.cov_samples, rm.z_1, rm.z_2, axes,
+                                  scale=rm.mean_u * rm.scale)
+    g[0] += rm.mean_u
+    np.testing.assert_array_equal(f, g)
+    pts = np.stack([a.ravel() for a in np.meshgrid(*axes, indexing="ij")])
+    u = rm(pts)
+    assert np.abs(f.reshape(dim, -1) - u).max() <= 1e-5 * rm.mean_u
