@@ -73,6 +73,23 @@ def cpu_baseline(w, n_sample: int) -> dict:
     }
 
 
+def sine_leg(repeats: int = 10) -> dict:
+    """Calibrated v_sin_f32 issue rate of this device (gsc_calibrate_sine):
+    the measured peak of the direct kernels' roofline, beside the rated one
+    (256 CUs x 4 SIMDs x 64 lanes / 8 cycles per wave64 sine x 2.4 GHz)."""
+    import ctypes
+
+    from gstools_core import _native
+
+    a, b = ctypes.c_double(), ctypes.c_double()
+    _native.check(_native.lib().gsc_calibrate_sine(int(repeats), ctypes.byref(a),
+                                                   ctypes.byref(b)))
+    return {"sin_per_s": a.value, "sin_fma_pairs_per_s": b.value,
+            "rated_sin_per_s": TRANS_PEAK, "frac_of_rated": a.value / TRANS_PEAK,
+            "kernel": "k_sine_rate: 2048 blocks x 256 threads, 8 independent chains x 4096 "
+                      "iterations per lane"}
+
+
 def time_steps(step, steps: int, warmup: int, stream):
     """Per-step times (ms) of `step` on `stream`, HIP events around each step."""
     import torch
@@ -109,22 +126,28 @@ def grid_leg(steps: int, warmup: int) -> dict:
     stream = torch.cuda.current_stream()
     res = {"workload": f"3-D grid {shape[0]}x{shape[1]}x{shape[2]}, M={md.m} Gaussian (l=16), "
                        "FP32 output", "n_points": n, "n_modes": md.m}
-    for path, prec, acc in (("factored", "fp16x3", "f32x2"), ("factored", "fp16x3", "f32"),
-                            ("factored", "fp16x3", "f64"), ("factored", "bf16x6", "f32x2"),
-                            ("factored", "fp16x1", "f32"), ("direct", "auto", "f32x2")):
-        nb = grid.grid_workspace_bytes(3, md.m, shape, path=path, precision=prec)
+    for path, prec, acc, tile in (("factored", "fp16x3", "f32x2", "256"),
+                                  ("factored", "fp16x3", "f32x2", "128"),
+                                  ("factored", "fp16x3", "f32", "256"),
+                                  ("factored", "fp16x3", "f64", "256"),
+                                  ("factored", "bf16x6", "f32x2", "256"),
+                                  ("factored", "bf16x6", "f32x2", "128"),
+                                  ("factored", "fp16x1", "f32", "256"),
+                                  ("direct", "auto", "f32x2", "256")):
+        nb = grid.grid_workspace_bytes(3, md.m, shape, path=path, precision=prec, tile=tile)
         work = torch.empty(nb, dtype=torch.uint8, device=dev)
 
         def step():
             grid.summate_grid_dev(3, md.m, cov.data_ptr(), z1.data_ptr(), z2.data_ptr(),
                                   [a.data_ptr() for a in axes], shape, out.data_ptr(),
                                   work.data_ptr(), nb, path=path, precision=prec, out_f32=True,
-                                  stream=stream.cuda_stream, accum=acc)
+                                  stream=stream.cuda_stream, accum=acc, tile=tile)
 
         st = steps if path == "factored" else max(3, steps // 4)
         per, total = time_steps(step, st, warmup, stream)
         ms = total / st
-        key = f"{path}_{prec}_{acc}" if path == "factored" else "direct"
+        key = (f"{path}_{prec}_{acc}" + ("_tile128" if tile == "128" else "")
+               if path == "factored" else "direct")
         res[key] = {"ms_per_step": ms, "point_modes_per_s": n * md.m / (ms * 1e-3)}
         if path == "factored":
             nprod = {"fp16x3": 3, "bf16x6": 6, "fp16x1": 1}[prec]
@@ -162,21 +185,23 @@ def c5_leg(steps: int, warmup: int) -> dict:
     res = {"workload": f"3-D grid {shape[0]}x{shape[1]}x{shape[2]}, M={md.m} exponential-model "
                        "modes (multivariate t, 1 dof, l=8), FP32 output",
            "n_points": n, "n_modes": md.m}
-    for path, prec in (("factored", "fp16x3"), ("factored", "bf16x3"), ("factored", "bf16x6"),
-                       ("direct", "auto")):
-        nb = grid.grid_workspace_bytes(3, md.m, shape, path=path, precision=prec)
+    for path, prec, tile in (("factored", "fp16x3", "256"), ("factored", "fp16x3", "128"),
+                             ("factored", "bf16x3", "256"), ("factored", "bf16x6", "256"),
+                             ("factored", "bf16x6", "128"), ("direct", "auto", "256")):
+        nb = grid.grid_workspace_bytes(3, md.m, shape, path=path, precision=prec, tile=tile)
         work = torch.empty(nb, dtype=torch.uint8, device=dev)
 
         def step():
             grid.summate_grid_dev(3, md.m, cov.data_ptr(), z1.data_ptr(), z2.data_ptr(),
                                   [a.data_ptr() for a in axes], shape, out.data_ptr(),
                                   work.data_ptr(), nb, path=path, precision=prec, out_f32=True,
-                                  stream=stream.cuda_stream, accum="f32x2")
+                                  stream=stream.cuda_stream, accum="f32x2", tile=tile)
 
         st = steps if path == "factored" else 2
         _, total = time_steps(step, st, warmup, stream)
         ms = total / st
-        key = f"factored_{prec}" if path == "factored" else "direct"
+        key = (f"factored_{prec}" + ("_tile128" if tile == "128" else "")
+               if path == "factored" else "direct")
         res[key] = {"ms_per_step": ms, "point_modes_per_s": n * md.m / (ms * 1e-3)}
         if path == "factored":
             nprod = {"fp16x3": 3, "bf16x3": 3, "bf16x6": 6}[prec]
@@ -201,32 +226,49 @@ def ensemble_leg(steps: int, warmup: int) -> dict:
     ens_w = workloads.c4()
     reals = [(w.modes.cov_samples, w.modes.z1, w.modes.z2, w.axes) for w in ens_w]
     scales = [w.modes.scale for w in ens_w]
-    t0 = time.perf_counter()
-    tuner = Tuner(path=os.path.join("/tmp", "gsc_tuning.json"))
-    paths = [tuner.choose(2, tuple(len(a) for a in w.axes), w.modes.m, "fp16x3", "f32x2")
-             for w in ens_w]
-    t_tune = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    ens = Ensemble(reals, paths=paths, scales=scales, out_dtype=np.float32, precision="fp16x3")
-    t_plan = time.perf_counter() - t0
     stream = torch.cuda.current_stream()
+    # the library rule (every realisation factored, fp16x3, 256-row tiles)
+    t0 = time.perf_counter()
+    ens = Ensemble(reals, scales=scales, out_dtype=np.float32, precision="fp16x3")
+    t_plan = time.perf_counter() - t0
     per, total = time_steps(lambda: ens.run(stream.cuda_stream), steps, warmup, stream)
     ms = total / steps
     info = ens.info
+    ens.close()
+    del ens
+    # the autotuner's plan: path and tile shape per realisation, split format
+    # and accumulation per plan, from per-class timings (cached)
+    t0 = time.perf_counter()
+    tuner = Tuner(path=os.path.join("/tmp", "gsc_tuning.json"))
+    tuned = Ensemble(reals, scales=scales, out_dtype=np.float32, tune=True, tuner=tuner)
+    t_tune = time.perf_counter() - t0
+    _, total_t = time_steps(lambda: tuned.run(stream.cuda_stream), steps, warmup, stream)
+    ms_t = total_t / steps
+    tinfo = tuned.info
+    choice = tuned.tuned
+    tuned.close()
+    del tuned
+    from collections import Counter
+    tuned_res = {"ms_per_step": ms_t, "point_modes_per_s": tinfo["point_modes"] / (ms_t * 1e-3),
+                 "workspace_gb": tinfo["workspace_bytes"] / 1e9, "tune_and_plan_s": t_tune,
+                 "classes_measured": tuner.measured, "precision": choice["precision"],
+                 "accum": choice["accum"],
+                 "choices": dict(Counter(f"{p}/{t}" if p == "factored" else p
+                                         for p, t in zip(choice["paths"], choice["tiles"])))}
     res = {"workload": "512 2-D grids, sides uniform in {64..1024}, M=1000 Gaussian modes "
                        "(l uniform in [5,50]) per realisation, FP32 output, fp16x3",
            "point_modes": info["point_modes"], "factored": info["factored"],
            "direct": info["direct"], "workspace_gb": info["workspace_bytes"] / 1e9,
-           "tune_s": t_tune, "plan_s": t_plan, "ms_per_step": ms,
-           "point_modes_per_s": info["point_modes"] / (ms * 1e-3)}
-    ens.close()
-    del ens
+           "plan_s": t_plan, "ms_per_step": ms,
+           "point_modes_per_s": info["point_modes"] / (ms * 1e-3), "tuned": tuned_res}
     # the same grids with their modes drawn on the device from per-realisation
     # seeds inside the graph (sampler + tables + GEMM per replay)
     seeded = Ensemble.seeded("gaussian", [w.axes for w in ens_w], [w.modes.m for w in ens_w],
                              [w.modes.len_scale for w in ens_w],
-                             [19031977 + r for r in range(len(ens_w))], paths=paths,
-                             scales=scales, out_dtype=np.float32, precision="fp16x3")
+                             [19031977 + r for r in range(len(ens_w))], scales=scales,
+                             out_dtype=np.float32, precision=choice["precision"],
+                             accum=choice["accum"], paths=choice["paths"],
+                             tiles=choice["tiles"])
     _, total_s = time_steps(lambda: seeded.run(stream.cuda_stream), steps, warmup, stream)
     ms_s = total_s / steps
     res["seeded_on_device"] = {"ms_per_step": ms_s,
@@ -399,9 +441,9 @@ def main() -> int:
     ap.add_argument("--m", type=int, default=1000)
     ap.add_argument("--no-grid", action="store_true", help="skip the structured-grid leg")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble leg")
-    ap.add_argument("--legs", default="direct,incompr,fourier,grid,c5,ensemble,krige",
-                    help="comma list of legs: direct, incompr, fourier, grid, c5, ensemble, "
-                         "krige, copy "
+    ap.add_argument("--legs", default="sine,direct,incompr,fourier,grid,c5,ensemble,krige",
+                    help="comma list of legs: sine (v_sin_f32 issue-rate calibration), direct, "
+                         "incompr, fourier, grid, c5, ensemble, krige, copy "
                          "(a 1 GiB device copy, the reference for HBM byte counters)")
     args = ap.parse_args()
     legs = set(args.legs.split(","))
@@ -430,8 +472,13 @@ def main() -> int:
               flush=True)
         if legs == {"copy"}:
             return 0
+    sine = None
+    if "sine" in legs:
+        sine = sine_leg()
+        log(f"sine calibration: {sine['sin_per_s'] / 1e12:.2f} T/s "
+            f"({sine['frac_of_rated']:.3f} of rated)")
     if "direct" not in legs:
-        res = {}
+        res = {"sine_calibration": sine} if sine else {}
         if "incompr" in legs:
             res["incompr"] = incompr_leg(max(3, args.steps // 2), 1)
         if "fourier" in legs:
@@ -487,9 +534,22 @@ def main() -> int:
     log(f"ms/step {ms_per_step:.3f} (min {min(per):.3f}, wall {wall * 1e3 / args.steps:.3f})")
 
     # roofline: one hardware sine per point-mode (amplitude-phase form), i.e.
-    # N*M transcendentals per summation; the measured time is the whole
-    # device call (cell sort, mode split and k_direct_local, all on `stream`)
-    avg_launch_s = ms_per_step * 1e-3
+    # N*M transcendentals per launch of the summation kernel
+    # (k_direct_local32), whose launches are timed here with HIP events on
+    # `stream` (gsc_direct_phase_times: an event between the cell sort and the
+    # kernel); the whole device call gives the step's fraction beside it
+    import ctypes
+
+    from gstools_core import _native
+
+    ph = (ctypes.c_double * 3)()
+    _native.check(_native.lib().gsc_direct_phase_times(
+        _native.GSC_FIELD_SCALAR, d, m, n, None, cov.data_ptr(), z1.data_ptr(), z2.data_ptr(),
+        pos.data_ptr(), n, out.data_ptr(), n, work.data_ptr(), nb,
+        ctypes.byref(_native.make_opts()), ctypes.c_void_p(sh), args.steps, ph))
+    sort_ms, kernel_ms, call_ms = ph[0], ph[1], ph[2]
+    log(f"phases: sort {sort_ms:.3f} ms, kernel {kernel_ms:.3f} ms, call {call_ms:.3f} ms")
+    avg_launch_s = kernel_ms * 1e-3
     achieved = n * m / avg_launch_s / 1e12
     traffic = None  # HBM bytes per step, measured by rocprofv3 --pmc (profiles/)
     try:
@@ -499,6 +559,7 @@ def main() -> int:
             traffic = t["fetch_bytes"] + t["write_bytes"]
     except (OSError, KeyError, ValueError):
         pass
+    step_tsin = n * m / (ms_per_step * 1e-3) / 1e12
     roofline = {
         "bound": "valu-transcendental",
         "achieved": round(achieved, 4),
@@ -506,12 +567,21 @@ def main() -> int:
         "unit": "Tsin/s",
         "frac": round(achieved / (TRANS_PEAK / 1e12), 4),
         "traffic": traffic,
-        "kernel": "gsc_summate_dev: cell sort + k_direct_local32<3,double>",
+        "kernel": "k_direct_local32<3,double> (the summation kernel of gsc_summate_dev)",
+        "kernel_ms": round(kernel_ms, 4),
         "work_per_launch": f"{n}*{m} point-modes, 1 v_sin_f32 + 1 v_fma_f32 each on the VALU; "
                            "phases on v_mfma_f32_32x32x16_f16 (1024 point-modes per instruction)",
+        "step": {"ms": round(ms_per_step, 4), "sort_ms": round(sort_ms, 4),
+                 "achieved": round(step_tsin, 4),
+                 "frac": round(step_tsin / (TRANS_PEAK / 1e12), 4)},
         "traffic_note": "bytes per step (whole device call) from FETCH_SIZE x2 + WRITE_SIZE x1 "
                         "(copy-calibrated); algorithmic 320 MB",
     }
+    if sine is not None:
+        cal = sine["sin_per_s"] / 1e12
+        roofline["calibrated_peak"] = round(cal, 4)
+        roofline["frac_of_calibrated"] = round(achieved / cal, 4)
+        roofline["step"]["frac_of_calibrated"] = round(step_tsin / cal, 4)
 
     inc_res = None
     if "incompr" in legs:
@@ -568,6 +638,16 @@ def main() -> int:
     h_ms = (time.perf_counter() - t_h) * 1e3 / calls
     host_arrays = {"ms_per_call": h_ms, "point_modes_per_s": n * m / (h_ms * 1e-3),
                    "note": "gstools_core.summate on numpy arrays, wall clock, copies included"}
+    ni = 4_000_000
+    host_pos_i = np.ascontiguousarray(w.pos[:, :ni])
+    gstools_core.summate_incompr(md.cov_samples, md.z1, md.z2, host_pos_i)
+    t_h = time.perf_counter()
+    for _ in range(calls):
+        gstools_core.summate_incompr(md.cov_samples, md.z1, md.z2, host_pos_i)
+    hi_ms = (time.perf_counter() - t_h) * 1e3 / calls
+    host_arrays["incompr_4e6"] = {"ms_per_call": hi_ms,
+                                  "point_modes_per_s": ni * m / (hi_ms * 1e-3),
+                                  "note": "summate_incompr, (3, N) Fortran-order result"}
 
     cpu = None
     if args.cpu_sample > 0:
@@ -591,6 +671,7 @@ def main() -> int:
                                f"M={m} Gaussian modes (l=20), seed 19031977",
                    "path": "direct", "n_points": n, "n_modes": m, "dim": d},
         "roofline": roofline,
+        "sine_calibration": sine,
         "cpu_baseline": cpu,
         "direct_variants": direct_variants,
         "host_arrays": host_arrays,

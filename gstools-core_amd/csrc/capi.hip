@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "gsc_factored.h"
+#include "gsc_host.h"
 #include "gsc_internal.h"
 #include "gstools_core_amd.h"
 
@@ -53,7 +54,12 @@ struct DeviceCtx {
   hipStream_t stream = nullptr;
   void* buf = nullptr;
   size_t cap = 0;
+  gsc::Staging stage;  // pinned chunks of the host <-> device copies
 };
+
+// Host arrays of at least this many bytes go through the pinned chunks; below
+// it, one pageable copy costs less than the pipeline's set-up.
+constexpr size_t kStageMin = size_t(4) << 20;
 constexpr int kMaxDevices = 64;
 DeviceCtx g_dev[kMaxDevices];
 
@@ -108,6 +114,9 @@ int validate_common(int dim, int64_t M, int64_t N, const double* cov, const doub
   return GSC_OK;
 }
 
+// GEMM tile shape of the factored path (GSC_TILE_AUTO: 256-row blocks).
+int grid_tile(const gsc_opts* o) { return o->tile == GSC_TILE_AUTO ? GSC_TILE_256 : o->tile; }
+
 int validate_opts(const gsc_opts* o) {
   if (o->out_dtype != GSC_F64 && o->out_dtype != GSC_F32)
     return fail(GSC_ERR_INVALID, "out_dtype must be GSC_F64 or GSC_F32");
@@ -115,31 +124,27 @@ int validate_opts(const gsc_opts* o) {
     return fail(GSC_ERR_INVALID, "unknown direct_variant %d", o->direct_variant);
   if (o->blocks_per_cu < 0 || o->blocks_per_cu > 64)
     return fail(GSC_ERR_INVALID, "blocks_per_cu must be in 0..64");
+  if (o->tile < GSC_TILE_AUTO || o->tile > GSC_TILE_128)
+    return fail(GSC_ERR_INVALID, "unknown tile shape %d", o->tile);
   return GSC_OK;
 }
 
-// Gathers a strided 2-D host array (rows, cols) into dense row-major `dst`.
+// Gathers a strided 2-D host array (rows, cols) into dense row-major `dst`,
+// in column ranges on the host pool (nthreads: 0 = all workers).
 void gather2(const double* src, int64_t rows, int64_t cols, int64_t sr, int64_t sc, double* dst,
              int nthreads) {
   if (sc == 1 && sr == cols) {
     std::memcpy(dst, src, sizeof(double) * rows * cols);
     return;
   }
-  auto body = [&](int64_t c0, int64_t c1) {
+  constexpr int64_t kCols = int64_t(1) << 16;
+  const int64_t pieces = rows * cols < (1 << 20) ? 1 : (cols + kCols - 1) / kCols;
+  const int64_t step = (cols + pieces - 1) / pieces;
+  gsc::parallel_for(pieces, nthreads, [&](int64_t p) {
+    const int64_t c0 = p * step, c1 = std::min(cols, c0 + step);
     for (int64_t r = 0; r < rows; ++r)
       for (int64_t c = c0; c < c1; ++c) dst[r * cols + c] = src[r * sr + c * sc];
-  };
-  const int64_t work = rows * cols;
-  int nt = nthreads > 0 ? nthreads : (int)std::max(1u, std::thread::hardware_concurrency());
-  if (work < (1 << 20) || nt <= 1) {
-    body(0, cols);
-    return;
-  }
-  nt = (int)std::min<int64_t>(nt, 64);
-  std::vector<std::thread> th;
-  for (int t = 0; t < nt; ++t)
-    th.emplace_back(body, cols * t / nt, cols * (t + 1) / nt);
-  for (auto& t : th) t.join();
+  });
 }
 
 void gather1(const double* src, int64_t n, int64_t s, double* dst) {
@@ -169,13 +174,20 @@ size_t direct_work_bytes(int kind, int dim, int64_t M, int64_t N, const gsc_opts
 
 // Launches the direct kernel of `kind` on device buffers; `work` holds the
 // mode table (written by launch_prep) followed by the binned kernel's area.
+// split: null, or an event recorded before the summation kernel (after the
+// cell sort of the LOCAL variant).
 hipError_t launch_direct(int kind, int dim, int64_t M, int64_t N, void* work, const double* pos,
                          int64_t pos_ld, void* out, int64_t out_ld, const gsc_opts* o,
-                         hipStream_t s) {
+                         hipStream_t s, hipEvent_t split = nullptr) {
   if (use_binned(kind, dim, N, o)) {
     void* bin = static_cast<char*>(work) + align256(gsc::mode_table_bytes(dim, M));
     return gsc::launch_direct_binned(dim, M, N, work, pos, pos_ld, out, out_ld,
-                                     kind == gsc::KIND_INCOMPR, o->out_dtype, o->scale, bin, s);
+                                     kind == gsc::KIND_INCOMPR, o->out_dtype, o->scale, bin, s,
+                                     split);
+  }
+  if (split) {
+    const hipError_t e = hipEventRecord(split, s);
+    if (e != hipSuccess) return e;
   }
   if (kind == gsc::KIND_INCOMPR) {
     const int variant = o->direct_variant == GSC_DIRECT_MFMA ? GSC_DIRECT_MFMA : GSC_DIRECT_VALU;
@@ -205,8 +217,13 @@ int host_summate(int kind, int dim, int64_t M, int64_t N, const double* sf, int6
 
   const int ncomp = kind == gsc::KIND_INCOMPR ? dim : 1;
   const size_t elt = o->out_dtype == GSC_F32 ? 4 : 8;
+  // result view: (ncomp, N) with strides (out_sd, out_sp)
+  const int64_t o_sd = kind == gsc::KIND_INCOMPR ? out_sd : N;
+  // the reference's Fortran-order (d, N) result (field.rs:166-174) is the
+  // device's (d, N) rows transposed: done on the device, then one flat copy
+  const bool out_fortran = kind == gsc::KIND_INCOMPR && out_sd == 1 && out_sp == ncomp;
   try {
-    // Dense host staging of the (small) mode arrays, and of pos when strided.
+    // Dense host staging of the (small) mode arrays
     const int64_t Mm = std::max<int64_t>(M, 1);
     std::vector<double> modes((size_t)(dim * Mm + 3 * Mm), 0.0);
     if (M > 0) {
@@ -215,15 +232,6 @@ int host_summate(int kind, int dim, int64_t M, int64_t N, const double* sf, int6
       gather1(z2, M, z2_s, modes.data() + (dim + 1) * M);
       if (kind == gsc::KIND_FOURIER) gather1(sf, M, sf_s, modes.data() + (dim + 2) * M);
     }
-    std::vector<double> pos_dense;
-    const double* pos_src = pos;
-    if (!(pos_sp == 1 && pos_sd == N)) {
-      pos_dense.resize((size_t)(dim * N));
-      gather2(pos, dim, N, pos_sd, pos_sp, pos_dense.data(), o->num_threads);
-      pos_src = pos_dense.data();
-    }
-    const bool out_dense = kind == gsc::KIND_INCOMPR ? (out_sp == 1 && out_sd == N) : out_sp == 1;
-
     int dev = 0;
     if ((rc = resolve_device(o, &dev))) return rc;
     DeviceCtx& c = g_dev[dev];
@@ -231,12 +239,16 @@ int host_summate(int kind, int dim, int64_t M, int64_t N, const double* sf, int6
     GSC_HIP(hipSetDevice(dev));
     if ((rc = check_arch(dev))) return rc;
     if (!c.stream) GSC_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    const bool staged_in = sizeof(double) * (size_t)dim * (size_t)N >= kStageMin &&
+                           gsc::staging_ready(c.stage);
+    const bool staged_out = elt * (size_t)ncomp * (size_t)N >= kStageMin &&
+                            gsc::staging_ready(c.stage);
 
     const size_t b_table = align256(direct_work_bytes(kind, dim, M, N, o));
     const size_t b_modes = align256(sizeof(double) * modes.size());
     const size_t b_pos = align256(sizeof(double) * dim * N);
     const size_t b_out = align256(elt * ncomp * N);
-    const size_t need = b_table + b_modes + b_pos + b_out;
+    const size_t need = b_table + b_modes + b_pos + b_out * (out_fortran ? 2 : 1);
     if (need > c.cap) {
       if (c.buf) GSC_HIP(hipFree(c.buf));
       c.buf = nullptr;
@@ -253,31 +265,62 @@ int host_summate(int kind, int dim, int64_t M, int64_t N, const double* sf, int6
 
     GSC_HIP(hipMemcpyAsync(d_modes, modes.data(), sizeof(double) * modes.size(),
                            hipMemcpyHostToDevice, c.stream));
-    GSC_HIP(hipMemcpyAsync(d_pos, pos_src, sizeof(double) * dim * N, hipMemcpyHostToDevice,
-                           c.stream));
+    std::vector<double> pos_dense;
+    if (staged_in) {
+      GSC_HIP(gsc::stage_upload(c.stage, c.stream, d_pos, pos, dim, N, pos_sd, pos_sp,
+                                o->num_threads));
+    } else {
+      const double* pos_src = pos;
+      if (!(pos_sp == 1 && pos_sd == N)) {
+        pos_dense.resize((size_t)(dim * N));
+        gather2(pos, dim, N, pos_sd, pos_sp, pos_dense.data(), o->num_threads);
+        pos_src = pos_dense.data();
+      }
+      GSC_HIP(hipMemcpyAsync(d_pos, pos_src, sizeof(double) * dim * N, hipMemcpyHostToDevice,
+                             c.stream));
+    }
     GSC_HIP(gsc::launch_prep(kind, dim, M, d_modes, d_modes + dim * M, d_modes + (dim + 1) * M,
                              kind == gsc::KIND_FOURIER ? d_modes + (dim + 2) * M : nullptr,
                              d_table, c.stream));
     GSC_HIP(launch_direct(kind, dim, M, N, d_table, d_pos, N, d_out, N, o, c.stream));
 
-    std::vector<char> out_tmp;
-    void* host_dst = out;
-    if (!out_dense) {
-      out_tmp.resize(elt * ncomp * N);
-      host_dst = out_tmp.data();
+    // result: a flat (1, ncomp N) copy for dense C-order and Fortran-order
+    // views, else the strided (ncomp, N) scatter
+    const void* d_res = d_out;
+    int64_t rows = ncomp, cols = N, sr = o_sd, sc = out_sp;
+    if (out_fortran) {
+      void* d_t = static_cast<char*>(d_out) + b_out;
+      GSC_HIP(gsc::launch_transpose_cn(d_out, d_t, ncomp, N, elt, c.stream));
+      d_res = d_t;
+      rows = 1;
+      cols = ncomp * N;
+      sr = cols;
+      sc = 1;
     }
-    GSC_HIP(hipMemcpyAsync(host_dst, d_out, elt * ncomp * N, hipMemcpyDeviceToHost, c.stream));
-    GSC_HIP(hipStreamSynchronize(c.stream));
-    if (!out_dense) {
-      for (int comp = 0; comp < ncomp; ++comp)
-        for (int64_t j = 0; j < N; ++j) {
-          const int64_t src = comp * N + j;
-          const int64_t dst = kind == gsc::KIND_INCOMPR ? comp * out_sd + j * out_sp : j * out_sp;
-          if (elt == 8)
-            static_cast<double*>(out)[dst] = reinterpret_cast<const double*>(out_tmp.data())[src];
-          else
-            static_cast<float*>(out)[dst] = reinterpret_cast<const float*>(out_tmp.data())[src];
-        }
+    if (staged_out) {
+      GSC_HIP(gsc::stage_download(c.stage, c.stream, d_res, out, rows, cols, sr, sc, elt,
+                                  o->num_threads));
+    } else {
+      const bool flat = sc == 1 && (rows == 1 || sr == cols);
+      std::vector<char> out_tmp;
+      void* host_dst = out;
+      if (!flat) {
+        out_tmp.resize(elt * ncomp * N);
+        host_dst = out_tmp.data();
+      }
+      GSC_HIP(hipMemcpyAsync(host_dst, d_res, elt * ncomp * N, hipMemcpyDeviceToHost, c.stream));
+      GSC_HIP(hipStreamSynchronize(c.stream));
+      if (!flat) {
+        for (int64_t r = 0; r < rows; ++r)
+          for (int64_t j = 0; j < cols; ++j) {
+            const int64_t src = r * cols + j;
+            const int64_t dst = r * sr + j * sc;
+            if (elt == 8)
+              static_cast<double*>(out)[dst] = reinterpret_cast<const double*>(out_tmp.data())[src];
+            else
+              static_cast<float*>(out)[dst] = reinterpret_cast<const float*>(out_tmp.data())[src];
+          }
+      }
     }
   } catch (const std::bad_alloc&) {
     return fail(GSC_ERR_ALLOC, "host allocation failed");
@@ -290,7 +333,7 @@ int host_summate(int kind, int dim, int64_t M, int64_t N, const double* sf, int6
 int dev_common(int kind, int dim, int64_t M, int64_t N, const double* sf, const double* cov,
                const double* z1, const double* z2, const double* pos, int64_t pos_ld, void* out,
                int64_t out_ld, void* work, size_t work_bytes, const gsc_opts* opts_in,
-               void* stream) {
+               void* stream, hipEvent_t split = nullptr) {
   gsc_opts tmp;
   const gsc_opts* o = opts_or_default(opts_in, &tmp);
   int rc = validate_common(dim, M, N, cov, z1, z2, pos, out);
@@ -311,7 +354,7 @@ int dev_common(int kind, int dim, int64_t M, int64_t N, const double* sf, const 
   if (N == 0) return GSC_OK;
   hipStream_t s = static_cast<hipStream_t>(stream);
   GSC_HIP(gsc::launch_prep(kind, dim, M, cov, z1, z2, sf, work, s));
-  GSC_HIP(launch_direct(kind, dim, M, N, work, pos, pos_ld, out, out_ld, o, s));
+  GSC_HIP(launch_direct(kind, dim, M, N, work, pos, pos_ld, out, out_ld, o, s, split));
   return GSC_OK;
 }
 
@@ -400,7 +443,8 @@ int field_comps(int field, int dim) { return field == GSC_FIELD_INCOMPR ? dim : 
 size_t grid_work_bytes(int field, int dim, int64_t M, const int64_t* shape, const gsc_opts* o,
                        int path) {
   if (path == GSC_PATH_FACTORED)
-    return gsc::plan_factored(dim, M, shape, o->precision, field_comps(field, dim)).bytes;
+    return gsc::plan_factored(dim, M, shape, o->precision, field_comps(field, dim),
+                              grid_tile(o)).bytes;
   if (field != GSC_FIELD_INCOMPR) return gsc::mode_table_bytes(dim, M);
   const int64_t N = grid_points(dim, shape);
   return align256(sizeof(double) * (size_t)dim * (size_t)N) +
@@ -419,7 +463,7 @@ int grid_launch(int field, int dim, int64_t M, const double* sf, const double* c
     const int kind = field == GSC_FIELD_FOURIER ? gsc::KIND_FOURIER : gsc::KIND_SCALAR;
     GSC_HIP(gsc::launch_prep(kind, dim, M, cov, z1, z2, sf, work, s));
     const gsc::FactoredPlan p =
-        gsc::plan_factored(dim, M, shape, o->precision, field_comps(field, dim));
+        gsc::plan_factored(dim, M, shape, o->precision, field_comps(field, dim), grid_tile(o));
     GSC_HIP(gsc::launch_factored(p, M, axes, work, out, o->out_dtype, o->scale, o->accum, s));
     return GSC_OK;
   }
@@ -596,6 +640,7 @@ void gsc_opts_init(gsc_opts* o) {
   o->accum = GSC_ACCUM_F32X2;
   o->direct_variant = GSC_DIRECT_LOCAL;
   o->scale = 1.0;
+  o->tile = GSC_TILE_AUTO;
 }
 
 const char* gsc_version(void) { return "1.1.0-mi355x.2"; }
@@ -667,6 +712,48 @@ int gsc_summate_dev_noalloc(int dim, int64_t M, int64_t N, const double* cov, co
   if ((rc = scratch_address(dev, &work))) return rc;
   return dev_common(gsc::KIND_SCALAR, dim, M, N, nullptr, cov, z1, z2, pos, pos_ld, out, 0, work,
                     (size_t)GSC_NOALLOC_SCRATCH_BYTES, o, stream);
+}
+
+int gsc_direct_phase_times(int field, int dim, int64_t M, int64_t N, const double* sf,
+                           const double* cov, const double* z1, const double* z2,
+                           const double* pos, int64_t pos_ld, void* out, int64_t out_ld,
+                           void* work, size_t work_bytes, const gsc_opts* opts, void* stream,
+                           int repeats, double* ms) {
+  if (field < GSC_FIELD_SCALAR || field > GSC_FIELD_FOURIER)
+    return fail(GSC_ERR_INVALID, "unknown field kind %d", field);
+  if (repeats < 1 || repeats > 10000 || !ms)
+    return fail(GSC_ERR_INVALID, "repeats must be in 1..10000 with a result array");
+  const int kind = field_kind(field);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // events: start, split and end of every call
+  std::vector<hipEvent_t> ev((size_t)(3 * repeats), nullptr);
+  hipError_t e = hipSuccess;
+  for (auto& x : ev)
+    if (e == hipSuccess) e = hipEventCreate(&x);
+  int rc = GSC_OK;
+  for (int r = 0; r < repeats && e == hipSuccess && rc == GSC_OK; ++r) {
+    e = hipEventRecord(ev[(size_t)(3 * r)], s);
+    if (e == hipSuccess)
+      rc = dev_common(kind, dim, M, N, sf, cov, z1, z2, pos, pos_ld, out, out_ld, work,
+                      work_bytes, opts, stream, ev[(size_t)(3 * r + 1)]);
+    if (e == hipSuccess && rc == GSC_OK) e = hipEventRecord(ev[(size_t)(3 * r + 2)], s);
+  }
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (e == hipSuccess && rc == GSC_OK) e = hipEventSynchronize(ev.back());
+  for (int r = 0; r < repeats && e == hipSuccess && rc == GSC_OK; ++r) {
+    float a = 0.0f, b = 0.0f;
+    e = hipEventElapsedTime(&a, ev[(size_t)(3 * r)], ev[(size_t)(3 * r + 1)]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&b, ev[(size_t)(3 * r + 1)], ev[(size_t)(3 * r + 2)]);
+    acc[0] += a;
+    acc[1] += b;
+    acc[2] += a + b;
+  }
+  for (auto& x : ev)
+    if (x) (void)hipEventDestroy(x);
+  if (rc != GSC_OK) return rc;
+  if (e != hipSuccess) return fail(GSC_ERR_HIP, "phase timing failed: %s", hipGetErrorString(e));
+  for (int i = 0; i < 3; ++i) ms[i] = acc[i] / repeats;
+  return GSC_OK;
 }
 
 int gsc_summate_incompr_dev(int dim, int64_t M, int64_t N, const double* cov, const double* z1,
@@ -833,6 +920,103 @@ int gsc_summate_grid_field_dev(int field, int dim, int64_t M, const double* sf,
                      static_cast<hipStream_t>(stream));
 }
 
+int gsc_tune_grid_time(int field, int dim, int64_t M, const int64_t* shape, const gsc_opts* opts,
+                       int repeats, double* ms) {
+  gsc_opts tmp;
+  const gsc_opts* o = opts_or_default(opts, &tmp);
+  static const double kOwnFactors = 1.0;  // the run supplies its own spectrum factors
+  int rc = validate_grid(dim, M, shape, shape);
+  if (rc || (rc = validate_opts(o)) || (rc = validate_precision(o)) ||
+      (rc = validate_field(field, dim, M, &kOwnFactors)))
+    return rc;
+  if (!ms) return fail(GSC_ERR_INVALID, "null result pointer");
+  if (repeats < 1 || repeats > 10000) return fail(GSC_ERR_INVALID, "repeats must be in 1..10000");
+  const int path = choose_grid_path(dim, M, shape, o);
+  if (path == GSC_PATH_FACTORED && dim < 2)
+    return fail(GSC_ERR_INVALID, "the factored path needs a 2-D or 3-D grid");
+  const int64_t N = grid_points(dim, shape);
+  *ms = 0.0;
+  if (N == 0) return GSC_OK;
+  int dev = 0;
+  if ((rc = resolve_device(o, &dev))) return rc;
+  GSC_HIP(hipSetDevice(dev));
+  if ((rc = check_arch(dev))) return rc;
+  const size_t elt = o->out_dtype == GSC_F32 ? 4 : 8;
+  const int64_t Mm = std::max<int64_t>(M, 1);
+  int64_t ax_total = 0;
+  for (int i = 0; i < dim; ++i) ax_total += shape[i];
+  const size_t b_modes = align256(sizeof(double) * (size_t)(dim + 3) * (size_t)Mm);
+  const size_t b_ax = align256(sizeof(double) * (size_t)std::max<int64_t>(ax_total, 1));
+  const size_t b_out = align256(elt * (size_t)field_comps(field, dim) * (size_t)N);
+  const size_t b_work = align256(grid_work_bytes(field, dim, M, shape, o, path));
+  char* mem = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  if (hipMalloc(&mem, b_modes + b_ax + b_out + b_work) != hipSuccess)
+    return fail(GSC_ERR_ALLOC, "hipMalloc of %zu bytes failed", b_modes + b_ax + b_out + b_work);
+  hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  for (int i = 0; i < 2 && e == hipSuccess; ++i) e = hipEventCreate(&ev[i]);
+  double* modes = reinterpret_cast<double*>(mem);
+  double* ax = reinterpret_cast<double*>(mem + b_modes);
+  if (e == hipSuccess) {
+    // unit-spaced axes and Gaussian modes drawn on the device (length scale
+    // an eighth of the largest extent); Fourier factors of one
+    std::vector<double> hax((size_t)std::max<int64_t>(ax_total, 1));
+    std::vector<double> ones((size_t)Mm, 1.0);
+    int64_t off = 0, ext = 1;
+    for (int i = 0; i < dim; ++i) {
+      for (int64_t k = 0; k < shape[i]; ++k) hax[(size_t)(off + k)] = (double)k;
+      off += shape[i];
+      ext = std::max(ext, shape[i]);
+    }
+    e = hipMemcpy(ax, hax.data(), sizeof(double) * hax.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(modes + (dim + 2) * Mm, ones.data(), sizeof(double) * ones.size(),
+                    hipMemcpyHostToDevice);
+    if (e == hipSuccess && M > 0) {
+      gsc::SampleArgs a{};
+      a.model = GSC_MODEL_GAUSSIAN;
+      a.dim = dim;
+      a.count = 1;
+      a.m_total = M;
+      a.len_scale1 = (double)ext / 8.0;
+      a.seed1 = 19031977;
+      a.cov = modes;
+      a.z1 = modes + dim * M;
+      a.z2 = modes + (dim + 1) * M;
+      e = gsc::launch_sample_modes(a, st);
+    }
+  }
+  if (e == hipSuccess) {
+    const double* axes[3] = {nullptr, nullptr, nullptr};
+    int64_t off = 0;
+    for (int i = 0; i < dim; ++i) {
+      axes[i] = ax + off;
+      off += shape[i];
+    }
+    void* out = mem + b_modes + b_ax;
+    void* work = mem + b_modes + b_ax + b_out;
+    for (int r = 0; r <= repeats && rc == GSC_OK; ++r) {
+      if (r == 1 && (e = hipEventRecord(ev[0], st)) != hipSuccess) break;
+      rc = grid_launch(field, dim, M, modes + (dim + 2) * Mm, modes, modes + dim * M,
+                       modes + (dim + 1) * M, axes, shape, out, work, o, path, st);
+    }
+    if (rc == GSC_OK && e == hipSuccess) e = hipEventRecord(ev[1], st);
+    if (rc == GSC_OK && e == hipSuccess) e = hipEventSynchronize(ev[1]);
+    float t = 0.0f;
+    if (rc == GSC_OK && e == hipSuccess) e = hipEventElapsedTime(&t, ev[0], ev[1]);
+    if (rc == GSC_OK && e == hipSuccess) *ms = (double)t / repeats;
+  }
+  if (st) (void)hipStreamSynchronize(st);
+  for (int i = 0; i < 2; ++i)
+    if (ev[i]) (void)hipEventDestroy(ev[i]);
+  if (st) (void)hipStreamDestroy(st);
+  (void)hipFree(mem);
+  if (rc != GSC_OK) return rc;
+  if (e != hipSuccess) return fail(GSC_ERR_HIP, "tuning run failed: %s", hipGetErrorString(e));
+  return GSC_OK;
+}
+
 int gsc_summate_grid_dev(int dim, int64_t M, const double* cov, const double* z1,
                          const double* z2, const double* const* axes, const int64_t* shape,
                          void* out, void* work, size_t work_bytes, const gsc_opts* opts,
@@ -872,14 +1056,17 @@ void ensemble_release(gsc_ensemble* p) {
 }
 
 // sample: null, or the mode sampler that fills cov / z1 / z2 at the start
-// of every replay (seeded plans).
+// of every replay (seeded plans).  tiles: per-realisation gsc_tile of the
+// factored ones, or null (opts->tile for all).
 int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes,
-                   const int64_t* m_counts, const int32_t* paths, const double* scales,
-                   const double* cov, const double* z1, const double* z2, const double* axes,
-                   void* out, const gsc_opts* o, const gsc::SampleArgs* sample) {
+                   const int64_t* m_counts, const int32_t* paths, const int32_t* tiles,
+                   const double* scales, const double* cov, const double* z1, const double* z2,
+                   const double* axes, void* out, const gsc_opts* o,
+                   const gsc::SampleArgs* sample) {
   const size_t elt = o->out_dtype == GSC_F32 ? 4 : 8;
   std::vector<gsc::EnsReal> rs((size_t)count);
-  std::vector<int> fact, dir, group_len;
+  std::vector<int> fact, dir;
+  std::vector<int> tile_of((size_t)count, GSC_TILE_256);
   std::vector<size_t> tab_off((size_t)count), f_off((size_t)count);
   std::vector<gsc::FactoredPlan> plans((size_t)count);
   const int tile_cols = gsc::gemm_tile_cols(gsc::precision_splits(o->precision), o->accum);
@@ -909,6 +1096,7 @@ int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes
     e.scale = scales ? scales[r] : o->scale;
     gsc_opts po = *o;
     po.path = paths ? paths[r] : GSC_PATH_AUTO;
+    if (tiles) po.tile = tiles[r];
     e.path = choose_grid_path(dim, M, sh, &po);
     m_off += M;
     for (int i = 0; i < dim; ++i) a_off += sh[i];
@@ -918,7 +1106,9 @@ int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes
     ws += align256(gsc::mode_table_bytes(dim, M));
     if (e.npts == 0) continue;
     if (e.path == GSC_PATH_FACTORED) {
-      const gsc::FactoredPlan fp = gsc::plan_factored(dim, M, sh, o->precision);
+      tile_of[(size_t)r] = grid_tile(&po);
+      const gsc::FactoredPlan fp =
+          gsc::plan_factored(dim, M, sh, o->precision, 1, tile_of[(size_t)r]);
       e.trans = fp.trans;
       plans[(size_t)r] = fp;
       e.mp = fp.mp;
@@ -938,67 +1128,95 @@ int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes
     }
     plan->point_modes += (double)e.npts * (double)M;
   }
-  // longest first: most modes, then most tiles; then a longest-processing-time
-  // split into kXcdGroups groups of whole realisations (one per XCD), each
-  // group keeping that order
-  std::stable_sort(fact.begin(), fact.end(), [&](int a, int b) {
-    const gsc::EnsReal &ea = rs[(size_t)a], &eb = rs[(size_t)b];
-    if (ea.mp != eb.mp) return ea.mp > eb.mp;
-    return ea.row_tiles * ea.col_tiles > eb.row_tiles * eb.col_tiles;
-  });
-  {
+  // GEMM groups by tile shape (256-row tiles first); in each, longest first
+  // (most modes, then most tiles), then a longest-processing-time split into
+  // kXcdGroups groups of whole realisations (one per XCD), each keeping that
+  // order
+  const int tile_ids[2] = {GSC_TILE_256, GSC_TILE_128};
+  std::vector<int> gemm_order[2];
+  std::vector<int64_t> tile_start[2], group_start[2];
+  int ngemm = 0;
+  for (int q = 0; q < 2; ++q) {
+    std::vector<int> lst;
+    for (int r : fact)
+      if (tile_of[(size_t)r] == tile_ids[q]) lst.push_back(r);
+    std::stable_sort(lst.begin(), lst.end(), [&](int a, int b) {
+      const gsc::EnsReal &ea = rs[(size_t)a], &eb = rs[(size_t)b];
+      if (ea.mp != eb.mp) return ea.mp > eb.mp;
+      return ea.row_tiles * ea.col_tiles > eb.row_tiles * eb.col_tiles;
+    });
     std::vector<std::vector<int>> groups(gsc::kXcdGroups);
     std::vector<double> load(gsc::kXcdGroups, 0.0);
-    for (int r : fact) {
+    for (int r : lst) {
       const gsc::EnsReal& e = rs[(size_t)r];
       const double w = (double)(e.row_tiles * e.col_tiles) * e.mp;
       const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
       groups[(size_t)g].push_back(r);
       load[(size_t)g] += w;
     }
-    fact.clear();
-    for (auto& g : groups) fact.insert(fact.end(), g.begin(), g.end());
-    group_len.assign(gsc::kXcdGroups, 0);
-    for (int g = 0; g < gsc::kXcdGroups; ++g) group_len[(size_t)g] = (int)groups[(size_t)g].size();
+    std::vector<int>& ord = gemm_order[q];
+    std::vector<int> glen;
+    for (auto& g : groups) {
+      ord.insert(ord.end(), g.begin(), g.end());
+      glen.push_back((int)g.size());
+    }
+    tile_start[q].assign(ord.size() + 1, 0);
+    for (size_t i = 0; i < ord.size(); ++i) {
+      const gsc::EnsReal& e = rs[(size_t)ord[i]];
+      tile_start[q][i + 1] = tile_start[q][i] + e.row_tiles * e.col_tiles;
+    }
+    group_start[q].assign(gsc::kXcdGroups + 1, 0);
+    size_t k = 0;
+    for (int g = 0; g < gsc::kXcdGroups; ++g) {
+      k += (size_t)glen[(size_t)g];
+      group_start[q][(size_t)g + 1] = tile_start[q][k];
+    }
+    if (!ord.empty()) ngemm = q + 1;
   }
-  std::stable_sort(dir.begin(), dir.end(), [&](int a, int b) {
-    return rs[(size_t)a].M * rs[(size_t)a].npts > rs[(size_t)b].M * rs[(size_t)b].npts;
-  });
-  std::vector<int64_t> tile_start(fact.size() + 1, 0), blk_start(dir.size() + 1, 0);
-  std::vector<int64_t> task_start(3 * fact.size() + 1, 0);
-  for (size_t i = 0; i < fact.size(); ++i) {
-    const gsc::EnsReal& e = rs[(size_t)fact[i]];
-    tile_start[i + 1] = tile_start[i] + e.row_tiles * e.col_tiles;
-    // table tasks: groups of 64 rows of S (na, 3-D only), A (arows) and C (ncp)
+  // table tasks: every factored realisation, groups of 64 rows of S (na,
+  // 3-D only), A (arows) and C (ncp)
+  std::vector<int> all_fact;
+  for (int q = 0; q < 2; ++q) all_fact.insert(all_fact.end(), gemm_order[q].begin(), gemm_order[q].end());
+  std::vector<int64_t> task_start(3 * all_fact.size() + 1, 0);
+  for (size_t i = 0; i < all_fact.size(); ++i) {
+    const gsc::EnsReal& e = rs[(size_t)all_fact[i]];
     task_start[3 * i + 1] = task_start[3 * i] + (dim == 3 ? (e.na + 63) / 64 : 0);
     task_start[3 * i + 2] = task_start[3 * i + 1] + e.arows / 64;
     task_start[3 * i + 3] = task_start[3 * i + 2] + e.ncp / 64;
   }
+  std::stable_sort(dir.begin(), dir.end(), [&](int a, int b) {
+    return rs[(size_t)a].M * rs[(size_t)a].npts > rs[(size_t)b].M * rs[(size_t)b].npts;
+  });
+  std::vector<int64_t> blk_start(dir.size() + 1, 0);
   for (size_t i = 0; i < dir.size(); ++i) {
     const gsc::EnsReal& e = rs[(size_t)dir[i]];
     blk_start[i + 1] = blk_start[i] + (e.npts + gsc::kEnsPoints - 1) / gsc::kEnsPoints;
   }
-  // tile ranges of the groups
-  std::vector<int64_t> group_start(gsc::kXcdGroups + 1, 0);
-  {
-    size_t i = 0;
-    for (int g = 0; g < gsc::kXcdGroups; ++g) {
-      i += (size_t)group_len[(size_t)g];
-      group_start[(size_t)g + 1] = tile_start[i];
-    }
-  }
-  plan->n_fact = (int64_t)fact.size();
+  plan->n_fact = (int64_t)all_fact.size();
   plan->n_dir = (int64_t)dir.size();
-  // control block: descriptors, lists, prefix sums, counters
-  const size_t c_desc = align256(sizeof(gsc::EnsReal) * (size_t)count);
-  const size_t c_fact = align256(sizeof(int) * (fact.size() + 1));
-  const size_t c_ts = align256(sizeof(int64_t) * tile_start.size());
-  const size_t c_dir = align256(sizeof(int) * (dir.size() + 1));
-  const size_t c_bs = align256(sizeof(int64_t) * blk_start.size());
-  const size_t c_tk = align256(sizeof(int64_t) * task_start.size());
-  const size_t c_gs = align256(sizeof(int64_t) * group_start.size());
-  const size_t c_cnt = 256;
-  const size_t ctrl = c_desc + c_fact + c_ts + c_dir + c_bs + c_tk + c_gs + c_cnt;
+  // control block: descriptors, lists, prefix sums, counters (each piece
+  // 256-byte aligned; offsets collected first, then one upload)
+  std::vector<char> hctrl;
+  // appends `bytes` from src (zeros when src is null) at a 256-byte boundary
+  auto put = [&hctrl](const void* src, size_t bytes) {
+    const size_t off = hctrl.size();
+    hctrl.resize(off + align256(std::max<size_t>(bytes, 1)), 0);
+    if (src != nullptr && bytes > 0) std::memcpy(hctrl.data() + off, src, bytes);
+    return off;
+  };
+  const size_t c_desc = put(rs.data(), sizeof(gsc::EnsReal) * rs.size());
+  const size_t c_fact = put(all_fact.data(), sizeof(int) * all_fact.size());
+  const size_t c_tk = put(task_start.data(), sizeof(int64_t) * task_start.size());
+  const size_t c_dir = put(dir.data(), sizeof(int) * dir.size());
+  const size_t c_bs = put(blk_start.data(), sizeof(int64_t) * blk_start.size());
+  size_t c_go[2], c_ts[2], c_gs[2];
+  for (int q = 0; q < 2; ++q) {
+    c_go[q] = put(gemm_order[q].data(), sizeof(int) * gemm_order[q].size());
+    c_ts[q] = put(tile_start[q].data(), sizeof(int64_t) * tile_start[q].size());
+    c_gs[q] = put(group_start[q].data(), sizeof(int64_t) * group_start[q].size());
+  }
+  const size_t c_cnt = put(nullptr, sizeof(int) * gsc::kEnsCounters);
+  const size_t ctrl = hctrl.size();
   plan->mem_bytes = ctrl + ws;
   if (hipMalloc(&plan->mem, plan->mem_bytes) != hipSuccess) {
     plan->mem = nullptr;
@@ -1022,40 +1240,40 @@ int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes
       e.C = reinterpret_cast<float2*>(f + (fp.off_c - fp.off_s));
     }
   }
-  std::vector<char> hctrl(ctrl, 0);
-  std::memcpy(hctrl.data(), rs.data(), sizeof(gsc::EnsReal) * rs.size());
-  std::memcpy(hctrl.data() + c_desc, fact.data(), sizeof(int) * fact.size());
-  std::memcpy(hctrl.data() + c_desc + c_fact, tile_start.data(), sizeof(int64_t) * tile_start.size());
-  std::memcpy(hctrl.data() + c_desc + c_fact + c_ts, dir.data(), sizeof(int) * dir.size());
-  std::memcpy(hctrl.data() + c_desc + c_fact + c_ts + c_dir, blk_start.data(),
-              sizeof(int64_t) * blk_start.size());
-  std::memcpy(hctrl.data() + c_desc + c_fact + c_ts + c_dir + c_bs, task_start.data(),
-              sizeof(int64_t) * task_start.size());
-  std::memcpy(hctrl.data() + c_desc + c_fact + c_ts + c_dir + c_bs + c_tk, group_start.data(),
-              sizeof(int64_t) * group_start.size());
+  std::memcpy(hctrl.data() + c_desc, rs.data(), sizeof(gsc::EnsReal) * rs.size());
   GSC_HIP(hipMemcpy(base, hctrl.data(), ctrl, hipMemcpyHostToDevice));
 
   int cus = 256;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, plan->device);
   gsc::EnsLaunch el{};
-  el.d = reinterpret_cast<const gsc::EnsReal*>(base);
+  el.d = reinterpret_cast<const gsc::EnsReal*>(base + c_desc);
   el.count = count;
-  el.fact = reinterpret_cast<const int*>(base + c_desc);
-  el.nfact = (int)fact.size();
-  el.tile_start = reinterpret_cast<const int64_t*>(base + c_desc + c_fact);
-  el.dir = reinterpret_cast<const int*>(base + c_desc + c_fact + c_ts);
-  el.ndir = (int)dir.size();
-  el.blk_start = reinterpret_cast<const int64_t*>(base + c_desc + c_fact + c_ts + c_dir);
-  el.task_start = reinterpret_cast<const int64_t*>(base + c_desc + c_fact + c_ts + c_dir + c_bs);
+  el.fact = reinterpret_cast<const int*>(base + c_fact);
+  el.nfact = (int)all_fact.size();
+  el.task_start = reinterpret_cast<const int64_t*>(base + c_tk);
   el.n_table_tasks = task_start.back();
-  el.group_start =
-      reinterpret_cast<const int64_t*>(base + c_desc + c_fact + c_ts + c_dir + c_bs + c_tk);
-  el.counters = reinterpret_cast<int*>(base + c_desc + c_fact + c_ts + c_dir + c_bs + c_tk + c_gs);
+  el.dir = reinterpret_cast<const int*>(base + c_dir);
+  el.ndir = (int)dir.size();
+  el.blk_start = reinterpret_cast<const int64_t*>(base + c_bs);
+  el.counters = reinterpret_cast<int*>(base + c_cnt);
+  el.dir_counter = el.counters + 2 * gsc::kXcdGroups;
   el.dim = dim;
   el.max_tiles = max_tiles;
-  // the FP64-flush GEMM runs two blocks per CU (it fits 256 registers)
-  const int64_t bpc = o->blocks_per_cu > 0 ? o->blocks_per_cu : (o->accum == GSC_ACCUM_F64 ? 2 : 1);
-  el.gemm_blocks = (int)std::max<int64_t>(1, std::min<int64_t>(tile_start.back(), bpc * cus));
+  el.ngemm = ngemm;
+  for (int q = 0; q < 2; ++q) {
+    gsc::EnsGemmGroup& g = el.gemm[q];
+    g.fact = reinterpret_cast<const int*>(base + c_go[q]);
+    g.nfact = (int)gemm_order[q].size();
+    g.tile_start = reinterpret_cast<const int64_t*>(base + c_ts[q]);
+    g.group_start = reinterpret_cast<const int64_t*>(base + c_gs[q]);
+    g.counters = el.counters + q * gsc::kXcdGroups;
+    g.brows = gsc::tile_block_rows(tile_ids[q]);
+    // blocks per CU the GEMM is built for: 256-row tiles one (two with the
+    // FP64 flush, which fits 256 registers), 128-row tiles two (four)
+    const int64_t bpc_auto = (o->accum == GSC_ACCUM_F64 ? 2 : 1) * (q == 1 ? 2 : 1);
+    const int64_t bpc = o->blocks_per_cu > 0 ? o->blocks_per_cu : bpc_auto;
+    g.gemm_blocks = (int)std::max<int64_t>(1, std::min<int64_t>(tile_start[q].back(), bpc * cus));
+  }
   el.direct_blocks = (int)std::max<int64_t>(1, std::min<int64_t>(blk_start.back(), 8 * (int64_t)cus));
 
   GSC_HIP(hipStreamCreateWithFlags(&plan->cap, hipStreamNonBlocking));
@@ -1092,7 +1310,8 @@ int validate_sample(int model, int dim, int64_t M, double len_scale) {
 // sampler description (the plan owns the mode arrays and draws them in its
 // graph).
 int ensemble_create(gsc_ensemble** plan, int count, int dim, const int64_t* shapes,
-                    const int64_t* m_counts, const int32_t* paths, const double* scales,
+                    const int64_t* m_counts, const int32_t* paths, const int32_t* tiles,
+                    const double* scales,
                     const double* cov, const double* z1, const double* z2, const double* axes,
                     void* out, const gsc_opts* opts, const SeededModes* seeded) {
   if (!plan) return fail(GSC_ERR_INVALID, "null plan pointer");
@@ -1114,6 +1333,8 @@ int ensemble_create(gsc_ensemble** plan, int count, int dim, const int64_t* shap
         return fail(GSC_ERR_INVALID, "realisation %d: negative axis length", r);
     if (paths && (paths[r] < GSC_PATH_AUTO || paths[r] > GSC_PATH_FACTORED))
       return fail(GSC_ERR_INVALID, "realisation %d: unknown path %d", r, paths[r]);
+    if (tiles && (tiles[r] < GSC_TILE_AUTO || tiles[r] > GSC_TILE_128))
+      return fail(GSC_ERR_INVALID, "realisation %d: unknown tile shape %d", r, tiles[r]);
     m_total += m_counts[r];
     pts += grid_points(dim, shapes + (int64_t)r * dim);
   }
@@ -1183,7 +1404,8 @@ int ensemble_create(gsc_ensemble** plan, int count, int dim, const int64_t* shap
       }
     }
     if (rc == GSC_OK)
-      rc = ensemble_build(p, count, dim, shapes, m_counts, paths, scales, cov, z1, z2, axes, out,
+      rc = ensemble_build(p, count, dim, shapes, m_counts, paths, tiles, scales, cov, z1, z2,
+                          axes, out,
                           o, seeded ? &sa : nullptr);
   } catch (const std::bad_alloc&) {
     rc = fail(GSC_ERR_ALLOC, "host allocation failed");
@@ -1204,8 +1426,30 @@ int gsc_ensemble_create(gsc_ensemble** plan, int count, int dim, const int64_t* 
                         const int64_t* m_counts, const int32_t* paths, const double* scales,
                         const double* cov, const double* z1, const double* z2,
                         const double* axes, void* out, const gsc_opts* opts) {
-  return ensemble_create(plan, count, dim, shapes, m_counts, paths, scales, cov, z1, z2, axes,
-                         out, opts, nullptr);
+  return ensemble_create(plan, count, dim, shapes, m_counts, paths, nullptr, scales, cov, z1, z2,
+                         axes, out, opts, nullptr);
+}
+
+int gsc_ensemble_create_tuned(gsc_ensemble** plan, int count, int dim, const int64_t* shapes,
+                              const int64_t* m_counts, const int32_t* paths, const int32_t* tiles,
+                              const double* scales, const double* cov, const double* z1,
+                              const double* z2, int model, const double* len_scales,
+                              const uint64_t* seeds, const double* axes, void* out,
+                              const gsc_opts* opts) {
+  if (!seeds)
+    return ensemble_create(plan, count, dim, shapes, m_counts, paths, tiles, scales, cov, z1, z2,
+                           axes, out, opts, nullptr);
+  if (model != GSC_MODEL_GAUSSIAN && model != GSC_MODEL_EXPONENTIAL)
+    return fail(GSC_ERR_INVALID, "unknown spectral model %d", model);
+  if (!len_scales) return fail(GSC_ERR_INVALID, "null len_scales");
+  if (count < 1 || count > 20000 || !m_counts)
+    return fail(GSC_ERR_INVALID, "count must be in 1..20000 with m_counts given");
+  for (int r = 0; r < count; ++r)
+    if (!(len_scales[r] > 0.0) || !std::isfinite(len_scales[r]))
+      return fail(GSC_ERR_INVALID, "realisation %d: len_scale must be positive and finite", r);
+  SeededModes sm{model, len_scales, seeds};
+  return ensemble_create(plan, count, dim, shapes, m_counts, paths, tiles, scales, nullptr,
+                         nullptr, nullptr, axes, out, opts, &sm);
 }
 
 int gsc_ensemble_create_seeded(gsc_ensemble** plan, int count, int dim, const int64_t* shapes,
@@ -1222,8 +1466,8 @@ int gsc_ensemble_create_seeded(gsc_ensemble** plan, int count, int dim, const in
     if (!(len_scales[r] > 0.0) || !std::isfinite(len_scales[r]))
       return fail(GSC_ERR_INVALID, "realisation %d: len_scale must be positive and finite", r);
   SeededModes sm{model, len_scales, seeds};
-  return ensemble_create(plan, count, dim, shapes, m_counts, paths, scales, nullptr, nullptr,
-                         nullptr, axes, out, opts, &sm);
+  return ensemble_create(plan, count, dim, shapes, m_counts, paths, nullptr, scales, nullptr,
+                         nullptr, nullptr, axes, out, opts, &sm);
 }
 
 int gsc_ensemble_reseed(gsc_ensemble* plan, const uint64_t* seeds, void* stream) {
@@ -1301,6 +1545,31 @@ int gsc_sample_modes(int model, int dim, int64_t M, double len_scale, uint64_t s
 int gsc_ensemble_run(gsc_ensemble* plan, void* stream) {
   if (!plan || !plan->exec) return fail(GSC_ERR_INVALID, "invalid ensemble plan");
   GSC_HIP(hipGraphLaunch(plan->exec, static_cast<hipStream_t>(stream)));
+  return GSC_OK;
+}
+
+int gsc_ensemble_time(gsc_ensemble* plan, int repeats, double* ms) {
+  if (!plan || !plan->exec) return fail(GSC_ERR_INVALID, "invalid ensemble plan");
+  if (repeats < 1 || repeats > 10000 || !ms)
+    return fail(GSC_ERR_INVALID, "repeats must be in 1..10000 with a result pointer");
+  hipStream_t st = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  hipError_t e = hipSetDevice(plan->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  for (int i = 0; i < 2 && e == hipSuccess; ++i) e = hipEventCreate(&ev[i]);
+  if (e == hipSuccess) e = hipGraphLaunch(plan->exec, st);  // warm-up
+  if (e == hipSuccess) e = hipEventRecord(ev[0], st);
+  for (int r = 0; r < repeats && e == hipSuccess; ++r) e = hipGraphLaunch(plan->exec, st);
+  if (e == hipSuccess) e = hipEventRecord(ev[1], st);
+  if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+  float t = 0.0f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, ev[0], ev[1]);
+  if (st) (void)hipStreamSynchronize(st);
+  for (int i = 0; i < 2; ++i)
+    if (ev[i]) (void)hipEventDestroy(ev[i]);
+  if (st) (void)hipStreamDestroy(st);
+  if (e != hipSuccess) return fail(GSC_ERR_HIP, "ensemble timing failed: %s", hipGetErrorString(e));
+  *ms = (double)t / repeats;
   return GSC_OK;
 }
 

@@ -1226,7 +1226,7 @@ bool binned_supported(int dim, int64_t N) {
 hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table,
                                 const double* pos, int64_t pos_ld, void* out, int64_t out_ld,
                                 bool vec, int out_dtype, double scale, void* bin_work,
-                                hipStream_t s) {
+                                hipStream_t s, hipEvent_t split) {
   if (N <= 0) return hipSuccess;
   const BinLayout L = bin_layout(M, N);
   BinParams* prm = at<BinParams>(bin_work, L.params);
@@ -1270,6 +1270,7 @@ hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table
     if (!vec && r0w != nullptr)                                                                  \
       k_cell_r0<D><<<static_cast<unsigned>(L.max_cells), 256, 0, s>>>(prm, centre, sm, r0w,      \
                                                                       L.r0ld);                   \
+    if (split) (void)hipEventRecord(split, s);                                                   \
     if (vec && f32)                                                                              \
       k_direct_local_vec<D, float><<<vb, 256, 0, s>>>(prm, centre, wave_cell, recs, btpl, ampt,  \
                                                       sm, rough, pos, pos_ld,                    \

@@ -115,21 +115,24 @@ hipError_t launch_expand_grid(int dim, const int64_t* shape, const double* const
   return hipGetLastError();
 }
 
-FactoredPlan plan_factored(int dim, int64_t M, const int64_t* shape, int precision, int ncomp) {
+FactoredPlan plan_factored(int dim, int64_t M, const int64_t* shape, int precision, int ncomp,
+                           int tile) {
   FactoredPlan p{};
   p.dim = dim;
   p.ncomp = ncomp;
+  p.brows = tile_block_rows(tile);
+  const int64_t br = p.brows;
   p.ns = precision_splits(precision);
   p.bf16 = precision_is_bf16(precision);
   p.mp = static_cast<int>(((M > 0 ? M : 1) + 31) / 32 * 32);
   p.nch = p.mp / 16;
   p.kp = 2 * static_cast<int64_t>(p.mp);
   if (dim == 2) {
-    // rows are padded to kBlockRows and columns to up to kMaxTileCols: take
-    // axis 1 as the row (A) axis when that pads less; the GEMM then stores
-    // the C-order output transposed (row stride 1, column stride nrow)
-    auto work = [](int64_t r, int64_t c) {
-      return (r + kBlockRows - 1) / kBlockRows * ((c + kMaxTileCols - 1) / kMaxTileCols);
+    // rows are padded to the block rows and columns to up to kMaxTileCols:
+    // take axis 1 as the row (A) axis when that pads less; the GEMM then
+    // stores the C-order output transposed (row stride 1, column stride nrow)
+    auto work = [br](int64_t r, int64_t c) {
+      return (r + br - 1) / br * br * ((c + kMaxTileCols - 1) / kMaxTileCols);
     };
     p.trans = work(shape[1], shape[0]) < work(shape[0], shape[1]) ? 1 : 0;
     p.na = 1;
@@ -142,10 +145,10 @@ FactoredPlan plan_factored(int dim, int64_t M, const int64_t* shape, int precisi
     p.nb_out = shape[1];
     p.ncols = shape[2];
   }
-  p.arows = (p.nrow + kBlockRows - 1) / kBlockRows * kBlockRows;
+  p.arows = (p.nrow + br - 1) / br * br;
   p.ncp = (p.ncols + kMaxTileCols - 1) / kMaxTileCols * kMaxTileCols;
   p.ncpt = p.ncp * ncomp;
-  p.row_tiles = (dim == 3 ? p.na : 1) * (p.arows / kBlockRows);
+  p.row_tiles = (dim == 3 ? p.na : 1) * (p.arows / br);
   auto al = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
   size_t off = al(mode_table_bytes(3, M));
   p.off_s = off;
@@ -209,7 +212,7 @@ hipError_t launch_factored(const FactoredPlan& p, int64_t M, const double* const
   const int64_t npts = (kr ? p.na : 1) * p.nrow * p.ncols;
   GemmArgs ga{A,       p.arows, p.nch, S,     p.na,  C,  p.ncp, p.nrow, p.nb_out, out,
               out_dtype != 1, p.trans ? 1 : p.ncols, p.ncols, p.trans ? p.nrow : 1, scale, kr,
-              accum, p.ncpt, npts};
+              accum, p.ncpt, npts, p.brows};
   const int64_t tile_cols = gemm_tile_cols(p.ns, accum);
   const int64_t tpc = (p.ncols + tile_cols - 1) / tile_cols;
   const dim3 grid(static_cast<unsigned>(p.row_tiles), static_cast<unsigned>(tpc * p.ncomp));
@@ -232,8 +235,7 @@ hipError_t launch_ensemble(const EnsLaunch& e, int precision, int out_dtype, int
     if (err != hipSuccess) return err;
   }
   if (e.ndir > 0) {
-    const int* cnt = e.counters + kXcdGroups;
-    int* c1 = const_cast<int*>(cnt);
+    int* c1 = e.dir_counter;
     if (e.dim == 3) {
       if (f32) k_ens_direct<3, float><<<e.direct_blocks, 256, 0, s>>>(e.d, e.dir, e.blk_start, e.ndir, c1);
       else k_ens_direct<3, double><<<e.direct_blocks, 256, 0, s>>>(e.d, e.dir, e.blk_start, e.ndir, c1);

@@ -39,7 +39,11 @@ inline bool precision_is_bf16(int prec) { return prec == 2 || prec == 3 || prec 
 //   [off_c, bytes)   C
 // (all chunk-major: the 16 modes of one row are contiguous, rows of one
 // chunk follow each other).
-constexpr int kBlockRows = 256;  // GEMM block rows: 4 waves x 64
+// GEMM block rows per tile shape (gsc_opts.tile): 256 (4 waves x 64 rows,
+// one wave per SIMD) or 128 (4 waves x 32 rows, two blocks per CU).
+constexpr int kBlockRowsTall = 256;
+constexpr int kBlockRowsShort = 128;
+inline int tile_block_rows(int tile) { return tile == 2 ? kBlockRowsShort : kBlockRowsTall; }
 constexpr int kMaxTileCols = 128;
 struct FactoredPlan {
   int dim;
@@ -50,13 +54,14 @@ struct FactoredPlan {
   int64_t kp;        // 2 * mp
   int64_t na;        // 3-D: axis 0 (one block row band per index); 2-D: 1
   int64_t nrow;      // length of the A axis
-  int64_t arows;     // nrow padded to kBlockRows
+  int brows;         // GEMM block rows (kBlockRowsTall or kBlockRowsShort)
+  int64_t arows;     // nrow padded to brows
   int64_t nb_out;    // 3-D: output rows per axis-0 index (= nrow); 2-D: 0
   int64_t ncols;     // last-axis length
   int64_t ncp;       // ncols padded to kMaxTileCols (one field component)
   int ncomp;         // field components: 1, or dim for the incompressible field
   int64_t ncpt;      // ncomp * ncp: columns of the C table and of the GEMM
-  int64_t row_tiles; // 3-D: na * arows / kBlockRows; 2-D: arows / kBlockRows
+  int64_t row_tiles; // 3-D: na * arows / brows; 2-D: arows / brows
   int trans;         // 2-D with axis 1 as the A axis (it pads less); the output is stored transposed
   size_t off_s, off_a, off_c, bytes;
 };
@@ -71,7 +76,7 @@ constexpr int gemm_tile_cols(int ns, int accum) {
 }
 
 FactoredPlan plan_factored(int dim, int64_t M, const int64_t* shape, int precision,
-                           int ncomp = 1);
+                           int ncomp = 1, int tile = 0);
 
 // One realisation of an ensemble (device-resident descriptor).
 struct EnsReal {
@@ -100,31 +105,42 @@ struct EnsReal {
 };
 
 // Launch sequence of an ensemble (all pointers device-resident).  fact / dir
-// list the realisations on each path, tile_start / blk_start their prefix sums
-// of work items (n + 1 entries), counters kEnsCounters ints.  max_tiles bounds the
-// batched prep grid; the *_blocks are the persistent grid sizes.
+// list the realisations on each path, task_start / blk_start their prefix
+// sums of table tasks and direct work items.  The factored realisations run in
+// up to two persistent GEMM launches, one per tile shape: each has its own
+// realisation list, tile prefix sums, XCD groups and counters.  max_tiles
+// bounds the batched prep grid; the *_blocks are the persistent grid sizes.
+constexpr int kXcdGroups = 8;     // ensemble GEMM work groups (one per XCD of an MI355X)
+struct EnsGemmGroup {
+  const int* fact;             // realisations of this tile shape, longest first per XCD group
+  int nfact;
+  const int64_t* tile_start;   // nfact + 1 prefix sums of their GEMM tiles
+  const int64_t* group_start;  // tile ranges of the kXcdGroups groups (kXcdGroups + 1)
+  int* counters;               // kXcdGroups work counters
+  int gemm_blocks;
+  int brows;                   // block rows of the tile shape (256 or 128)
+};
 struct EnsLaunch {
   const EnsReal* d;
   int count;
-  const int* fact;
+  const int* fact;             // every factored realisation (table tasks)
   int nfact;
-  const int64_t* tile_start;
   const int64_t* task_start;   // table tasks: 3 nfact + 1 prefix sums (roles S, A, C)
-  const int64_t* group_start;  // tile ranges of the kXcdGroups groups (kXcdGroups + 1)
   int64_t n_table_tasks;
+  EnsGemmGroup gemm[2];
+  int ngemm;
   const int* dir;
   int ndir;
   const int64_t* blk_start;
-  int* counters;
+  int* dir_counter;
+  int* counters;               // all kEnsCounters counters (reset at the start of a run)
   int dim;
   int max_tiles;
-  int gemm_blocks;
   int direct_blocks;
 };
 constexpr int kEnsPoints = 2048;  // points per direct work item
-constexpr int kXcdGroups = 8;     // ensemble GEMM work groups (one per XCD of an MI355X)
-// counters: kXcdGroups GEMM group counters, then the direct-kernel counter
-constexpr int kEnsCounters = kXcdGroups + 1;
+// counters: kXcdGroups per GEMM launch (two), then the direct-kernel counter
+constexpr int kEnsCounters = 2 * kXcdGroups + 1;
 hipError_t launch_ensemble(const EnsLaunch& e, int precision, int out_dtype, int accum,
                            hipStream_t stream);
 
@@ -162,6 +178,7 @@ struct GemmArgs {
   int accum;
   int64_t ncpt;        // columns of the C table: ncomp blocks of ncp
   int64_t comp_stride; // output elements between field components
+  int brows;           // block rows: 256 or 128 (the plan's tile shape)
 };
 
 // Expects the mode table at the start of `work` (launch_prep, KIND_SCALAR).

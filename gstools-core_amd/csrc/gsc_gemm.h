@@ -82,12 +82,15 @@ constexpr int cols_blocks() {
   return ACC == ACC_F64 ? 1 : (NS == 3 ? 2 : 4);
 }
 
-// Waves per SIMD the GEMM kernels are built for: the one-column-block FP64
-// flush variant fits 256 registers, so a second block hides the first one's
-// barrier and load waits; the wider tiles need more than 256 registers.
-template <int ACC>
+// Waves per SIMD the GEMM kernels are built for.  RB = 2 (256-row blocks,
+// 64 rows per wave): the wide tiles need more than 256 registers, one wave
+// per SIMD; the one-column-block FP64 flush variant fits 256, so a second
+// block hides the first one's barrier and load waits.  RB = 1 (128-row
+// blocks, 32 rows per wave): half the accumulators, two blocks per CU, so one
+// block's barrier and fragment waits are covered by the other's MFMAs.
+template <int ACC, int RB>
 constexpr int gemm_waves() {
-  return ACC == ACC_F64 ? 2 : 1;
+  return RB == 1 ? (ACC == ACC_F64 ? 4 : 2) : (ACC == ACC_F64 ? 2 : 1);
 }
 
 static_assert(32 * cols_blocks<2, ACC_F32X2>() == gemm_tile_cols(2, ACC_F32X2) &&
@@ -298,18 +301,19 @@ struct TileJob {
   int64_t ldc;  // column stride of the output (1, or the row count of a transposed store)
 };
 
-// Output of one wave's 64 x 32 JN tile (rows past nvalid and columns past
+// Output of one wave's 32 RB x 32 JN tile (rows past nvalid and columns past
 // ncols are not stored).
-template <int ACC, int JN, bool TR, typename OutT, typename A2>
+template <int ACC, int JN, int RB, bool TR, typename OutT, typename A2>
 __device__ __forceinline__ void gemm_store(OutT* __restrict__ out, const TileJob& jb,
-                                           const f32x16_t (&acc)[2][JN], const A2& acc2,
+                                           const f32x16_t (&acc)[RB][JN], const A2& acc2,
                                            int wave, int hl, int lr) {
   const int64_t ldo = jb.ldo;
-  const int nvalid = static_cast<int>(jb.nvalid < kBlockRows ? jb.nvalid : kBlockRows);
+  constexpr int kRows = 128 * RB;
+  const int nvalid = static_cast<int>(jb.nvalid < kRows ? jb.nvalid : kRows);
   const float scale32 = static_cast<float>(jb.scale);
 #pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    const int r0 = 64 * wave + 32 * i + 4 * hl;
+  for (int i = 0; i < RB; ++i) {
+    const int r0 = 32 * RB * wave + 32 * i + 4 * hl;
 #pragma unroll
     for (int j = 0; j < JN; ++j) {
       const int64_t col = jb.col0 + 32 * j + lr;
@@ -337,7 +341,7 @@ __device__ __forceinline__ void gemm_store(OutT* __restrict__ out, const TileJob
   }
 }
 
-template <typename T, int NS, bool KR, int ACC, int JN, bool TR = false>
+template <typename T, int NS, bool KR, int ACC, int JN, int RB, bool TR = false>
 __device__ __forceinline__ void gemm_tile(GemmSmem<T, NS, JN>& sm, const TileJob& jb) {
   using V8 = typename Fmt<T>::v8;
   using A2 = typename AccT<ACC>::type;
@@ -349,9 +353,9 @@ __device__ __forceinline__ void gemm_tile(GemmSmem<T, NS, JN>& sm, const TileJob
   const int lr = lane & 31;  // row / column within a 32 x 32 block
   const int nch = jb.nch;
 
-  // A fragments: row arow0 + 64 wave + 32 i + lr, packed pairs 8 ks + 4 hl .. + 3
+  // A fragments: row arow0 + 32 RB wave + 32 i + lr, packed pairs 8 ks + 4 hl .. + 3
   const __attribute__((address_space(1))) uint32_t* abase =
-      gmem(jb.A) + (hl * jb.arows + jb.arow0 + 64 * wave + lr) * 4;
+      gmem(jb.A) + (hl * jb.arows + jb.arow0 + 32 * RB * wave + lr) * 4;
   const int64_t a_ks = 2 * jb.arows * 4;
   const int64_t a_chunk = 2 * a_ks;
   const int64_t a_part = a_chunk * nch;
@@ -428,30 +432,36 @@ __device__ __forceinline__ void gemm_tile(GemmSmem<T, NS, JN>& sm, const TileJob
     }
   };
 
-  V8 af[2][2][NS][2];  // [chunk parity][ks][part][row block]
+  V8 af[2][2][NS][RB];  // [chunk parity][ks][part][row block]
   auto aload = [&](const int set, int c) {
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks)
 #pragma unroll
       for (int s = 0; s < NS; ++s)
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < RB; ++i)
           af[set][ks][s][i] = *reinterpret_cast<const __attribute__((address_space(1))) V8*>(
               abase + s * a_part + c * a_chunk + ks * a_ks + 128 * i);
   };
-  V8 bf[2][NS][JN];  // [ks][part][column block]
+  // B fragments: both k halves of a chunk held at once with RB = 2 (read a
+  // half-step ahead); with RB = 1 one half at a time, read just before its
+  // MFMAs (the registers go to the second wave on the SIMD, which covers the
+  // LDS latency)
+  constexpr int KB = RB == 2 ? 2 : 1;
+  V8 bf[KB][NS][JN];  // [ks][part][column block]
   auto bread = [&](const int ks, const int buf) {
 #pragma unroll
     for (int s = 0; s < NS; ++s)
 #pragma unroll
       for (int j = 0; j < JN; ++j)
-        bf[ks][s][j] = *reinterpret_cast<const V8*>(&sm.B[buf][s][32 * j + lr][16 * ks + 8 * hl]);
+        bf[ks % KB][s][j] =
+            *reinterpret_cast<const V8*>(&sm.B[buf][s][32 * j + lr][16 * ks + 8 * hl]);
   };
 
-  f32x16_t acc[2][JN];
-  A2 acc2[2][JN][ACC == ACC_F32 ? 1 : 16];
+  f32x16_t acc[RB][JN];
+  A2 acc2[RB][JN][ACC == ACC_F32 ? 1 : 16];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < RB; ++i)
 #pragma unroll
     for (int j = 0; j < JN; ++j) {
 #pragma unroll
@@ -466,10 +476,10 @@ __device__ __forceinline__ void gemm_tile(GemmSmem<T, NS, JN>& sm, const TileJob
 #pragma unroll
       for (int sb = 0; sb < NS - sa; ++sb)
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < RB; ++i)
 #pragma unroll
           for (int j = 0; j < JN; ++j)
-            acc[i][j] = mfma(af[set][ks][sa][i], bf[ks][sb][j], acc[i][j]);
+            acc[i][j] = mfma(af[set][ks][sa][i], bf[ks % KB][sb][j], acc[i][j]);
   };
 
   // One chunk c (register set and LDS buffer c % 2).  The barrier sits
@@ -479,12 +489,29 @@ __device__ __forceinline__ void gemm_tile(GemmSmem<T, NS, JN>& sm, const TileJob
   // A wave issues in order, so the memory and ALU work of each half is
   // spread between its MFMAs (scheduling groups): a burst of loads would
   // stall the MFMAs behind it.
-  constexpr int kM = 2 * JN * NS * (NS + 1) / 2;  // MFMAs per half
-  constexpr int kA = 4 * NS;                       // A fragment loads per chunk
+  constexpr int kM = RB * JN * NS * (NS + 1) / 2;  // MFMAs per half
+  constexpr int kA = 2 * RB * NS;                   // A fragment loads per chunk
   constexpr int kB = NS * JN;                      // B fragment reads per half
   constexpr int kW = NS * GC;                      // B tile writes per chunk
   constexpr int kG = 2 * GC + (KR ? 1 : 0);        // C (and S) loads per chunk
+  // RB = 1: one barrier per chunk after both halves, all of chunk c's
+  // fragments read and chunk c + 1's tile written by then
+  auto step1 = [&](int c, const int set) {
+    const int cn = c + 1 < nch ? c + 1 : c;
+    aload(set ^ 1, cn);
+    bread(0, set);
+    mfma_ks(set, 0);
+    gstore(set ^ 1);
+    bread(1, set);
+    mfma_ks(set, 1);
+    __syncthreads();
+    gload(c + 2 < nch ? c + 2 : cn);
+  };
   auto step = [&](int c, const int set) {
+    if constexpr (RB == 1) {
+      step1(c, set);
+      return;
+    }
     const int cn = c + 1 < nch ? c + 1 : c;
     aload(set ^ 1, cn);
     bread(1, set);
@@ -520,7 +547,7 @@ __device__ __forceinline__ void gemm_tile(GemmSmem<T, NS, JN>& sm, const TileJob
   gload(nch > 1 ? 1 : 0);
   aload(0, 0);
   __syncthreads();
-  bread(0, 0);
+  if constexpr (RB == 2) bread(0, 0);
   // nch is even (modes padded to 32): the unrolled pair fixes set and buffer
   for (int c = 0; c < nch; c += 2) {
     step(c, 0);
@@ -528,7 +555,7 @@ __device__ __forceinline__ void gemm_tile(GemmSmem<T, NS, JN>& sm, const TileJob
     if constexpr (ACC != ACC_F32) {
       if ((c + 2) % kFlush == 0 || c + 2 == nch) {
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < RB; ++i)
 #pragma unroll
           for (int j = 0; j < JN; ++j)
 #pragma unroll
@@ -546,16 +573,16 @@ __device__ __forceinline__ void gemm_tile(GemmSmem<T, NS, JN>& sm, const TileJob
   // multiples of ldo.  FP32 output is scaled in FP32 (the sums are FP32
   // already unless ACC_F64).
   if (jb.out_f64)
-    gemm_store<ACC, JN, TR>(static_cast<double*>(jb.out), jb, acc, acc2, wave, hl, lr);
+    gemm_store<ACC, JN, RB, TR>(static_cast<double*>(jb.out), jb, acc, acc2, wave, hl, lr);
   else
-    gemm_store<ACC, JN, TR>(static_cast<float*>(jb.out), jb, acc, acc2, wave, hl, lr);
+    gemm_store<ACC, JN, RB, TR>(static_cast<float*>(jb.out), jb, acc, acc2, wave, hl, lr);
 }
 
 // Tile job of row tile rt, column tile ct of one contraction.  Column tiles
 // run component-major: tiles_per_comp tiles of 32 JN output columns per field
 // component, component i's C-table columns starting at i ncp and its output
 // at out + i comp_stride.
-template <bool KR, int JN>
+template <bool KR, int JN, int RB>
 __device__ __forceinline__ TileJob make_job(const uint32_t* A, int64_t arows, int nch,
                                             const float2* S, int64_t na, const float2* C,
                                             int64_t ncpt, int64_t nrow, int64_t nb_out,
@@ -563,9 +590,9 @@ __device__ __forceinline__ TileJob make_job(const uint32_t* A, int64_t arows, in
                                             int64_t ncols, double scale, int64_t rt,
                                             int64_t ct, int64_t ldc, int64_t tiles_per_comp,
                                             int64_t ncp, int64_t comp_stride) {
-  const int64_t nbt = arows / kBlockRows;
+  const int64_t nbt = arows / (128 * RB);
   const int64_t a = KR ? rt / nbt : 0;
-  const int64_t arow0 = (rt - a * nbt) * kBlockRows;
+  const int64_t arow0 = (rt - a * nbt) * (128 * RB);
   const int64_t comp = ct / tiles_per_comp;
   TileJob jb;
   jb.A = A;
@@ -590,17 +617,18 @@ __device__ __forceinline__ TileJob make_job(const uint32_t* A, int64_t arows, in
   return jb;
 }
 
-template <typename T, int NS, bool KR, int ACC, bool TR>
+template <typename T, int NS, bool KR, int ACC, bool TR, int RB>
 __global__ void __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(gemm_waves<ACC>(), gemm_waves<ACC>())))
+__attribute__((amdgpu_waves_per_eu(gemm_waves<ACC, RB>(), gemm_waves<ACC, RB>())))
 k_factored_gemm(GemmArgs g) {
   constexpr int JN = cols_blocks<NS, ACC>();
   __shared__ __attribute__((aligned(16))) GemmSmem<T, NS, JN> sm;
   const int64_t tpc = (g.ncols + 32 * JN - 1) / (32 * JN);
-  const TileJob jb = make_job<KR, JN>(g.A, g.arows, g.nch, g.S, g.na, g.C, g.ncpt, g.nrow,
-                                      g.nb_out, g.out, g.out_f64, g.ldo, g.ncols, g.scale,
-                                      blockIdx.x, blockIdx.y, g.ldc, tpc, g.ncp, g.comp_stride);
-  gemm_tile<T, NS, KR, ACC, JN, TR>(sm, jb);
+  const TileJob jb = make_job<KR, JN, RB>(g.A, g.arows, g.nch, g.S, g.na, g.C, g.ncpt, g.nrow,
+                                          g.nb_out, g.out, g.out_f64, g.ldo, g.ncols, g.scale,
+                                          blockIdx.x, blockIdx.y, g.ldc, tpc, g.ncp,
+                                          g.comp_stride);
+  gemm_tile<T, NS, KR, ACC, JN, RB, TR>(sm, jb);
 }
 
 // Tables of all factored realisations: one block per task (realisation,
@@ -690,9 +718,9 @@ k_ens_tables(const EnsReal* __restrict__ d, const int* __restrict__ fact,
 // to the XCDs round-robin, so a realisation's tables stay in one XCD's L2 --
 // and, once its group is exhausted, helps the others in turn.  Every group
 // counter only grows, so every block leaves every group and exits.
-template <typename T, int NS, bool KR, int ACC>
+template <typename T, int NS, bool KR, int ACC, int RB>
 __global__ void __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(gemm_waves<ACC>(), gemm_waves<ACC>())))
+__attribute__((amdgpu_waves_per_eu(gemm_waves<ACC, RB>(), gemm_waves<ACC, RB>())))
 k_ens_gemm(const EnsReal* __restrict__ d, const int* __restrict__ fact,
            const int64_t* __restrict__ tile_start, int nfact,
            const int64_t* __restrict__ group_start, int* __restrict__ counters, bool out_f64) {
@@ -715,12 +743,12 @@ k_ens_gemm(const EnsReal* __restrict__ d, const int* __restrict__ fact,
       const int64_t rt = local / r.col_tiles, ct = local - rt * r.col_tiles;
       // a transposed realisation (2-D): rows are y, columns x of the C-order
       // (x, y) output, so the row stride is 1 and the column stride nrow
-      const TileJob jb = make_job<KR, JN>(r.A, r.arows, r.mp / 16, r.S, r.na, r.C, r.ncp, r.nrow,
-                                          r.nb_out, r.out, out_f64, r.trans ? 1 : r.ncols,
-                                          r.ncols, r.scale, rt, ct, r.trans ? r.nrow : 1,
-                                          r.col_tiles, r.ncp, 0);
-      if (!KR && r.trans) gemm_tile<T, NS, KR, ACC, JN, true>(sm, jb);
-      else gemm_tile<T, NS, KR, ACC, JN, false>(sm, jb);
+      const TileJob jb = make_job<KR, JN, RB>(r.A, r.arows, r.mp / 16, r.S, r.na, r.C, r.ncp,
+                                              r.nrow, r.nb_out, r.out, out_f64,
+                                              r.trans ? 1 : r.ncols, r.ncols, r.scale, rt, ct,
+                                              r.trans ? r.nrow : 1, r.col_tiles, r.ncp, 0);
+      if (!KR && r.trans) gemm_tile<T, NS, KR, ACC, JN, RB, true>(sm, jb);
+      else gemm_tile<T, NS, KR, ACC, JN, RB, false>(sm, jb);
     }
   }
 }
@@ -738,10 +766,15 @@ hipError_t tables_launch(const TableArgs& a, dim3 grid, hipStream_t s) {
 template <typename T, int NS>
 hipError_t factored_gemm_launch(const GemmArgs& a, dim3 grid, hipStream_t s) {
   using namespace gemm;
-#define GSC_G(KR, ACC)                                                  \
-  do {                                                                  \
-    if (!KR && a.ldc != 1) k_factored_gemm<T, NS, KR, ACC, true><<<grid, 256, 0, s>>>(a); \
-    else k_factored_gemm<T, NS, KR, ACC, false><<<grid, 256, 0, s>>>(a);                  \
+#define GSC_GR(KR, ACC, RB)                                                                \
+  do {                                                                                     \
+    if (!KR && a.ldc != 1) k_factored_gemm<T, NS, KR, ACC, true, RB><<<grid, 256, 0, s>>>(a); \
+    else k_factored_gemm<T, NS, KR, ACC, false, RB><<<grid, 256, 0, s>>>(a);                  \
+  } while (0)
+#define GSC_G(KR, ACC)                        \
+  do {                                        \
+    if (a.brows == 128) GSC_GR(KR, ACC, 1);   \
+    else GSC_GR(KR, ACC, 2);                  \
   } while (0)
   if (a.kr) {
     if (a.accum == ACC_F32X2) GSC_G(true, ACC_F32X2);
@@ -753,6 +786,7 @@ hipError_t factored_gemm_launch(const GemmArgs& a, dim3 grid, hipStream_t s) {
     else GSC_G(false, ACC_F64);
   }
 #undef GSC_G
+#undef GSC_GR
   return hipGetLastError();
 }
 
@@ -761,19 +795,30 @@ hipError_t ens_launch(const EnsLaunch& e, int accum, bool out_f64, hipStream_t s
   using namespace gemm;
   k_ens_tables<T, NS><<<static_cast<unsigned>(e.n_table_tasks), 256, 0, s>>>(
       e.d, e.fact, e.task_start, e.nfact);
-#define GSC_E(KR, ACC)                                                                      \
-  k_ens_gemm<T, NS, KR, ACC><<<e.gemm_blocks, 256, 0, s>>>(e.d, e.fact, e.tile_start, e.nfact, \
-                                                           e.group_start, e.counters, out_f64)
-  if (e.dim == 3) {
-    if (accum == ACC_F32X2) GSC_E(true, ACC_F32X2);
-    else if (accum == ACC_F32) GSC_E(true, ACC_F32);
-    else GSC_E(true, ACC_F64);
-  } else {
-    if (accum == ACC_F32X2) GSC_E(false, ACC_F32X2);
-    else if (accum == ACC_F32) GSC_E(false, ACC_F32);
-    else GSC_E(false, ACC_F64);
-  }
+  for (int q = 0; q < e.ngemm; ++q) {
+    const EnsGemmGroup& g = e.gemm[q];
+    if (g.nfact == 0) continue;
+#define GSC_ER(KR, ACC, RB)                                                                \
+  k_ens_gemm<T, NS, KR, ACC, RB><<<g.gemm_blocks, 256, 0, s>>>(e.d, g.fact, g.tile_start, \
+                                                              g.nfact, g.group_start,    \
+                                                              g.counters, out_f64)
+#define GSC_E(KR, ACC)                          \
+  do {                                          \
+    if (g.brows == 128) GSC_ER(KR, ACC, 1);     \
+    else GSC_ER(KR, ACC, 2);                    \
+  } while (0)
+    if (e.dim == 3) {
+      if (accum == ACC_F32X2) GSC_E(true, ACC_F32X2);
+      else if (accum == ACC_F32) GSC_E(true, ACC_F32);
+      else GSC_E(true, ACC_F64);
+    } else {
+      if (accum == ACC_F32X2) GSC_E(false, ACC_F32X2);
+      else if (accum == ACC_F32) GSC_E(false, ACC_F32);
+      else GSC_E(false, ACC_F64);
+    }
 #undef GSC_E
+#undef GSC_ER
+  }
   return hipGetLastError();
 }
 

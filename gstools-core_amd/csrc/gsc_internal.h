@@ -56,10 +56,12 @@ hipError_t launch_direct_incompr(int dim, int64_t M, int64_t N, const void* tabl
 // vec = true: out[c * out_ld + j] for c < dim (incompressible, amp[c]).
 bool binned_supported(int dim, int64_t N);
 size_t binned_work_bytes(int dim, int64_t M, int64_t N);
+// split: null, or an event recorded on `stream` between the cell sort and the
+// summation kernel (phase timing).
 hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table,
                                 const double* pos, int64_t pos_ld, void* out, int64_t out_ld,
                                 bool vec, int out_dtype, double scale, void* bin_work,
-                                hipStream_t stream);
+                                hipStream_t stream, hipEvent_t split = nullptr);
 
 // Seeded spectral-mode sampling (sampling.hip): Philox4x64-10 words of key
 // (seed, stream_id), Box-Muller normals, k = g / l (model 0, Gaussian) or

@@ -47,6 +47,7 @@ class GscOpts(ctypes.Structure):
         ("direct_variant", ctypes.c_int32),
         ("blocks_per_cu", ctypes.c_int32),
         ("scale", ctypes.c_double),
+        ("tile", ctypes.c_int32),
     ]
 
 
@@ -90,10 +91,19 @@ SIGNATURES = {
                                       _i64, _vpp, _i64p, _i64p, _vp, _optp]),
     "gsc_summate_grid_field_dev": (_int, [_int, _int, _i64, _vp, _vp, _vp, _vp, _vpp, _i64p, _vp,
                                           _vp, _sz, _optp, _vp]),
+    "gsc_tune_grid_time": (_int, [_int, _int, _i64, _i64p, _optp, _int,
+                                  ctypes.POINTER(ctypes.c_double)]),
+    "gsc_ensemble_create_tuned": (_int, [ctypes.POINTER(_vp), _int, _int, _i64p, _i64p,
+                                         ctypes.POINTER(ctypes.c_int32),
+                                         ctypes.POINTER(ctypes.c_int32),
+                                         ctypes.POINTER(ctypes.c_double), _vp, _vp, _vp, _int,
+                                         ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_uint64), _vp, _vp, _optp]),
     "gsc_ensemble_create": (_int, [ctypes.POINTER(_vp), _int, _int, _i64p, _i64p,
                                    ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double),
                                    _vp, _vp, _vp, _vp, _vp, _optp]),
     "gsc_ensemble_run": (_int, [_vp, _vp]),
+    "gsc_ensemble_time": (_int, [_vp, _int, ctypes.POINTER(ctypes.c_double)]),
     "gsc_ensemble_info": (_int, [_vp, _i64p, _i64p, ctypes.POINTER(_sz),
                                  ctypes.POINTER(ctypes.c_double)]),
     "gsc_ensemble_destroy": (None, [_vp]),
@@ -111,6 +121,11 @@ SIGNATURES = {
                          _vp, _i64, _optp]),
     "gsc_krige_workspace_bytes": (_sz, [_i64, _i64, _int]),
     "gsc_krige_dev": (_int, [_i64, _i64, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _sz, _optp, _vp]),
+    "gsc_direct_phase_times": (_int, [_int, _int, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _i64, _vp,
+                                      _i64, _vp, _sz, _optp, _vp, _int,
+                                      ctypes.POINTER(ctypes.c_double)]),
+    "gsc_calibrate_sine": (_int, [_int, ctypes.POINTER(ctypes.c_double),
+                                  ctypes.POINTER(ctypes.c_double)]),
     "gsc_device_alloc": (_vp, [_sz, _int]),
     "gsc_device_free": (None, [_vp]),
     "gsc_memcpy_to_device": (_int, [_vp, _vp, _sz]),
@@ -123,16 +138,20 @@ _lock = threading.Lock()
 
 
 def _preload_hip_runtime() -> None:
-    """Makes the process use ONE HIP runtime when PyTorch is installed.
+    """Chooses the HIP runtime the library binds to.
 
     PyTorch-ROCm ships its own libamdhip64.so with the same SONAME
-    (libamdhip64.so.7) as /opt/rocm's.  Whichever is loaded first serves every
-    later request for that SONAME, and torch does not initialise on a runtime
-    other than its own.  Loading torch's copy first (RTLD_GLOBAL) lets this
-    library and torch share it in either import order.  GSC_HIP_RUNTIME=system
-    keeps the system runtime instead.
+    (libamdhip64.so.7) as /opt/rocm's; the first one loaded serves every later
+    request for that SONAME in the process.  GSC_HIP_RUNTIME:
+      auto (default): when torch is already imported its runtime is loaded,
+              and this library binds to it; otherwise the system runtime
+              (/opt/rocm) is used.  torch does not initialise on a runtime
+              other than its own, so import torch first to use both;
+      torch:  preload torch's copy even when torch is not imported yet;
+      system: preload nothing.
     """
-    if os.environ.get("GSC_HIP_RUNTIME", "auto") == "system":
+    mode = os.environ.get("GSC_HIP_RUNTIME", "auto")
+    if mode != "torch":
         return
     import importlib.util
     spec = importlib.util.find_spec("torch")
@@ -141,6 +160,18 @@ def _preload_hip_runtime() -> None:
     cand = os.path.join(list(spec.submodule_search_locations)[0], "lib", "libamdhip64.so")
     if os.path.exists(cand):
         ctypes.CDLL(cand, mode=ctypes.RTLD_GLOBAL)
+
+
+def hip_runtime_path() -> str:
+    """File of the HIP runtime mapped into this process ("" if none)."""
+    try:
+        with open("/proc/self/maps") as f:
+            for line in f:
+                if "libamdhip64.so" in line:
+                    return line.split()[-1]
+    except OSError:
+        pass
+    return ""
 
 
 def lib():
@@ -165,11 +196,12 @@ def lib():
 
 
 ACCUM = {"f64": 0, "f32x2": 1, "f32": 2}
+TILES = {"auto": 0, "256": 1, "128": 2}
 DIRECT_VARIANTS = {"mfma": 0, "valu": 1, "local": 2}
 
 
 def make_opts(out_dtype=GSC_F64, scale=1.0, device=-1, num_threads=0, path=GSC_PATH_AUTO,
-              precision=0, accum=1, direct_variant=2) -> GscOpts:
+              precision=0, accum=1, direct_variant=2, tile=0) -> GscOpts:
     o = GscOpts()
     lib().gsc_opts_init(ctypes.byref(o))
     o.out_dtype = out_dtype
@@ -180,6 +212,7 @@ def make_opts(out_dtype=GSC_F64, scale=1.0, device=-1, num_threads=0, path=GSC_P
     o.precision = int(precision)
     o.accum = int(accum)
     o.direct_variant = int(direct_variant)
+    o.tile = int(tile)
     return o
 
 

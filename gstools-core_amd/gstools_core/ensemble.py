@@ -8,8 +8,10 @@ work lists over all realisations (GEMM tiles for factored ones, point blocks
 for direct ones) and captures the launches in a HIP graph; ``run`` replays it.
 
 The per-realisation choice between the direct kernel and the factored MFMA
-contraction comes from the library rule, an explicit ``paths`` list, or the
-autotuner (``tune=True``, see gstools_core.tuner).
+contraction, and the GEMM tile shape of each factored realisation, come from
+the library rule, explicit ``paths`` / ``tiles`` lists, or the autotuner
+(``tune=True``, see gstools_core.tuner), which also picks the plan's split
+format and accumulation when they are left to it.
 """
 from __future__ import annotations
 
@@ -32,9 +34,11 @@ class Ensemble:
     """
 
     def __init__(self, realisations: Sequence, *, precision: str = "auto",
-                 out_dtype=np.float32, accum: str = "f32x2", paths: Optional[Sequence[str]] = None,
+                 out_dtype=np.float32, accum: Optional[str] = None,
+                 paths: Optional[Sequence[str]] = None,
                  scales: Optional[Sequence[float]] = None, tune: bool = False, tuner=None,
-                 device: int = -1, blocks_per_cu: int = 0):
+                 device: int = -1, blocks_per_cu: int = 0, tile: str = "auto",
+                 tiles: Optional[Sequence[str]] = None):
         if len(realisations) == 0:
             raise ValueError("an ensemble needs at least one realisation")
         covs, z1s, z2s, grids, ms = [], [], [], [], []
@@ -50,17 +54,17 @@ class Ensemble:
             grids.append(axes)
             ms.append(cov.shape[1])
         args = self._setup(grids, ms, precision, out_dtype, accum, paths, scales, tune, tuner,
-                           device, blocks_per_cu)
+                           device, blocks_per_cu, tile, tiles)
         cat = lambda xs: np.concatenate(xs) if xs else np.zeros(0)  # noqa: E731
         self._cov = _native.DeviceBuffer.from_array(cat(covs), device)
         self._z1 = _native.DeviceBuffer.from_array(cat(z1s), device)
         self._z2 = _native.DeviceBuffer.from_array(cat(z2s), device)
         plan = ctypes.c_void_p()
-        c_shapes, c_ms, path_codes, scale_arr, opts = args
-        rc = _native.lib().gsc_ensemble_create(
-            ctypes.byref(plan), self.count, self.dim, c_shapes, c_ms, path_codes, scale_arr,
-            self._cov.ptr, self._z1.ptr, self._z2.ptr, self._ax.ptr, self._out.ptr,
-            ctypes.byref(opts))
+        c_shapes, c_ms, path_codes, tile_codes, scale_arr, opts = args
+        rc = _native.lib().gsc_ensemble_create_tuned(
+            ctypes.byref(plan), self.count, self.dim, c_shapes, c_ms, path_codes, tile_codes,
+            scale_arr, self._cov.ptr, self._z1.ptr, self._z2.ptr, 0, None, None, self._ax.ptr,
+            self._out.ptr, ctypes.byref(opts))
         _native.check(rc)
         self._plan = plan
 
@@ -81,19 +85,21 @@ class Ensemble:
             raise ValueError("one mode count, length scale and seed per realisation")
         self = cls.__new__(cls)
         self._cov = self._z1 = self._z2 = None
-        c_shapes, c_ms, path_codes, scale_arr, opts = self._setup(
+        c_shapes, c_ms, path_codes, tile_codes, scale_arr, opts = self._setup(
             grids, [int(m) for m in m_counts], kw.pop("precision", "auto"),
-            kw.pop("out_dtype", np.float32), kw.pop("accum", "f32x2"), kw.pop("paths", None),
+            kw.pop("out_dtype", np.float32), kw.pop("accum", None), kw.pop("paths", None),
             kw.pop("scales", None), kw.pop("tune", False), kw.pop("tuner", None),
-            kw.pop("device", -1), kw.pop("blocks_per_cu", 0))
+            kw.pop("device", -1), kw.pop("blocks_per_cu", 0), kw.pop("tile", "auto"),
+            kw.pop("tiles", None))
         if kw:
             raise TypeError(f"unexpected keyword arguments {sorted(kw)}")
         c_len = (ctypes.c_double * n)(*[float(v) for v in len_scales])
         c_seeds = (ctypes.c_uint64 * n)(*[int(v) & 0xFFFFFFFFFFFFFFFF for v in seeds])
         plan = ctypes.c_void_p()
-        rc = _native.lib().gsc_ensemble_create_seeded(
-            ctypes.byref(plan), self.count, self.dim, c_shapes, c_ms, path_codes, scale_arr,
-            MODELS[model], c_len, c_seeds, self._ax.ptr, self._out.ptr, ctypes.byref(opts))
+        rc = _native.lib().gsc_ensemble_create_tuned(
+            ctypes.byref(plan), self.count, self.dim, c_shapes, c_ms, path_codes, tile_codes,
+            scale_arr, None, None, None, MODELS[model], c_len, c_seeds, self._ax.ptr,
+            self._out.ptr, ctypes.byref(opts))
         _native.check(rc)
         self._plan = plan
         return self
@@ -107,7 +113,7 @@ class Ensemble:
                                                         ctypes.c_void_p(stream or None)))
 
     def _setup(self, grids, ms, precision, out_dtype, accum, paths, scales, tune, tuner, device,
-               blocks_per_cu):
+               blocks_per_cu, tile="auto", tiles=None):
         """Geometry, output buffer and options shared by both constructors."""
         dims = {len(g) for g in grids}
         if len(dims) != 1:
@@ -123,15 +129,24 @@ class Ensemble:
         shapes = [[len(a) for a in g] for g in grids]
         self.shapes = [tuple(s) for s in shapes]
         self.m_counts = list(ms)
+        self.tuned = None
         if tune:
             from .tuner import Tuner
             tuner = tuner or Tuner()
-            paths = [tuner.choose(self.dim, s, m, precision, accum) for s, m in zip(shapes, ms)]
+            precision, accum, paths, tiles = tuner.plan(self.dim, shapes, ms, precision, accum)
+            self.tuned = {"precision": precision, "accum": accum, "paths": list(paths),
+                          "tiles": list(tiles)}
+        accum = accum or "f32x2"
         path_codes = None
         if paths is not None:
             if len(paths) != self.count:
                 raise ValueError("one path per realisation")
             path_codes = (ctypes.c_int32 * self.count)(*[PATHS[p] for p in paths])
+        tile_codes = None
+        if tiles is not None:
+            if len(tiles) != self.count:
+                raise ValueError("one tile shape per realisation")
+            tile_codes = (ctypes.c_int32 * self.count)(*[_native.TILES[str(t)] for t in tiles])
         scale_arr = None
         if scales is not None:
             if len(scales) != self.count:
@@ -144,11 +159,12 @@ class Ensemble:
                                          device)
         opts = _native.make_opts(
             out_dtype=_native.GSC_F32 if self.out_dtype == np.float32 else _native.GSC_F64,
-            precision=_native.PRECISIONS[precision], accum=_native.ACCUM[accum], device=device)
+            precision=_native.PRECISIONS[precision], accum=_native.ACCUM[accum], device=device,
+            tile=_native.TILES[str(tile)])
         opts.blocks_per_cu = int(blocks_per_cu)
         c_shapes = (ctypes.c_int64 * (self.count * self.dim))(*[v for s in shapes for v in s])
         c_ms = (ctypes.c_int64 * self.count)(*ms)
-        return c_shapes, c_ms, path_codes, scale_arr, opts
+        return c_shapes, c_ms, path_codes, tile_codes, scale_arr, opts
 
     @property
     def info(self) -> dict:
@@ -163,6 +179,13 @@ class Ensemble:
     def run(self, stream: int = 0) -> None:
         """Replays the captured graph on `stream` (0: null stream); does not synchronise."""
         _native.check(_native.lib().gsc_ensemble_run(self._plan, ctypes.c_void_p(stream or None)))
+
+    def time(self, repeats: int = 10) -> float:
+        """Mean ms of one replay (HIP events, private stream, after a warm-up)."""
+        ms = ctypes.c_double()
+        _native.check(_native.lib().gsc_ensemble_time(self._plan, int(repeats),
+                                                      ctypes.byref(ms)))
+        return ms.value
 
     @property
     def out_ptr(self) -> int:

@@ -30,9 +30,12 @@ def _axes_args(axes):
     return axes
 
 
-def _opts(path, precision, out_code, scale, device, nt, accum="f32x2", variant="valu"):
+def _opts(path, precision, out_code, scale, device, nt, accum="f32x2", variant="valu",
+          tile="auto"):
     if path not in PATHS:
         raise ValueError(f"path must be one of {sorted(PATHS)}")
+    if str(tile) not in _native.TILES:
+        raise ValueError(f"tile must be one of {sorted(_native.TILES)}")
     if precision not in _native.PRECISIONS:
         raise ValueError(f"precision must be one of {sorted(_native.PRECISIONS)}")
     if accum not in _native.ACCUM:
@@ -40,7 +43,8 @@ def _opts(path, precision, out_code, scale, device, nt, accum="f32x2", variant="
     return _native.make_opts(out_dtype=out_code, scale=scale, device=device, num_threads=nt,
                              path=PATHS[path], precision=_native.PRECISIONS[precision],
                              accum=_native.ACCUM[accum],
-                             direct_variant=_native.DIRECT_VARIANTS[variant])
+                             direct_variant=_native.DIRECT_VARIANTS[variant],
+                             tile=_native.TILES[str(tile)])
 
 
 def grid_path(cov_samples, axes, path="auto", precision="auto") -> str:
@@ -60,7 +64,7 @@ FIELDS = {"scalar": _native.GSC_FIELD_SCALAR, "incompr": _native.GSC_FIELD_INCOM
 
 
 def _grid_call(field, sf, cov_samples, z1, z2, axes, num_threads, path, precision, out_dtype,
-               scale, device, accum, variant):
+               scale, device, accum, variant, tile="auto"):
     cov = _array_arg("modes" if field == "fourier" else "cov_samples", cov_samples, 2)
     z1 = _array_arg("z1", z1, 1)
     z2 = _array_arg("z2", z2, 1)
@@ -87,7 +91,7 @@ def _grid_call(field, sf, cov_samples, z1, z2, axes, num_threads, path, precisio
     out = np.empty(((d,) if field == "incompr" else ()) + shape, dtype=dt)
     if out.size == 0:
         return out
-    o = _opts(path, precision, code, scale, device, nt, accum, variant)
+    o = _opts(path, precision, code, scale, device, nt, accum, variant, tile)
     c_axes = (ctypes.c_void_p * d)(*[a.ctypes.data for a in axes])
     c_as = (ctypes.c_int64 * d)(*[a.strides[0] // 8 for a in axes])
     c_shape = (ctypes.c_int64 * d)(*shape)
@@ -100,7 +104,8 @@ def _grid_call(field, sf, cov_samples, z1, z2, axes, num_threads, path, precisio
 
 
 def summate_grid(cov_samples, z1, z2, axes, num_threads=None, *, path="auto", precision="auto",
-                 out_dtype=np.float64, scale=1.0, device=-1, accum="f32x2", variant="valu"):
+                 out_dtype=np.float64, scale=1.0, device=-1, accum="f32x2", variant="valu",
+                 tile="auto"):
     """Field on the grid axes[0] x ... x axes[d-1] (d in 1..3), C order, shape (len(ax) ...).
 
     Equal to ``summate(cov_samples, z1, z2, pos)`` with pos the expanded grid
@@ -108,46 +113,46 @@ def summate_grid(cov_samples, z1, z2, axes, num_threads=None, *, path="auto", pr
     precision's error (fp16x3 default: below 1e-5 sigma).
     """
     return _grid_call("scalar", None, cov_samples, z1, z2, axes, num_threads, path, precision,
-                      out_dtype, scale, device, accum, variant)
+                      out_dtype, scale, device, accum, variant, tile)
 
 
 def summate_incompr_grid(cov_samples, z1, z2, axes, num_threads=None, *, path="auto",
                          precision="auto", out_dtype=np.float64, scale=1.0, device=-1,
-                         accum="f32x2", variant="valu"):
+                         accum="f32x2", variant="valu", tile="auto"):
     """Incompressible field (field.rs:97-182) on the grid axes[0] x .. x
     axes[d-1], d in {2, 3}: shape (d, len(axes[0]), ...), C order.  Equal to
     ``summate_incompr`` on the expanded grid; the factored path folds the
     projector into the last axis' table (one contraction for all components)."""
     return _grid_call("incompr", None, cov_samples, z1, z2, axes, num_threads, path, precision,
-                      out_dtype, scale, device, accum, variant)
+                      out_dtype, scale, device, accum, variant, tile)
 
 
 def summate_fourier_grid(spectrum_factor, modes, z1, z2, axes, num_threads=None, *,
                          path="auto", precision="auto", out_dtype=np.float64, scale=1.0,
-                         device=-1, accum="f32x2", variant="valu"):
+                         device=-1, accum="f32x2", variant="valu", tile="auto"):
     """Fourier-method field (field.rs:219-249) on the grid axes[0] x .. x
     axes[d-1]: shape (len(axes[0]), ...), C order; equal to ``summate_fourier``
     on the expanded grid."""
     return _grid_call("fourier", spectrum_factor, modes, z1, z2, axes, num_threads, path,
-                      precision, out_dtype, scale, device, accum, variant)
+                      precision, out_dtype, scale, device, accum, variant, tile)
 
 
 def grid_workspace_bytes(dim, m, shape, path="auto", precision="auto", field="scalar",
-                         variant="valu") -> int:
+                         variant="valu", tile="auto") -> int:
     c_shape = (ctypes.c_int64 * dim)(*shape)
-    o = _opts(path, precision, _native.GSC_F64, 1.0, -1, 0, variant=variant)
+    o = _opts(path, precision, _native.GSC_F64, 1.0, -1, 0, variant=variant, tile=tile)
     return int(_native.lib().gsc_grid_field_workspace_bytes(FIELDS[field], dim, m, c_shape,
                                                             ctypes.byref(o)))
 
 
 def summate_grid_dev(dim, m, cov, z1, z2, axes_ptrs, shape, out, work, work_bytes, *,
                      path="auto", precision="auto", out_f32=False, scale=1.0, stream=0,
-                     accum="f32x2", variant="valu", field="scalar", sf=None):
+                     accum="f32x2", variant="valu", field="scalar", sf=None, tile="auto"):
     """Device form: every pointer is a device address (int); see
     gsc_summate_grid_field_dev (field "scalar", "incompr" or "fourier"; sf:
     the spectrum factors of the Fourier field)."""
     o = _opts(path, precision, _native.GSC_F32 if out_f32 else _native.GSC_F64, scale, -1, 0,
-              accum, variant)
+              accum, variant, tile)
     c_axes = (ctypes.c_void_p * dim)(*axes_ptrs)
     c_shape = (ctypes.c_int64 * dim)(*shape)
     rc = _native.lib().gsc_summate_grid_field_dev(FIELDS[field], dim, m, sf, cov, z1, z2, c_axes,

@@ -1,29 +1,56 @@
-"""Autotuner: direct kernel vs factored MFMA contraction per grid shape class.
+"""Autotuner for structured grids: evaluation path, split format, accumulation
+and GEMM tile shape, per shape class.
 
-For a structured grid the two paths compute the same field (to within the
-factored format's error); which is faster depends on the grid shape (padding
-of the 128 x 128 GEMM tiles, table size relative to the output) and on M.
-The tuner measures both on synthetic, seeded modes for a shape class (every
-axis length rounded up to a power of two, M rounded up to a multiple of 256),
-keeps the faster one, and caches the decision as JSON keyed by device, library
-version, class, precision and accumulation.
+For a structured grid the direct kernel and the factored MFMA contraction
+compute the same field (within the factored format's error); which is faster,
+and which split format, accumulation and block tile of the contraction, depends
+on the grid shape (padding of the GEMM tiles, table size relative to the
+output) and on M.  The candidates are only those that meet 1e-5 sigma:
+
+  direct                                   (the per-point kernel on the grid)
+  factored  x  {fp16x3, bf16x6}            (bf16x3 and the single products miss the bar)
+            x  {f32x2, f64}                (two-level accumulation)
+            x  {256, 128}                  (GEMM block rows; 128 runs two blocks per CU)
+
+A shape class is the grid with every axis rounded up to a power of two (at
+least 16) and M rounded up to a multiple of 256.  Each candidate is timed on
+the class by the library itself (gsc_tune_grid_time: HIP events around
+`repeats` calls after a warm-up, synthetic seeded modes), and the table of
+times is cached as JSON keyed by device, library version, field and class.
+A grid inside a class is estimated from the class time scaled by its padded
+work (the GEMM's padded rows x columns x modes, or points x modes for the
+direct kernel), so the choice also follows each grid's own tile padding.
+
+Ensembles are timed in their own setting: a single call of a small grid is
+dominated by launch latency and by how many CUs its few tiles occupy, while
+in a persistent plan only the throughput per tile counts.  For a plan the
+tuner therefore times every candidate on a plan of k copies of the class
+grid (k chosen for about 2e10 point-modes per replay, at most 64; gsc_ensemble_time),
+per split format and accumulation, and picks the pair with the smallest
+summed estimate, then per realisation its path and tile shape.
 
 Cache file: $GSC_TUNE_CACHE, else ~/.cache/gstools_core_amd/tuning.json.
 """
 from __future__ import annotations
 
+import ctypes
 import json
 import os
-import time
-from typing import Dict, Sequence
-
-import numpy as np
+from typing import Dict, List, Optional, Sequence, Tuple
 
 from . import _native
-from .grid import summate_grid_dev, grid_workspace_bytes
+from .grid import FIELDS, PATHS
 
 _DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "gstools_core_amd",
                               "tuning.json")
+
+FORMATS = ("fp16x3", "bf16x6")  # the split formats that meet 1e-5 sigma
+ACCUMS = ("f32x2", "f64")
+TILES = ("256", "128")
+
+_TILE_ROWS = {"256": 256, "128": 128}
+_TILE_COLS = {("fp16x3", "f32x2"): 128, ("bf16x6", "f32x2"): 64, ("fp16x3", "f64"): 32,
+              ("bf16x6", "f64"): 32}
 
 
 def _pow2(n: int) -> int:
@@ -32,13 +59,55 @@ def _pow2(n: int) -> int:
 
 
 def shape_class(shape: Sequence[int], m: int):
+    """(axes rounded up to powers of two (>= 16), M rounded up to 256)."""
     return tuple(_pow2(s) for s in shape), max(256, (int(m) + 255) // 256 * 256)
 
 
+def config_key(path: str, precision: str = "", accum: str = "", tile: str = "") -> str:
+    return "direct" if path == "direct" else f"factored/{precision}/{accum}/{tile}"
+
+
+def parse_config(key: str) -> dict:
+    if key == "direct":
+        return {"path": "direct"}
+    _, precision, accum, tile = key.split("/")
+    return {"path": "factored", "precision": precision, "accum": accum, "tile": tile}
+
+
+def padded_work(shape: Sequence[int], m: int, key: str) -> float:
+    """Point-mode work of one evaluation as the kernels pad it: the direct
+    kernel N M; the factored GEMM (rows padded to the block rows, columns to
+    the block tile, modes to 32), in the orientation of a 2-D plan that pads
+    less."""
+    shape = [int(s) for s in shape]
+    if key == "direct":
+        n = 1
+        for s in shape:
+            n *= s
+        return float(n) * max(int(m), 1)
+    c = parse_config(key)
+    br = _TILE_ROWS[c["tile"]]
+    tc = _TILE_COLS[(c["precision"], c["accum"])]
+    mp = (max(int(m), 1) + 31) // 32 * 32
+
+    def pad(n, k):
+        return (n + k - 1) // k * k
+
+    if len(shape) == 2:
+        a = pad(shape[0], br) * ((shape[1] + 127) // 128 * 128)
+        b = pad(shape[1], br) * ((shape[0] + 127) // 128 * 128)
+        rows, cols = (shape[1], shape[0]) if b < a else (shape[0], shape[1])
+        return float(pad(rows, br)) * pad(cols, tc) * mp
+    return float(shape[0]) * pad(shape[1], br) * pad(shape[2], tc) * mp
+
+
 class Tuner:
-    def __init__(self, path: str = None, repeats: int = 3):
+    def __init__(self, path: Optional[str] = None, repeats: int = 10,
+                 formats: Sequence[str] = FORMATS, accums: Sequence[str] = ACCUMS,
+                 tiles: Sequence[str] = TILES):
         self.path = path or os.environ.get("GSC_TUNE_CACHE", _DEFAULT_CACHE)
-        self.repeats = repeats
+        self.repeats = int(repeats)
+        self.formats, self.accums, self.tiles = tuple(formats), tuple(accums), tuple(tiles)
         self.table: Dict[str, dict] = {}
         try:
             with open(self.path) as f:
@@ -46,60 +115,152 @@ class Tuner:
         except (OSError, ValueError):
             self.table = {}
         self.device_key = f"gfx950|{_native.lib().gsc_version().decode()}"
+        self.measured = 0  # classes measured (not found in the cache) by this tuner
 
-    def _key(self, dim, shape, m, precision, accum) -> str:
+    def _key(self, dim, shape, m, field) -> str:
         cls, mb = shape_class(shape, m)
-        return f"{self.device_key}|{dim}|{'x'.join(map(str, cls))}|{mb}|{precision}|{accum}"
+        return f"{self.device_key}|{field}|{dim}|{'x'.join(map(str, cls))}|{mb}"
 
-    def measure(self, dim: int, shape: Sequence[int], m: int, precision: str = "auto",
-                accum: str = "f32x2") -> dict:
-        """Times both paths on the shape class of (shape, m); returns ms per path."""
+    def candidates(self, dim: int) -> List[str]:
+        keys = ["direct"]
+        if dim >= 2:
+            keys += [config_key("factored", p, a, t) for p in self.formats for a in self.accums
+                     for t in self.tiles]
+        return keys
+
+    def time_config(self, dim: int, shape: Sequence[int], m: int, key: str,
+                    field: str = "scalar") -> float:
+        """ms of one call (library timing, HIP events) for this exact shape."""
+        c = parse_config(key)
+        opts = _native.make_opts(
+            out_dtype=_native.GSC_F32, path=PATHS[c["path"]],
+            precision=_native.PRECISIONS[c.get("precision", "auto")],
+            accum=_native.ACCUM[c.get("accum", "f32x2")],
+            direct_variant=_native.DIRECT_VARIANTS["valu"],
+            tile=_native.TILES[c.get("tile", "auto")])
+        c_shape = (ctypes.c_int64 * dim)(*[int(s) for s in shape])
+        ms = ctypes.c_double()
+        _native.check(_native.lib().gsc_tune_grid_time(FIELDS[field], dim, int(m), c_shape,
+                                                       ctypes.byref(opts), self.repeats,
+                                                       ctypes.byref(ms)))
+        return ms.value
+
+    def measure(self, dim: int, shape: Sequence[int], m: int, field: str = "scalar") -> dict:
+        """Times every candidate on the class of (shape, m): config -> ms."""
         cls, mb = shape_class(shape, m)
-        rng = np.random.default_rng(12345)
-        cov = rng.normal(0, 0.05, size=(dim, mb))
-        z1, z2 = rng.normal(size=mb), rng.normal(size=mb)
-        axes = [np.arange(n, dtype=np.float64) for n in cls]
-        bufs = [_native.DeviceBuffer.from_array(a) for a in (cov, z1, z2, *axes)]
-        n = int(np.prod(cls))
-        out = _native.DeviceBuffer(n * 4)
-        res = {}
-        try:
-            for path in ("direct", "factored"):
-                nb = grid_workspace_bytes(dim, mb, cls, path=path, precision=precision)
-                work = _native.DeviceBuffer(nb)
+        return {k: self.time_config(dim, cls, mb, k, field) for k in self.candidates(dim)}
 
-                def step():
-                    summate_grid_dev(dim, mb, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr,
-                                     [b.ptr for b in bufs[3:]], cls, out.ptr, work.ptr, nb,
-                                     path=path, precision=precision, out_f32=True, accum=accum)
+    def class_times(self, dim: int, shape: Sequence[int], m: int,
+                    field: str = "scalar") -> dict:
+        """The class's table of times, measured on first use and cached."""
+        key = self._key(dim, shape, m, field)
+        hit = self.table.get(key)
+        missing = hit is None or any(k not in hit["ms"] for k in self.candidates(dim))
+        if missing:
+            ms = self.measure(dim, shape, m, field)
+            cls, mb = shape_class(shape, m)
+            hit = {"class": [list(cls), mb], "ms": ms, "best": min(ms, key=ms.get)}
+            self.table[key] = hit
+            self.measured += 1
+            self.save()
+        return hit
 
-                step()
-                _native.synchronize()
-                t0 = time.perf_counter()
-                for _ in range(self.repeats):
-                    step()
-                _native.synchronize()
-                res[path] = (time.perf_counter() - t0) * 1e3 / self.repeats
-                work.free()
-        finally:
-            out.free()
-            for b in bufs:
-                b.free()
-        return res
+    def estimate(self, dim: int, shape: Sequence[int], m: int, key: str,
+                 field: str = "scalar") -> float:
+        """Estimated ms of `key` on this grid: the class time scaled by padded work."""
+        hit = self.class_times(dim, shape, m, field)
+        cls, mb = hit["class"]
+        return hit["ms"][key] * padded_work(shape, m, key) / padded_work(cls, mb, key)
 
-    def choose(self, dim: int, shape: Sequence[int], m: int, precision: str = "auto",
-               accum: str = "f32x2") -> str:
-        """'direct' or 'factored' for this grid, measuring its class on first use."""
-        if dim < 2 or min(shape) == 0:
-            return "direct"
-        key = self._key(dim, shape, m, precision, accum)
+    def choose(self, dim: int, shape: Sequence[int], m: int, field: str = "scalar",
+               precision: Optional[str] = None, accum: Optional[str] = None) -> dict:
+        """Fastest configuration for this grid: {"path", "precision", "accum",
+        "tile", "ms"} (precision / accum fixed when given)."""
+        if dim < 2 or min(shape) == 0 or m == 0:
+            return {"path": "direct", "ms": 0.0}
+        best, best_ms = None, None
+        for k in self.candidates(dim):
+            c = parse_config(k)
+            if c["path"] == "factored" and ((precision not in (None, "auto") and
+                                             c["precision"] != precision) or
+                                            (accum is not None and c["accum"] != accum)):
+                continue
+            t = self.estimate(dim, shape, m, k, field)
+            if best_ms is None or t < best_ms:
+                best, best_ms = k, t
+        out = parse_config(best)
+        out["ms"] = best_ms
+        return out
+
+    def ensemble_class_times(self, dim: int, shape: Sequence[int], m: int, precision: str,
+                             accum: str) -> dict:
+        """ms per realisation of the class grid inside a persistent plan, for
+        the direct kernel and the factored path with each tile shape (one
+        format and accumulation), measured on first use and cached."""
+        import numpy as np
+
+        from .ensemble import Ensemble
+
+        key = f"ens|{self._key(dim, shape, m, 'scalar')}|{precision}|{accum}"
         hit = self.table.get(key)
         if hit is None:
-            ms = self.measure(dim, shape, m, precision, accum)
-            hit = {"path": min(ms, key=ms.get), "ms": ms}
+            cls, mb = shape_class(shape, m)
+            npm = float(np.prod(cls)) * mb
+            rng = np.random.default_rng(12345)
+            cov = rng.normal(0.0, 8.0 / max(cls), size=(dim, mb))
+            z1, z2 = rng.normal(size=mb), rng.normal(size=mb)
+            axes = [np.arange(n, dtype=np.float64) for n in cls]
+            times = {}
+            for cfg in ["direct"] + [config_key("factored", precision, accum, t)
+                                     for t in self.tiles]:
+                c = parse_config(cfg)
+                # the direct kernel is ~10x slower: fewer copies keep it short
+                target = 2e9 if cfg == "direct" else 2e10
+                k = int(min(64, max(1, round(target / npm))))
+                ens = Ensemble([(cov, z1, z2, axes)] * k, paths=[c["path"]] * k,
+                               tiles=[c.get("tile", "256")] * k, precision=precision,
+                               accum=accum, out_dtype=np.float32)
+                try:
+                    times[cfg] = ens.time(self.repeats) / k
+                finally:
+                    ens.close()
+            hit = {"class": [list(cls), mb], "ms": times}
             self.table[key] = hit
+            self.measured += 1
             self.save()
-        return hit["path"]
+        return hit
+
+    def plan(self, dim: int, shapes: Sequence[Sequence[int]], ms: Sequence[int],
+             precision: Optional[str] = None, accum: Optional[str] = None,
+             field: str = "scalar") -> Tuple[str, str, List[str], List[str]]:
+        """Ensemble plan: the (format, accumulation) pair that minimises the
+        summed estimated time over the realisations (one pair per plan), and
+        for every realisation its path and tile shape under that pair, from
+        per-class times measured inside persistent plans."""
+        pairs = [(p, a) for p in self.formats for a in self.accums
+                 if precision in (None, "auto", p) and accum in (None, a)]
+        if not pairs:
+            pairs = [(precision, accum or "f32x2")]
+        best = None
+        for p, a in pairs:
+            total, paths, tiles = 0.0, [], []
+            for sh, m in zip(shapes, ms):
+                if dim < 2 or min(sh) == 0 or m == 0:
+                    paths.append("direct")
+                    tiles.append("256")
+                    continue
+                hit = self.ensemble_class_times(dim, sh, m, p, a)
+                cls, mb = hit["class"]
+                est = {k: t * padded_work(sh, m, k) / padded_work(cls, mb, k)
+                       for k, t in hit["ms"].items()}
+                k = min(est, key=est.get)
+                total += est[k]
+                c = parse_config(k)
+                paths.append(c["path"])
+                tiles.append(c.get("tile", "256"))
+            if best is None or total < best[0]:
+                best = (total, p, a, paths, tiles)
+        return best[1], best[2], best[3], best[4]
 
     def save(self) -> None:
         try:

@@ -93,6 +93,17 @@ typedef enum gsc_direct_variant {
   GSC_DIRECT_LOCAL = 2  /* cell-local phases from split-bf16 matrix-core products (default) */
 } gsc_direct_variant;
 
+/* Block tile of the factored path's GEMM: 256 rows (4 waves of 64 rows, one
+ * wave per SIMD) or 128 rows (4 waves of 32 rows, two blocks per CU: one
+ * block's waits are covered by the other's matrix instructions; less row
+ * padding on small grids).  Columns: 128 (fp16x3 / bf16x3), 64 (bf16x6),
+ * 32 (FP64 flush). */
+typedef enum gsc_tile {
+  GSC_TILE_AUTO = 0,  /* library default (256 rows) */
+  GSC_TILE_256 = 1,
+  GSC_TILE_128 = 2
+} gsc_tile;
+
 typedef struct gsc_opts {
   int32_t out_dtype;       /* gsc_dtype of `out` (default GSC_F64, as the reference) */
   int32_t path;            /* gsc_path, structured entry points only */
@@ -104,6 +115,7 @@ typedef struct gsc_opts {
   int32_t blocks_per_cu;   /* persistent factored-GEMM blocks per CU in ensembles, 0 = auto
                               (2 with GSC_ACCUM_F64, else 1) */
   double scale;            /* multiplies every output value (1.0 = reference) */
+  int32_t tile;            /* gsc_tile, factored path only */
 } gsc_opts;
 
 /* Fills *opts with the defaults: f64 output, auto path/precision, device -1, scale 1. */
@@ -208,6 +220,20 @@ int gsc_summate_fourier_dev(int dim, int64_t M, int64_t N, const double* sf, con
                             void* stream);
 
 /*
+ * Phase timing of the device summators (no counterpart in the reference):
+ * runs the *_dev call of `field` (a gsc_field: scalar, incompressible with
+ * out_ld, or Fourier with sf) `repeats` times on `stream`, with HIP events
+ * before it, between the mode preparation and cell sort and the summation
+ * kernel, and after it; returns the mean ms of the preparation and sort in
+ * ms[0], of the summation kernel in ms[1] and of the whole call in ms[2].
+ */
+int gsc_direct_phase_times(int field, int dim, int64_t M, int64_t N, const double* sf,
+                           const double* cov, const double* z1, const double* z2,
+                           const double* pos, int64_t pos_ld, void* out, int64_t out_ld,
+                           void* work, size_t work_bytes, const gsc_opts* opts, void* stream,
+                           int repeats, double* ms);
+
+/*
  * Structured grids.  The field on the C-order grid axes[0] x ... x
  * axes[dim-1] (shape[i] coordinates on axis i, dim in {1, 2, 3}) equals
  * gsc_summate on the expanded point list
@@ -272,6 +298,16 @@ int gsc_summate_grid_field_dev(int field, int dim, int64_t M, const double* sf,
                                void* stream);
 
 /*
+ * Autotuner support (no counterpart in the reference): the mean time in ms
+ * of one gsc_summate_grid_field_dev call with these sizes and options, over
+ * `repeats` calls after one warm-up call, from HIP events on a private
+ * stream.  The library allocates its own synthetic inputs (unit-spaced axes,
+ * seeded Gaussian modes, spectrum factors of one) and frees them.
+ */
+int gsc_tune_grid_time(int field, int dim, int64_t M, const int64_t* shape,
+                       const gsc_opts* opts, int repeats, double* ms);
+
+/*
  * Ensembles of realisations on structured grids (no counterpart in the
  * reference, whose callers loop over realisations in Python and call summate
  * once per field).  `count` realisations of one dimension dim in {2, 3};
@@ -295,6 +331,9 @@ int gsc_ensemble_create(gsc_ensemble** plan, int count, int dim, const int64_t* 
                         const double* cov, const double* z1, const double* z2,
                         const double* axes, void* out, const gsc_opts* opts);
 int gsc_ensemble_run(gsc_ensemble* plan, void* stream);
+/* Mean ms of one replay of the plan over `repeats` replays after a warm-up
+ * replay, from HIP events on a private stream (autotuner support). */
+int gsc_ensemble_time(gsc_ensemble* plan, int repeats, double* ms);
 /* n_factored / n_direct realisations, workspace bytes, total point-modes. */
 int gsc_ensemble_info(const gsc_ensemble* plan, int64_t* n_factored, int64_t* n_direct,
                       size_t* workspace_bytes, double* point_modes);
@@ -335,6 +374,20 @@ int gsc_ensemble_create_seeded(gsc_ensemble** plan, int count, int dim, const in
                                const gsc_opts* opts);
 int gsc_ensemble_reseed(gsc_ensemble* plan, const uint64_t* seeds, void* stream);
 
+/* Either creator with a GEMM tile shape per realisation (gsc_tile, e.g. the
+ * autotuner's choice for its shape class): tiles[r] for r < count, or NULL
+ * (opts->tile for all).  The factored realisations of each tile shape share
+ * one persistent GEMM launch of the plan's graph.  seeds == NULL: the modes
+ * come from cov, z1, z2 (model and len_scales unused), as
+ * gsc_ensemble_create; otherwise they are drawn as gsc_ensemble_create_seeded
+ * does (cov, z1, z2 unused).  No counterpart in the reference. */
+int gsc_ensemble_create_tuned(gsc_ensemble** plan, int count, int dim, const int64_t* shapes,
+                              const int64_t* m_counts, const int32_t* paths, const int32_t* tiles,
+                              const double* scales, const double* cov, const double* z1,
+                              const double* z2, int model, const double* len_scales,
+                              const uint64_t* seeds, const double* axes, void* out,
+                              const gsc_opts* opts);
+
 /*
  * Kriging calculators, n conditions and N targets:
  *   krig_fac_i(j) = sum_k mat[k][i] vecs[k][j]
@@ -361,6 +414,14 @@ size_t gsc_krige_workspace_bytes(int64_t n, int64_t N, int with_error);
 int gsc_krige_dev(int64_t n, int64_t N, const double* mat, const double* vecs, int64_t vecs_ld,
                   const double* cond, double* field, double* error, void* work,
                   size_t work_bytes, const gsc_opts* opts, void* stream);
+
+/* Roofline calibration of the direct kernels (no counterpart in the
+ * reference): the v_sin_f32 rate the current device sustains, alone
+ * (*sin_per_s) and paired with one dependent FP32 FMA per sine
+ * (*sin_fma_per_s), from HIP events over `repeats` launches of a kernel with
+ * eight independent chains per lane and eight waves per SIMD.  Returns a
+ * gsc_status (no message is set). */
+int gsc_calibrate_sine(int repeats, double* sin_per_s, double* sin_fma_per_s);
 
 /* Device memory plumbing for hosts without their own allocator (e.g. plain
  * numpy callers of the device-resident entry points). */

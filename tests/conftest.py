@@ -16,6 +16,15 @@ for p in (ROOT, PKG):
 
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 
+# The GPU tests share device buffers with torch: importing it before the
+# library's first call puts both on torch's HIP runtime (gstools_core binds to
+# the runtime already loaded).  tests/test_gpu_system_runtime.py runs the
+# library without torch, on the system runtime, in a child process.
+try:
+    import torch  # noqa: F401
+except ImportError:  # the CPU tier does not need it
+    pass
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs a gfx950 GPU (runs on the MI355X box)")

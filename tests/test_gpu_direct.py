@@ -423,3 +423,40 @@ def test_local_rough_modes_keep_fp64_phase():
     err = _err(gi, oracle.summate_incompr(cov, z1, z2, pos)) / np.sqrt(m)
     print(f"mixed rough/smooth, incompressible: {err:.3e}")
     assert err <= 1e-5
+
+
+@pytest.mark.parametrize("n", [3000, 1_000_000])
+def test_incompr_output_layouts_through_the_c_abi(n):
+    # the reference's Fortran-order (d, N) result is written by a device
+    # transpose and one flat copy; C order and a reversed view take the
+    # strided scatter; small calls use one pageable copy, large ones the
+    # pinned chunks.  All layouts hold the same values.
+    import ctypes
+    from gstools_core import _native
+    w = workloads.c2(n=n)
+    md = w.modes
+    cov = np.ascontiguousarray(md.cov_samples)
+    opts = _native.make_opts()
+    lib = _native.lib()
+
+    def call(out, sd, sp):
+        rc = lib.gsc_summate_incompr(3, md.m, n, cov.ctypes.data, md.m, 1, md.z1.ctypes.data, 1,
+                                     md.z2.ctypes.data, 1, w.pos.ctypes.data, n, 1,
+                                     out.ctypes.data, sd, sp, ctypes.byref(opts))
+        _native.check(rc)
+
+    f = np.empty((3, n), order="F")
+    call(f, 1, 3)
+    c = np.empty((3, n))
+    call(c, n, 1)
+    np.testing.assert_array_equal(c, f)
+    big = np.zeros((3, 2 * n))
+    view = big[:, ::-2]  # negative point stride into a larger buffer
+    rc = lib.gsc_summate_incompr(3, md.m, n, cov.ctypes.data, md.m, 1, md.z1.ctypes.data, 1,
+                                 md.z2.ctypes.data, 1, w.pos.ctypes.data, n, 1,
+                                 view.ctypes.data, 2 * n, -2, ctypes.byref(opts))
+    _native.check(rc)
+    np.testing.assert_array_equal(view, f)
+    assert not big[:, 0::2].any()
+    np.testing.assert_array_equal(gstools_core.summate_incompr(md.cov_samples, md.z1, md.z2,
+                                                               w.pos), f)

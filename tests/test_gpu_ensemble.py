@@ -179,21 +179,56 @@ def test_ensemble_zero_mode_realisation():
     ens.close()
 
 
-def test_tuner_chooses_and_caches(tmp_path):
-    from gstools_core.tuner import Tuner
-    t = Tuner(path=str(tmp_path / "tune.json"), repeats=2)
-    p1 = t.choose(2, (300, 400), 1000, "fp16x3", "f64")
-    p2 = t.choose(2, (8, 9), 50, "fp16x3", "f64")
-    assert p1 in ("direct", "factored") and p2 in ("direct", "factored")
-    assert p1 == "factored"  # a 512 x 512 class at M = 1024 favours the MFMA path
+def test_tuner_measures_chooses_and_caches(tmp_path):
+    from gstools_core.tuner import Tuner, padded_work
+    t = Tuner(path=str(tmp_path / "tune.json"), repeats=3)
+    big = t.choose(2, (300, 400), 1000)
+    small = t.choose(2, (8, 9), 50)
+    assert t.measured == 2
+    for c in (big, small):
+        assert c["path"] in ("direct", "factored") and c["ms"] > 0
+        if c["path"] == "factored":
+            assert c["precision"] in ("fp16x3", "bf16x6") and c["tile"] in ("256", "128")
+    print("chosen", big, small)
+    assert big["path"] == "factored"  # a 512 x 512 class at M = 1024 favours the MFMA path
+    table = t.class_times(2, (300, 400), 1000)
+    assert len(table["ms"]) == 9 and table["best"] in table["ms"]
+    # a fixed format is honoured; a second tuner reads the cache without measuring
+    c = t.choose(2, (300, 400), 1000, precision="bf16x6", accum="f64")
+    assert c["path"] == "direct" or (c["precision"], c["accum"]) == ("bf16x6", "f64")
     t2 = Tuner(path=str(tmp_path / "tune.json"))
-    assert t2.choose(2, (300, 400), 1000, "fp16x3", "f64") == p1  # from the cache
-    reals = _reals(2, [(300, 400), (8, 9)], [1000, 50], 13)
+    assert t2.choose(2, (300, 400), 1000) == big and t2.measured == 0
+    # padding model: a grid inside a class never has more padded work than it
+    for key in table["ms"]:
+        assert padded_work((300, 400), 1000, key) <= padded_work((512, 512), 1024, key)
+    reals = _reals(2, [(300, 400), (8, 9), (64, 70)], [1000, 50, 300], 13)
     ens = Ensemble(reals, tune=True, tuner=t2, out_dtype=np.float64)
+    assert set(ens.tuned) == {"precision", "accum", "paths", "tiles"}
     ens.run()
     for r, (cov, z1, z2, axes) in zip(ens.results(), reals):
         ref = oracle.summate(cov, z1, z2, oracle.grid_points(axes)).reshape(r.shape)
         assert np.abs(r - ref).max() / np.sqrt(cov.shape[1]) <= 1e-5
+    ens.close()
+
+
+def test_mixed_tile_plan_matches_single_grid_calls():
+    # one plan with both GEMM tile shapes (two persistent launches) and direct
+    # realisations: each realisation equals its single-grid call with its tile
+    rng = np.random.default_rng(88)
+    count = 16
+    shapes = [tuple(int(v) for v in rng.integers(20, 300, size=2)) for _ in range(count)]
+    ms = [int(v) for v in rng.integers(16, 400, size=count)]
+    reals = _reals(2, shapes, ms, 89)
+    tiles = ["128" if i % 3 else "256" for i in range(count)]
+    paths = ["direct" if i % 5 == 4 else "factored" for i in range(count)]
+    ens = Ensemble(reals, paths=paths, tiles=tiles, out_dtype=np.float64, precision="fp16x3")
+    for rep in range(2):
+        ens.run()
+        for i, (x, (cov, z1, z2, axes)) in enumerate(zip(ens.results(), reals)):
+            y = grid.summate_grid(cov, z1, z2, axes, path=paths[i], precision="fp16x3",
+                                  tile=tiles[i])
+            np.testing.assert_allclose(x, y, rtol=0, atol=1e-9 * np.sqrt(ms[i]),
+                                       err_msg=f"replay {rep} realisation {i}")
     ens.close()
 
 
@@ -202,3 +237,26 @@ def test_device_buffers_roundtrip():
     b = _native.DeviceBuffer.from_array(a)
     np.testing.assert_array_equal(b.to_array(np.float64, 1000), a)
     b.free()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_ensemble_128_row_tiles_match_single_grid_calls(dim):
+    rng = np.random.default_rng(60 + dim)
+    count = 24
+    if dim == 2:
+        shapes = [tuple(int(v) for v in rng.integers(17, 400, size=2)) for _ in range(count)]
+    else:
+        shapes = [tuple(int(v) for v in rng.integers(17, 60, size=3)) for _ in range(count)]
+    ms = [int(v) for v in rng.integers(16, 500, size=count)]
+    reals = _reals(dim, shapes, ms, 70 + dim)
+    for accum in ("f32x2", "f64"):
+        ens = Ensemble(reals, paths=["factored"] * count, out_dtype=np.float64,
+                       precision="fp16x3", tile="128", accum=accum)
+        singles = [grid.summate_grid(cov, z1, z2, axes, path="factored", precision="fp16x3",
+                                     tile="128", accum=accum) for cov, z1, z2, axes in reals]
+        for rep in range(2):
+            ens.run()
+            for i, (x, y) in enumerate(zip(ens.results(), singles)):
+                np.testing.assert_allclose(x, y, rtol=0, atol=1e-9 * np.sqrt(ms[i]),
+                                           err_msg=f"{accum} replay {rep} realisation {i}")
+        ens.close()

@@ -41,12 +41,21 @@ def test_factored_small_grids(shape, precision):
     d = len(shape)
     cov, z1, z2 = _modes(d, 150, 4.0, 1000 + sum(shape))
     axes = [np.arange(n) * 0.75 - 3.0 for n in shape]
-    g = grid.summate_grid(cov, z1, z2, axes, path="factored", precision=precision)
-    assert g.shape == shape
     ref = oracle.summate(cov, z1, z2, oracle.grid_points(axes)).reshape(shape)
-    err = _scaled_err(g, ref, 150)
-    print(f"{shape} {precision}: {err:.3e}")
-    assert err <= TOL[precision]
+    res = {}
+    for tile in ("256", "128"):
+        for accum in ("f32x2", "f64"):
+            g = grid.summate_grid(cov, z1, z2, axes, path="factored", precision=precision,
+                                  tile=tile, accum=accum)
+            assert g.shape == shape
+            err = _scaled_err(g, ref, 150)
+            print(f"{shape} {precision} tile {tile} {accum}: {err:.3e}")
+            assert err <= TOL[precision]
+            res[tile, accum] = g
+    # both tile shapes against each other (they agree bit for bit when the
+    # plans keep the same 2-D orientation, within the format's error otherwise)
+    for accum in ("f32x2", "f64"):
+        assert _scaled_err(res["128", accum], res["256", accum], 150) <= 2 * TOL[precision]
 
 
 @pytest.mark.parametrize("shape", [(1000,), (37, 50), (5, 7, 200), (2, 3, 4)])
