@@ -141,7 +141,7 @@ __device__ __forceinline__ h8 a_fragment32(const float* rec, const float (&usc)[
   return __builtin_bit_cast(h8, v);
 }
 
-constexpr int kCellPts = 2048;      // target points per cell
+constexpr int kCellPts = 4096;      // target points per cell
 constexpr int kMinCellPts = 64;     // the cell count never exceeds N / kMinCellPts
 constexpr double kCellRev = 0.75;   // cells keep sum_k |kap_k| h_k / 2 below this (revolutions)
                                     // for all but the widest kBandSkip of the modes
@@ -185,7 +185,7 @@ constexpr size_t al256(size_t b) { return (b + 255) & ~static_cast<size_t>(255);
 
 struct BinLayout {
   int64_t max_cells, max_waves, nblk, blk_pts;
-  size_t params, box, centre, bcount, gsum, counts, offsets, wave_cell, recs, btpl, btpl32, amp, sm,
+  size_t params, box, centre, bcount, counts, offsets, wave_cell, recs, btpl, btpl32, amp, sm,
       rough, r0w, bytes;
   int64_t r0ld;  // row length of the (cell, mode) r0 table; 0: no table
 };
@@ -203,7 +203,6 @@ BinLayout bin_layout(int64_t M, int64_t N) {
   L.box = take(sizeof(double) * 6 * kBoxBlocks);
   L.centre = take(sizeof(double) * 4 * static_cast<size_t>(L.max_cells));
   L.bcount = take(sizeof(int) * static_cast<size_t>(L.nblk * L.max_cells));
-  L.gsum = take(sizeof(int) * static_cast<size_t>(kColGroups * L.max_cells));
   L.counts = take(sizeof(int) * static_cast<size_t>(L.max_cells));
   L.offsets = take(sizeof(int) * static_cast<size_t>(L.max_cells + 1));
 
@@ -605,49 +604,46 @@ k_bin_hist(const double* __restrict__ pos, int64_t ld, int64_t N, int64_t blk_pt
   for (int c = threadIdx.x; c < P.ncells; c += kSortThreads) row[c] = hist[c];
 }
 
-// Exclusive scan of every column of bcount over the point blocks, in three
-// passes parallel over (cell, group of kColBlocks blocks):
-//   sums of each group, scan of the group sums per cell (and the cell total),
-//   rescan of each group from its offset.
+// Exclusive scan of every column of bcount over the point blocks, and the
+// cell totals, in one launch: a block takes 16 cells x kColGroups groups of
+// kColBlocks point blocks; each thread sums its group, the 16 group sums of a
+// cell are scanned in LDS, and each thread rewrites its group from its offset.
+// Neighbouring threads take neighbouring cells, so every access to a row of
+// bcount is a run of 16 consecutive words.
+constexpr int kColCells = 256 / kColGroups;
+static_assert(kColCells * kColGroups == 256, "column scan block shape");
 __global__ void __launch_bounds__(256)
-k_col_sum(const BinParams* __restrict__ prm, int nblk, const int* __restrict__ bcount,
-          int* __restrict__ gsum) {
+k_col_scan(const BinParams* __restrict__ prm, int nblk, int* __restrict__ bcount,
+           int* __restrict__ counts) {
+  __shared__ int gs[kColCells][kColGroups + 1];
   const int ncells = prm->ncells;
-  const int c = blockIdx.x * 256 + threadIdx.x, g = blockIdx.y;
-  if (c >= ncells) return;
+  const int cl = threadIdx.x % kColCells, g = threadIdx.x / kColCells;
+  const int c = blockIdx.x * kColCells + cl;
+  const bool on = c < ncells;
+  const int b0 = g * kColBlocks, b1 = min(nblk, b0 + kColBlocks);
   int run = 0;
-  for (int b = g * kColBlocks; b < min(nblk, (g + 1) * kColBlocks); ++b)
-    run += bcount[static_cast<int64_t>(b) * ncells + c];
-  gsum[g * ncells + c] = run;
-}
-
-__global__ void __launch_bounds__(256)
-k_col_groups(const BinParams* __restrict__ prm, int ngroups, int* __restrict__ gsum,
-             int* __restrict__ counts) {
-  const int ncells = prm->ncells;
-  const int c = blockIdx.x * 256 + threadIdx.x;
-  if (c >= ncells) return;
-  int run = 0;
-  for (int g = 0; g < ngroups; ++g) {
-    const int v = gsum[g * ncells + c];
-    gsum[g * ncells + c] = run;
-    run += v;
+  if (on)
+    for (int b = b0; b < b1; ++b) run += bcount[static_cast<int64_t>(b) * ncells + c];
+  gs[cl][g] = run;
+  __syncthreads();
+  if (g == 0 && on) {
+    int sum = 0;
+    for (int q = 0; q < kColGroups; ++q) {
+      const int v = gs[cl][q];
+      gs[cl][q] = sum;
+      sum += v;
+    }
+    counts[c] = sum;
   }
-  counts[c] = run;
-}
-
-__global__ void __launch_bounds__(256)
-k_col_fix(const BinParams* __restrict__ prm, int nblk, int* __restrict__ bcount,
-          const int* __restrict__ gsum) {
-  const int ncells = prm->ncells;
-  const int c = blockIdx.x * 256 + threadIdx.x, g = blockIdx.y;
-  if (c >= ncells) return;
-  int run = gsum[g * ncells + c];
-  for (int b = g * kColBlocks; b < min(nblk, (g + 1) * kColBlocks); ++b) {
-    int* e = bcount + static_cast<int64_t>(b) * ncells + c;
-    const int v = *e;
-    *e = run;
-    run += v;
+  __syncthreads();
+  if (on) {
+    run = gs[cl][g];
+    for (int b = b0; b < b1; ++b) {
+      int* e = bcount + static_cast<int64_t>(b) * ncells + c;
+      const int v = *e;
+      *e = run;
+      run += v;
+    }
   }
 }
 
@@ -734,25 +730,44 @@ __device__ __forceinline__ void cell_centre(const BinParams& P, int e, double (&
 }
 
 // One wave per cell: records its centre and its waves in the wave -> cell
-// map and marks the slots past its points as padding.
+// map, marks the slots past its points as padding and, when r0w is given,
+// writes the cell's row of r0 words (r0_hi | r0_lo << 16, the fp16 parts of
+// the reduced FP64 phase at the centre) for every smooth mode up to the end
+// of the last chunk (zero phase data past n_smooth), which k_direct_local32
+// copies instead of evaluating r0 in every wave of the cell.
 template <int D>
 __global__ void __launch_bounds__(256)
 k_bin_waves(const BinParams* __restrict__ prm, const int* __restrict__ counts,
             const int* __restrict__ offsets, double* __restrict__ centre,
-            int* __restrict__ wave_cell, float4* __restrict__ recs) {
+            int* __restrict__ wave_cell, float4* __restrict__ recs,
+            const SmoothRec* __restrict__ sm, uint32_t* __restrict__ r0w, int64_t r0ld) {
   const int e = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   const BinParams P = *prm;
   if (e >= P.ncells) return;
-  if (lane == 0) {
-    double X[3];
-    cell_centre<D>(P, e, X);
+  double X[3];
+  cell_centre<D>(P, e, X);
+  if (lane == 0)
 #pragma unroll
     for (int k = 0; k < 3; ++k) centre[4 * e + k] = X[k];
-  }
   const int s0 = offsets[e], s1 = offsets[e + 1];
   for (int s = s0 + kWavePts * lane; s < s1; s += kWavePts * 64) wave_cell[s / kWavePts] = e;
   const int sp = s0 + counts[e] + lane;  // padding: fewer than kWavePts slots
   if (sp < s1) recs[sp] = make_rec(0.0f, 0.0f, 0.0f, -1);
+  if (r0w == nullptr || s1 == s0) return;  // no table, or no wave reads this row
+  uint32_t* row = r0w + e * r0ld;
+  const int end = (P.n_smooth + kModeChunk - 1) / kModeChunk * kModeChunk;
+#pragma unroll 4
+  for (int i = lane; i < end; i += 64) {
+    const SmoothRec sr = sm[i];
+    double t = __builtin_fma(sr.kap[0], X[0], sr.th);
+    t = __builtin_fma(sr.kap[1], X[1], t);
+    if constexpr (D == 3) t = __builtin_fma(sr.kap[2], X[2], t);
+    float r0 = static_cast<float>(t - rint(t));
+    __asm__ volatile("" : "+v"(r0));  // FP32 rounding step kept (see k_direct_local32)
+    uint16_t rh, rl;
+    split2h(r0, rh, rl);
+    row[i] = static_cast<uint32_t>(rh) | (static_cast<uint32_t>(rl) << 16);
+  }
 }
 
 // Point block b again (same threads and rounds as k_bin_hist): ranks its
@@ -810,36 +825,6 @@ k_bin_scatter(const double* __restrict__ pos, int64_t ld, int64_t N, int64_t blk
   }
 }
 
-// r0 word (r0_hi | r0_lo << 16, the fp16 parts of the reduced FP64 phase at
-// the cell centre) of every (cell, smooth mode) pair, up to the end of the
-// last chunk (zero phase data past n_smooth): computed once per cell here instead
-// of once per wave of the cell in k_direct_local32.  One block per cell.
-template <int D>
-__global__ void __launch_bounds__(256)
-k_cell_r0(const BinParams* __restrict__ prm, const double* __restrict__ centre,
-          const SmoothRec* __restrict__ sm, uint32_t* __restrict__ r0w, int64_t r0ld) {
-  const BinParams P = *prm;
-  const int cell = blockIdx.x;
-  if (cell >= P.ncells) return;
-  const double* xc = centre + 4 * static_cast<int64_t>(cell);
-  const double X0 = xc[0], X1 = xc[1], X2 = xc[2];
-  uint32_t* row = r0w + cell * r0ld;
-  // the chunks k_direct_local32 copies: n_smooth rounded up to whole chunks
-  const int end = (P.n_smooth + kModeChunk - 1) / kModeChunk * kModeChunk;
-#pragma unroll 2
-  for (int i = threadIdx.x; i < end; i += 256) {
-    const SmoothRec sr = sm[i];
-    double t = __builtin_fma(sr.kap[0], X0, sr.th);
-    t = __builtin_fma(sr.kap[1], X1, t);
-    if constexpr (D == 3) t = __builtin_fma(sr.kap[2], X2, t);
-    float r0 = static_cast<float>(t - rint(t));
-    __asm__ volatile("" : "+v"(r0));  // FP32 rounding step kept (see k_direct_local32)
-    uint16_t rh, rl;
-    split2h(r0, rh, rl);
-    row[i] = static_cast<uint32_t>(rh) | (static_cast<uint32_t>(rl) << 16);
-  }
-}
-
 // ---- main kernel -------------------------------------------------------------
 // 32 x 32 form of the local kernel: one v_mfma_f32_32x32x16_f16 gives the
 // phases of 32 points x 32 modes (the fp16 product above), about half the
@@ -892,7 +877,10 @@ k_direct_local32(const BinParams* __restrict__ prm, const double* __restrict__ c
 #pragma unroll
   for (int g = 0; g < 2; ++g) a[g] = a_fragment32<D>(rf + 4 * (32 * g + lr), usc, hl);
   uint2* wt = reinterpret_cast<uint2*>(lds[wv]);
-  const unsigned mu = hl ? ~0u : 0u;  // upper half-wave: r0 slots from the table
+  // upper half-wave: r0 slots from the table (opaque to the compiler, so the
+  // merge below stays one v_and_or_b32 instead of a select and an or)
+  unsigned mu = hl ? ~0u : 0u;
+  __asm__ volatile("" : "+v"(mu));
   f16x zero;
 #pragma unroll
   for (int v = 0; v < 16; ++v) zero[v] = 0.0f;
@@ -939,6 +927,8 @@ k_direct_local32(const BinParams* __restrict__ prm, const double* __restrict__ c
     // most 63 modes past n_smooth, inside the zeroed padding); table records
     // of the next tile clamped to the chunk
     const int ntile = min(kModeChunk / 32, (n_smooth - c0 + 31) / 32);
+    // per-mode B table: a wave-uniform tile base (scalar registers) plus this
+    // lane's fixed offset, so the loop advances the base with scalar adds
     const uint4* bp = btpl32 + 2 * c0;
     const unsigned bo = 2u * lr + hl;
     auto fragment = [&](const uint4& bq, const uint2& wq) {
@@ -955,15 +945,17 @@ k_direct_local32(const BinParams* __restrict__ prm, const double* __restrict__ c
       t0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[0], b, zero, 0, 0, 0);
       t1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[1], b, zero, 0, 0, 0);
     }
+    // the table of the tile after the last is read and never used: its
+    // entries 32 ntile + lr stay below 32 (kModeChunk / 32) + 32 <= kWaveLds / 8
+    static_assert(32 * (kModeChunk / 32) + 32 <= kWaveLds / 8, "prefetch inside the table");
     bq = bp[bo + 64];
-    wq = wt[min(32, kModeChunk - 32) + lr];
-    bp += 128;
+    wq = wt[32 + lr];
+#pragma unroll 2
     for (int q = 0; q + 1 < ntile; ++q) {
       const h8 b = fragment(bq, wq);
       const float am_next = __builtin_bit_cast(float, wq.y);
-      bq = bp[bo];
-      bp += 64;
-      wq = wt[min(32 * (q + 2), kModeChunk - 32) + lr];
+      bq = bp[64 * (q + 2) + bo];
+      wq = wt[32 * (q + 2) + lr];
 #pragma unroll
       for (int v = 0; v < 16; ++v)
         acc[0][v] = __builtin_fmaf(am, __builtin_amdgcn_sinf(t0[v]), acc[0][v]);
@@ -1233,7 +1225,6 @@ hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table
   double* box = at<double>(bin_work, L.box);
   double* centre = at<double>(bin_work, L.centre);
   int* bcount = at<int>(bin_work, L.bcount);
-  int* gsum = at<int>(bin_work, L.gsum);
   int* counts = at<int>(bin_work, L.counts);
   int* offsets = at<int>(bin_work, L.offsets);
   int* wave_cell = at<int>(bin_work, L.wave_cell);
@@ -1247,8 +1238,7 @@ hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table
   uint32_t* r0w = L.r0ld > 0 ? at<uint32_t>(bin_work, L.r0w) : nullptr;
   const ModeRec* modes = static_cast<const ModeRec*>(table);
   const int nblk = static_cast<int>(L.nblk);
-  const int ngroups = (nblk + kColBlocks - 1) / kColBlocks;
-  const unsigned cb = static_cast<unsigned>((L.max_cells + 255) / 256);
+  const unsigned cb = static_cast<unsigned>((L.max_cells + kColCells - 1) / kColCells);
   const unsigned wb = static_cast<unsigned>((L.max_cells + 3) / 4);  // a wave per cell
   const unsigned mb = static_cast<unsigned>((L.max_waves + 3) / 4);
   const unsigned vb = static_cast<unsigned>((2 * L.max_waves + 3) / 4);
@@ -1260,16 +1250,12 @@ hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table
     k_bin_params<D><<<1, 256, 0, s>>>(box, N, L.max_cells, modes, Mi, prm, btpl, btpl32, ampt,   \
                                       sm, rough);                                                \
     k_bin_hist<D><<<nblk, kSortThreads, 0, s>>>(pos, pos_ld, N, L.blk_pts, prm, bcount);         \
-    k_col_sum<<<dim3(cb, ngroups), 256, 0, s>>>(prm, nblk, bcount, gsum);                        \
-    k_col_groups<<<cb, 256, 0, s>>>(prm, ngroups, gsum, counts);                                 \
-    k_col_fix<<<dim3(cb, ngroups), 256, 0, s>>>(prm, nblk, bcount, gsum);                        \
+    k_col_scan<<<cb, 256, 0, s>>>(prm, nblk, bcount, counts);                                    \
     k_bin_offsets<<<1, kSortThreads, 0, s>>>(prm, counts, offsets);                              \
-    k_bin_waves<D><<<wb, 256, 0, s>>>(prm, counts, offsets, centre, wave_cell, recs);            \
+    k_bin_waves<D><<<wb, 256, 0, s>>>(prm, counts, offsets, centre, wave_cell, recs, sm,         \
+                                      vec ? nullptr : r0w, L.r0ld);                              \
     k_bin_scatter<D><<<nblk, kSortThreads, 0, s>>>(pos, pos_ld, N, L.blk_pts, prm, bcount,       \
                                                    offsets, recs);                               \
-    if (!vec && r0w != nullptr)                                                                  \
-      k_cell_r0<D><<<static_cast<unsigned>(L.max_cells), 256, 0, s>>>(prm, centre, sm, r0w,      \
-                                                                      L.r0ld);                   \
     if (split) (void)hipEventRecord(split, s);                                                   \
     if (vec && f32)                                                                              \
       k_direct_local_vec<D, float><<<vb, 256, 0, s>>>(prm, centre, wave_cell, recs, btpl, ampt,  \

@@ -278,9 +278,16 @@ k_tables(const ModeRec* __restrict__ modes, int M, int dim, int nch,
 //      and reads chunk c + 1's ks = 0 B fragments,
 //   4. issues chunk c's ks = 1 MFMAs.
 // ---------------------------------------------------------------------------
-template <typename T, int NS, int JN>
+// B tile slots: 256-row blocks (RB = 2) build the tile two chunks ahead into
+// four slots and synchronise once per two chunks; 128-row blocks (two per CU,
+// so the LDS is shared) one chunk ahead into two slots, one barrier per chunk.
+template <int RB>
+constexpr int b_slots() {
+  return RB == 2 ? 4 : 2;
+}
+template <typename T, int NS, int JN, int SLOTS>
 struct GemmSmem {
-  T B[2][NS][32 * JN][kLdB];
+  T B[SLOTS][NS][32 * JN][kLdB];
 };
 
 struct TileJob {
@@ -342,7 +349,8 @@ __device__ __forceinline__ void gemm_store(OutT* __restrict__ out, const TileJob
 }
 
 template <typename T, int NS, bool KR, int ACC, int JN, int RB, bool TR = false>
-__device__ __forceinline__ void gemm_tile(GemmSmem<T, NS, JN>& sm, const TileJob& jb) {
+__device__ __forceinline__ void gemm_tile(GemmSmem<T, NS, JN, b_slots<RB>()>& sm,
+                                          const TileJob& jb) {
   using V8 = typename Fmt<T>::v8;
   using A2 = typename AccT<ACC>::type;
   constexpr int GC = JN == 4 ? 2 : 1;  // B columns built per lane
@@ -507,6 +515,12 @@ __device__ __forceinline__ void gemm_tile(GemmSmem<T, NS, JN>& sm, const TileJob
     __syncthreads();
     gload(c + 2 < nch ? c + 2 : cn);
   };
+  // RB = 2: chunk c's fragments come from slot c % 4; the step builds chunk
+  // c + 2 into slot (c + 2) % 4, and odd steps end with the barrier that
+  // publishes the two chunks built since the last one.  Slot (c + 2) % 4 held
+  // chunk c - 2, whose last reads precede that barrier (see the order below:
+  // ks = 1 fragments at the start of a step, the next chunk's ks = 0
+  // fragments at its end).
   auto step = [&](int c, const int set) {
     if constexpr (RB == 1) {
       step1(c, set);
@@ -514,11 +528,11 @@ __device__ __forceinline__ void gemm_tile(GemmSmem<T, NS, JN>& sm, const TileJob
     }
     const int cn = c + 1 < nch ? c + 1 : c;
     aload(set ^ 1, cn);
-    bread(1, set);
+    bread(1, c & 3);
     mfma_ks(set, 0);
-    // at c = nch - 1 this rebuilds the last chunk into the buffer of chunk
-    // nch - 2, which no wave reads again
-    gstore(set ^ 1);
+    // past the last chunk this rebuilds the last one into a slot that no
+    // wave reads again
+    gstore((c + 2) & 3);
 #pragma unroll
     for (int k = 0; k < kM; ++k) {
       __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
@@ -528,9 +542,9 @@ __device__ __forceinline__ void gemm_tile(GemmSmem<T, NS, JN>& sm, const TileJob
       if (k >= kM - kW) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);      // DS write
     }
     __builtin_amdgcn_sched_barrier(0);
-    __syncthreads();
-    gload(c + 2 < nch ? c + 2 : cn);
-    bread(0, set ^ 1);
+    if (set == 1) __syncthreads();  // set = c % 2, fixed by the unrolled pair below
+    gload(c + 3 < nch ? c + 3 : nch - 1);
+    bread(0, (c + 1) & 3);
     mfma_ks(set, 1);
 #pragma unroll
     for (int k = 0; k < kM; ++k) {
@@ -545,6 +559,10 @@ __device__ __forceinline__ void gemm_tile(GemmSmem<T, NS, JN>& sm, const TileJob
   gload(0);
   gstore(0);
   gload(nch > 1 ? 1 : 0);
+  if constexpr (RB == 2) {
+    gstore(1);
+    gload(nch > 2 ? 2 : nch - 1);
+  }
   aload(0, 0);
   __syncthreads();
   if constexpr (RB == 2) bread(0, 0);
@@ -622,7 +640,7 @@ __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(gemm_waves<ACC, RB>(), gemm_waves<ACC, RB>())))
 k_factored_gemm(GemmArgs g) {
   constexpr int JN = cols_blocks<NS, ACC>();
-  __shared__ __attribute__((aligned(16))) GemmSmem<T, NS, JN> sm;
+  __shared__ __attribute__((aligned(16))) GemmSmem<T, NS, JN, b_slots<RB>()> sm;
   const int64_t tpc = (g.ncols + 32 * JN - 1) / (32 * JN);
   const TileJob jb = make_job<KR, JN, RB>(g.A, g.arows, g.nch, g.S, g.na, g.C, g.ncpt, g.nrow,
                                           g.nb_out, g.out, g.out_f64, g.ldo, g.ncols, g.scale,
@@ -725,7 +743,7 @@ k_ens_gemm(const EnsReal* __restrict__ d, const int* __restrict__ fact,
            const int64_t* __restrict__ tile_start, int nfact,
            const int64_t* __restrict__ group_start, int* __restrict__ counters, bool out_f64) {
   constexpr int JN = cols_blocks<NS, ACC>();
-  __shared__ __attribute__((aligned(16))) GemmSmem<T, NS, JN> sm;
+  __shared__ __attribute__((aligned(16))) GemmSmem<T, NS, JN, b_slots<RB>()> sm;
   __shared__ int64_t tile_s;
   const int home = blockIdx.x % kXcdGroups;
   for (int k = 0; k < kXcdGroups; ++k) {
