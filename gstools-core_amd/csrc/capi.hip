@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -1055,6 +1056,21 @@ void ensemble_release(gsc_ensemble* p) {
   delete p;
 }
 
+// True when x[i] = org + i step for every i < n within 8 units in the last
+// place of the axis' largest magnitude (np.arange and np.linspace axes): the
+// product form evaluates the coordinates from org and step, so a phase moves
+// by at most |k| 8 eps max|x| revolutions.
+bool uniform_axis(const double* x, int64_t n, double* org, double* step) {
+  if (n <= 0 || !std::isfinite(x[0]) || !std::isfinite(x[n - 1])) return false;
+  const double d = n > 1 ? (x[n - 1] - x[0]) / (double)(n - 1) : 0.0;
+  const double tol = 8.0 * DBL_EPSILON * std::max(std::fabs(x[0]), std::fabs(x[n - 1]));
+  for (int64_t i = 1; i + 1 < n; ++i)
+    if (!(std::fabs(x[i] - (x[0] + (double)i * d)) <= tol)) return false;
+  *org = x[0];
+  *step = d;
+  return std::isfinite(d);
+}
+
 // sample: null, or the mode sampler that fills cov / z1 / z2 at the start
 // of every replay (seeded plans).  tiles: per-realisation gsc_tile of the
 // factored ones, or null (opts->tile for all).
@@ -1070,6 +1086,15 @@ int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes
   std::vector<size_t> tab_off((size_t)count), f_off((size_t)count);
   std::vector<gsc::FactoredPlan> plans((size_t)count);
   const int tile_cols = gsc::gemm_tile_cols(gsc::precision_splits(o->precision), o->accum);
+  // 2-D: the axes on the host, to find the uniform ones (product form)
+  std::vector<double> hax;
+  if (dim == 2) {
+    int64_t total = 0;
+    for (int64_t i = 0; i < (int64_t)count * dim; ++i) total += shapes[i];
+    hax.resize((size_t)std::max<int64_t>(total, 1));
+    if (total > 0)
+      GSC_HIP(hipMemcpy(hax.data(), axes, sizeof(double) * (size_t)total, hipMemcpyDeviceToHost));
+  }
   // pass 1: geometry, offsets into the caller's arrays, path, workspace sizes
   int64_t m_off = 0, a_off = 0, o_off = 0;
   size_t ws = 0;
@@ -1121,25 +1146,51 @@ int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes
       e.row_tiles = fp.row_tiles;
       e.col_tiles = (fp.ncols + tile_cols - 1) / tile_cols;
       f_off[(size_t)r] = ws;  // S, A, C as laid out after the plan's own mode table
-      ws += align256(fp.bytes - fp.off_s);
+      const double* rax = dim == 2 ? hax.data() + (e.ax0 - axes) + (fp.trans ? sh[0] : 0) : nullptr;
+      const double* cax = dim == 2 ? hax.data() + (e.ax0 - axes) + (fp.trans ? 0 : sh[0]) : nullptr;
+      if (dim == 2 && uniform_axis(rax, fp.nrow, &e.org[0], &e.step[0]) &&
+          uniform_axis(cax, fp.ncols, &e.org[1], &e.step[1])) {
+        // product form: S per tile, A and C one tile
+        e.prod = 1;
+        e.tcols = tile_cols;
+        e.row_tiles = fp.arows / fp.brows;
+        e.na = e.row_tiles * e.col_tiles;
+        e.arows = fp.brows;
+        e.ncp = gsc::kMaxTileCols;
+        const size_t sb = align256(sizeof(float2) * (size_t)e.na * (size_t)fp.mp);
+        const size_t ab =
+            align256(sizeof(uint32_t) * 16 * (size_t)fp.ns * (size_t)fp.nch * (size_t)e.arows);
+        const size_t cb = align256(sizeof(float2) * (size_t)e.ncp * (size_t)fp.mp);
+        plans[(size_t)r].off_s = 0;  // offsets within the realisation's block
+        plans[(size_t)r].off_a = sb;
+        plans[(size_t)r].off_c = sb + ab;
+        ws += sb + ab + cb;
+      } else {
+        plans[(size_t)r].off_a -= fp.off_s;
+        plans[(size_t)r].off_c -= fp.off_s;
+        plans[(size_t)r].off_s = 0;
+        ws += align256(fp.bytes - fp.off_s);
+      }
       fact.push_back(r);
     } else {
       dir.push_back(r);
     }
     plan->point_modes += (double)e.npts * (double)M;
   }
-  // GEMM groups by tile shape (256-row tiles first); in each, longest first
-  // (most modes, then most tiles), then a longest-processing-time split into
+  // GEMM groups by tile shape (256-row tiles first) and operand form (tables
+  // of the whole grid, then the product form); in each, longest first (most
+  // modes, then most tiles), then a longest-processing-time split into
   // kXcdGroups groups of whole realisations (one per XCD), each keeping that
   // order
+  constexpr int NG = gsc::kEnsGemmGroups;
   const int tile_ids[2] = {GSC_TILE_256, GSC_TILE_128};
-  std::vector<int> gemm_order[2];
-  std::vector<int64_t> tile_start[2], group_start[2];
+  std::vector<int> gemm_order[NG];
+  std::vector<int64_t> tile_start[NG], group_start[NG];
   int ngemm = 0;
-  for (int q = 0; q < 2; ++q) {
+  for (int q = 0; q < NG; ++q) {
     std::vector<int> lst;
     for (int r : fact)
-      if (tile_of[(size_t)r] == tile_ids[q]) lst.push_back(r);
+      if (tile_of[(size_t)r] == tile_ids[q / 2] && rs[(size_t)r].prod == q % 2) lst.push_back(r);
     std::stable_sort(lst.begin(), lst.end(), [&](int a, int b) {
       const gsc::EnsReal &ea = rs[(size_t)a], &eb = rs[(size_t)b];
       if (ea.mp != eb.mp) return ea.mp > eb.mp;
@@ -1176,11 +1227,11 @@ int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes
   // table tasks: every factored realisation, groups of 64 rows of S (na,
   // 3-D only), A (arows) and C (ncp)
   std::vector<int> all_fact;
-  for (int q = 0; q < 2; ++q) all_fact.insert(all_fact.end(), gemm_order[q].begin(), gemm_order[q].end());
+  for (int q = 0; q < NG; ++q) all_fact.insert(all_fact.end(), gemm_order[q].begin(), gemm_order[q].end());
   std::vector<int64_t> task_start(3 * all_fact.size() + 1, 0);
   for (size_t i = 0; i < all_fact.size(); ++i) {
     const gsc::EnsReal& e = rs[(size_t)all_fact[i]];
-    task_start[3 * i + 1] = task_start[3 * i] + (dim == 3 ? (e.na + 63) / 64 : 0);
+    task_start[3 * i + 1] = task_start[3 * i] + (dim == 3 || e.prod ? (e.na + 63) / 64 : 0);
     task_start[3 * i + 2] = task_start[3 * i + 1] + e.arows / 64;
     task_start[3 * i + 3] = task_start[3 * i + 2] + e.ncp / 64;
   }
@@ -1209,8 +1260,8 @@ int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes
   const size_t c_tk = put(task_start.data(), sizeof(int64_t) * task_start.size());
   const size_t c_dir = put(dir.data(), sizeof(int) * dir.size());
   const size_t c_bs = put(blk_start.data(), sizeof(int64_t) * blk_start.size());
-  size_t c_go[2], c_ts[2], c_gs[2];
-  for (int q = 0; q < 2; ++q) {
+  size_t c_go[NG], c_ts[NG], c_gs[NG];
+  for (int q = 0; q < NG; ++q) {
     c_go[q] = put(gemm_order[q].data(), sizeof(int) * gemm_order[q].size());
     c_ts[q] = put(tile_start[q].data(), sizeof(int64_t) * tile_start[q].size());
     c_gs[q] = put(group_start[q].data(), sizeof(int64_t) * group_start[q].size());
@@ -1235,9 +1286,9 @@ int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes
     if (e.path == GSC_PATH_FACTORED && e.npts > 0) {
       const gsc::FactoredPlan& fp = plans[(size_t)r];
       char* f = work + f_off[(size_t)r];
-      e.S = reinterpret_cast<float2*>(f);
-      e.A = reinterpret_cast<uint32_t*>(f + (fp.off_a - fp.off_s));
-      e.C = reinterpret_cast<float2*>(f + (fp.off_c - fp.off_s));
+      e.S = reinterpret_cast<float2*>(f + fp.off_s);
+      e.A = reinterpret_cast<uint32_t*>(f + fp.off_a);
+      e.C = reinterpret_cast<float2*>(f + fp.off_c);
     }
   }
   std::memcpy(hctrl.data() + c_desc, rs.data(), sizeof(gsc::EnsReal) * rs.size());
@@ -1256,21 +1307,22 @@ int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes
   el.ndir = (int)dir.size();
   el.blk_start = reinterpret_cast<const int64_t*>(base + c_bs);
   el.counters = reinterpret_cast<int*>(base + c_cnt);
-  el.dir_counter = el.counters + 2 * gsc::kXcdGroups;
+  el.dir_counter = el.counters + NG * gsc::kXcdGroups;
   el.dim = dim;
   el.max_tiles = max_tiles;
   el.ngemm = ngemm;
-  for (int q = 0; q < 2; ++q) {
+  for (int q = 0; q < NG; ++q) {
     gsc::EnsGemmGroup& g = el.gemm[q];
     g.fact = reinterpret_cast<const int*>(base + c_go[q]);
     g.nfact = (int)gemm_order[q].size();
     g.tile_start = reinterpret_cast<const int64_t*>(base + c_ts[q]);
     g.group_start = reinterpret_cast<const int64_t*>(base + c_gs[q]);
     g.counters = el.counters + q * gsc::kXcdGroups;
-    g.brows = gsc::tile_block_rows(tile_ids[q]);
+    g.brows = gsc::tile_block_rows(tile_ids[q / 2]);
+    g.kr = dim == 3 || q % 2 == 1;
     // blocks per CU the GEMM is built for: 256-row tiles one (two with the
     // FP64 flush, which fits 256 registers), 128-row tiles two (four)
-    const int64_t bpc_auto = (o->accum == GSC_ACCUM_F64 ? 2 : 1) * (q == 1 ? 2 : 1);
+    const int64_t bpc_auto = (o->accum == GSC_ACCUM_F64 ? 2 : 1) * (q / 2 == 1 ? 2 : 1);
     const int64_t bpc = o->blocks_per_cu > 0 ? o->blocks_per_cu : bpc_auto;
     g.gemm_blocks = (int)std::max<int64_t>(1, std::min<int64_t>(tile_start[q].back(), bpc * cus));
   }

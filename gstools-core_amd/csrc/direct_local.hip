@@ -159,6 +159,7 @@ constexpr double kRoughRev = 1.0;   // bound on |kap . xi| (revolutions) of the 
 constexpr int kBoxBlocks = 1024;    // partial bounding boxes
 constexpr int kBoxUnroll = 4;       // points in flight per thread there
 constexpr size_t kR0TableCap = size_t(256) << 20;  // largest (cell, mode) r0 table
+constexpr int kVecR0Pad = 64;       // zero r0 entries past the smooth modes in a vector-kernel row
 
 struct alignas(16) BinParams {
   double lo[3];
@@ -187,7 +188,8 @@ struct BinLayout {
   int64_t max_cells, max_waves, nblk, blk_pts;
   size_t params, box, centre, bcount, counts, offsets, wave_cell, recs, btpl, btpl32, amp, sm,
       rough, r0w, bytes;
-  int64_t r0ld;  // row length of the (cell, mode) r0 table; 0: no table
+  int64_t r0ld;   // row length of the (cell, mode) r0 words of the scalar kernel; 0: no table
+  int64_t r0vld;  // row length of the (cell, mode) r0 parts of the vector kernel; 0: no table
 };
 
 BinLayout bin_layout(int64_t M, int64_t N) {
@@ -213,12 +215,18 @@ BinLayout bin_layout(int64_t M, int64_t N) {
   L.amp = take(sizeof(float4) * (m + kModeChunk));
   L.sm = take(sizeof(SmoothRec) * (m + kModeChunk));
   L.rough = take(sizeof(ModeRec) * m);
-  // r0 words of every (cell, smooth mode), when the table stays below
-  // kR0TableCap bytes; otherwise every wave evaluates its cell's r0 itself
+  // r0 of every (cell, smooth mode), when the table stays below kR0TableCap
+  // bytes; otherwise every wave evaluates its cell's r0 itself.  The scalar
+  // kernel's fp16 words (4 bytes, rows padded by a chunk) and the vector
+  // kernel's bf16 parts (8 bytes, rows padded by kVecR0Pad) share the region.
   const int64_t r0ld = static_cast<int64_t>(m) + kModeChunk;
+  const int64_t r0vld = static_cast<int64_t>(m) + kVecR0Pad;
   const size_t r0bytes = sizeof(uint32_t) * static_cast<size_t>(L.max_cells * r0ld);
+  const size_t r0vbytes = sizeof(uint2) * static_cast<size_t>(L.max_cells * r0vld);
   L.r0ld = r0bytes <= kR0TableCap ? r0ld : 0;
-  L.r0w = L.r0ld > 0 ? take(r0bytes) : 0;
+  L.r0vld = r0vbytes <= kR0TableCap ? r0vld : 0;
+  const size_t region = std::max(L.r0ld > 0 ? r0bytes : 0, L.r0vld > 0 ? r0vbytes : 0);
+  L.r0w = region > 0 ? take(region) : 0;
   L.bytes = off;
   return L;
 }
@@ -731,16 +739,20 @@ __device__ __forceinline__ void cell_centre(const BinParams& P, int e, double (&
 
 // One wave per cell: records its centre and its waves in the wave -> cell
 // map, marks the slots past its points as padding and, when r0w is given,
-// writes the cell's row of r0 words (r0_hi | r0_lo << 16, the fp16 parts of
-// the reduced FP64 phase at the centre) for every smooth mode up to the end
-// of the last chunk (zero phase data past n_smooth), which k_direct_local32
-// copies instead of evaluating r0 in every wave of the cell.
+// writes the cell's row of r0 (the reduced FP64 phase at the centre, rounded
+// to FP32) for every smooth mode, which the main kernels read instead of
+// evaluating r0 in every wave of the cell:
+// - scalar form (vec = false): words r0_hi | r0_lo << 16 (fp16 parts) up to
+//   the end of the last chunk, zero phase data past n_smooth;
+// - vector form: pairs (p0 | p1 << 16, p2) of the three bf16 parts, r0ld of
+//   them, zero past n_smooth.
 template <int D>
 __global__ void __launch_bounds__(256)
 k_bin_waves(const BinParams* __restrict__ prm, const int* __restrict__ counts,
             const int* __restrict__ offsets, double* __restrict__ centre,
             int* __restrict__ wave_cell, float4* __restrict__ recs,
-            const SmoothRec* __restrict__ sm, uint32_t* __restrict__ r0w, int64_t r0ld) {
+            const SmoothRec* __restrict__ sm, uint32_t* __restrict__ r0w, int64_t r0ld,
+            bool vec) {
   const int e = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   const BinParams P = *prm;
   if (e >= P.ncells) return;
@@ -754,6 +766,26 @@ k_bin_waves(const BinParams* __restrict__ prm, const int* __restrict__ counts,
   const int sp = s0 + counts[e] + lane;  // padding: fewer than kWavePts slots
   if (sp < s1) recs[sp] = make_rec(0.0f, 0.0f, 0.0f, -1);
   if (r0w == nullptr || s1 == s0) return;  // no table, or no wave reads this row
+  if (vec) {
+    uint2* row = reinterpret_cast<uint2*>(r0w) + e * r0ld;
+#pragma unroll 4
+    for (int i = lane; i < r0ld; i += 64) {
+      uint2 v = make_uint2(0u, 0u);
+      if (i < P.n_smooth) {
+        const SmoothRec sr = sm[i];
+        double t = __builtin_fma(sr.kap[0], X[0], sr.th);
+        t = __builtin_fma(sr.kap[1], X[1], t);
+        if constexpr (D == 3) t = __builtin_fma(sr.kap[2], X[2], t);
+        float r0 = static_cast<float>(t - rint(t));
+        __asm__ volatile("" : "+v"(r0));  // FP32 rounding step kept
+        uint16_t p[3];
+        split3(r0, p);
+        v = make_uint2(static_cast<uint32_t>(p[0]) | (static_cast<uint32_t>(p[1]) << 16), p[2]);
+      }
+      row[i] = v;
+    }
+    return;
+  }
   uint32_t* row = r0w + e * r0ld;
   const int end = (P.n_smooth + kModeChunk - 1) / kModeChunk * kModeChunk;
 #pragma unroll 4
@@ -1038,16 +1070,191 @@ __device__ __forceinline__ void sum_modes_vec(const ModeRec* __restrict__ modes,
 
 // Vector-valued form (incompressible fields): component c of point x is
 // sum_m amp_c(m) sin(2 pi t_m(x)), one sine per point-mode shared by the D
-// components.  The phases come from the same matrix-core product as in
-// k_direct_local, for 32 slots per wave (half a record group, two MFMA
-// groups), which leaves registers for D accumulator sets.  Wave w works on
-// slots 32 (w & 1) .. + 31 of record group w >> 1.
-constexpr int kVecChunk = 512;  // modes per (r0, amplitudes) table of a wave pair
-constexpr int kVecRedLd = 33;   // LDS row of the cross-lane sums (doubles)
+// components.  The phases come from 16 x 16 x 32 bf16 products (three-part
+// splits, 24 significant bits), for 32 slots per wave (half a record group,
+// two MFMA groups), which leaves registers for D accumulator sets.  Wave w
+// works on slots 32 (w & 1) .. + 31 of record group w >> 1.  FP32 partial
+// sums over kVecChunk modes are added into a second FP32 set.
+//
+// K slots 24 .. 26 (lane group kg = 3) carry r0: A holds 1 there and B the
+// three bf16 parts of the cell's r0, read from the per-cell table that
+// k_bin_waves writes, so the matrix instruction returns the whole local
+// phase.  The amplitudes of kVecAmpChunk modes at a time sit in the block's
+// LDS.  Without a per-cell table (too large) k_direct_local_vec_eval below
+// evaluates r0 per wave pair and chunk instead.
+constexpr int kVecChunk = 512;      // modes per FP32 accumulation interval
+constexpr int kVecAmpChunk = 1024;  // modes per amplitude table of a block (16 KB)
+constexpr int kVecRedLd = 33;       // LDS row of the cross-lane sums (doubles)
 static_assert(kVecChunk <= kModeChunk, "the mode tables are zero-padded by kModeChunk entries");
+static_assert(kVecAmpChunk <= kModeChunk, "the amplitude table reads stay in the padding");
+static_assert(kVecAmpChunk % kVecChunk == 0, "accumulation intervals split the table");
+// B fragment reads run two tiles (32 modes) past the last tile issued, and the
+// r0 lanes read 16 bytes (two entries) from an 8-byte entry
+static_assert(kVecR0Pad >= 32 + 16, "r0 row padding covers the reads past the smooth modes");
+
+// A fragment of the r0-in-K form: slots as a_fragment, plus ones in slots
+// 24 .. 26 (lane group 3).
+template <int D>
+__device__ __forceinline__ bf8 a_fragment_r0(const float* rec, int kg, float sc) {
+  if (kg == 3) {
+    constexpr uint16_t kOne = 0x3F80;  // bf16 1.0
+    const uint16_t v[8] = {kOne, kOne, kOne, 0, 0, 0, 0, 0};
+    return __builtin_bit_cast(bf8, v);
+  }
+  return a_fragment<D>(rec, kg, sc);
+}
+
 template <int D, typename OutT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4)))
-k_direct_local_vec(const BinParams* __restrict__ prm, const double* __restrict__ centre,
+k_direct_local_vec(const BinParams* __restrict__ prm, const int* __restrict__ wave_cell,
+                   const float4* __restrict__ recs, const uint4* __restrict__ btpl,
+                   const float4* __restrict__ ampt, const uint2* __restrict__ r0v,
+                   int64_t r0ld, const ModeRec* __restrict__ rough,
+                   const double* __restrict__ pos, int64_t ld, OutT* __restrict__ out,
+                   int64_t out_ld, double scale) {
+  __shared__ __attribute__((aligned(16))) f4 s_amp[kVecAmpChunk];
+  __shared__ double s_red[4][16 * kVecRedLd];
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int w = blockIdx.x * 4 + wv;
+  const BinParams P = *prm;
+  if (blockIdx.x * 4 >= 2 * P.nwaves) return;  // the whole block is past the list
+  // a wave past the list (in the last block) runs the loops on the last
+  // record group, for the block's barriers, and stores nothing
+  const bool live = w < 2 * P.nwaves;
+  const int n_smooth = P.n_smooth, n_rough = P.n_rough;
+  const int rg = live ? (w >> 1) : P.nwaves - 1;
+  const int kg = lane >> 4, j = lane & 15;
+  const float* rf =
+      reinterpret_cast<const float*>(recs + static_cast<int64_t>(rg) * kWavePts + 32 * (w & 1));
+  bf8 a[2];
+  // (a select chain, not P.h[kg]: a lane-dependent index would put P in scratch)
+  const double hk = kg == 0 ? P.h[0] : (kg == 1 ? P.h[1] : P.h[2]);
+  const float sck = kg < D ? static_cast<float>(ldexp(1.0, -half_cell_exp(hk))) : 0.0f;
+#pragma unroll
+  for (int g = 0; g < 2; ++g) a[g] = a_fragment_r0<D>(rf + (16 * g + j) * 4, kg, sck);
+  // B fragment source of this lane, mode j of a tile: the mode template
+  // (lane groups 0..2) or the cell's r0 parts (group 3; the 16-byte read
+  // also takes the next entry, which meets zero slots of A)
+  const char* bsrc =
+      kg < 3 ? reinterpret_cast<const char*>(btpl + 4 * j + kg)
+             : reinterpret_cast<const char*>(r0v + static_cast<int64_t>(wave_cell[rg]) * r0ld + j);
+  const int64_t bstep = kg < 3 ? 16 * 4 * sizeof(uint4) : 16 * sizeof(uint2);  // per tile
+  const f4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+  float acc[D][2][4], tot[D][2][4];
+#pragma unroll
+  for (int c = 0; c < D; ++c)
+#pragma unroll
+    for (int g = 0; g < 2; ++g)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        acc[c][g][v] = 0.0f;
+        tot[c][g][v] = 0.0f;
+      }
+  for (int a0 = 0; a0 < n_smooth; a0 += kVecAmpChunk) {
+    __syncthreads();  // the previous amplitude table is read
+    // zero amplitudes past n_smooth (the padding of ampt)
+    for (int i = threadIdx.x; i < kVecAmpChunk; i += 256) {
+      const float4 am = ampt[a0 + i];
+      s_amp[i] = f4{am.x, am.y, am.z, 0.0f};
+    }
+    __syncthreads();
+    const int a1 = min(a0 + kVecAmpChunk, n_smooth);
+    for (int c0 = a0; c0 < a1; c0 += kVecChunk) {
+      const int ntile = min(kVecChunk / 16, (a1 - c0 + 15) / 16);
+      const char* bp = bsrc + static_cast<int64_t>(c0 / 16) * bstep;
+      const f4* mp = s_amp + (c0 - a0) + j;
+      uint4 ba = *reinterpret_cast<const uint4*>(bp);
+      uint4 bb = *reinterpret_cast<const uint4*>(bp + bstep);
+      bp += 2 * bstep;
+      f4 ta[2], tb[2];
+      f4 ama, amb;
+      auto issue = [&](f4 (&t)[2], f4& am, uint4& bq) {
+        am = *mp;
+        mp += 16;
+#pragma unroll
+        for (int g = 0; g < 2; ++g)
+          t[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[g], __builtin_bit_cast(bf8, bq), zero,
+                                                         0, 0, 0);
+        bq = *reinterpret_cast<const uint4*>(bp);
+        bp += bstep;
+      };
+      auto consume = [&](const f4 (&t)[2], const f4& am) {
+#pragma unroll
+        for (int g = 0; g < 2; ++g)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const float sv = __builtin_amdgcn_sinf(t[g][v]);
+#pragma unroll
+            for (int c = 0; c < D; ++c) acc[c][g][v] = __builtin_fmaf(am[c], sv, acc[c][g][v]);
+          }
+      };
+      issue(ta, ama, ba);
+      int q = 0;
+      for (; q + 2 < ntile; q += 2) {
+        issue(tb, amb, bb);
+        consume(ta, ama);
+        issue(ta, ama, ba);
+        consume(tb, amb);
+      }
+      if (q + 1 < ntile) {
+        issue(tb, amb, bb);
+        consume(ta, ama);
+        consume(tb, amb);
+      } else {
+        consume(ta, ama);
+      }
+#pragma unroll
+      for (int c = 0; c < D; ++c)
+#pragma unroll
+        for (int g = 0; g < 2; ++g)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            tot[c][g][v] += acc[c][g][v];
+            acc[c][g][v] = 0.0f;
+          }
+    }
+  }
+  // cross-lane sums, one component at a time: tot[c][g][v] is slot
+  // 16 g + 4 (lane / 16) + v of the half, mode column j
+  const int p = live && lane < 32 ? __builtin_bit_cast(int, rf[lane * 4 + 3]) : -1;
+  double comp[D];
+  double* red = s_red[wv];
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    __asm__ volatile("" ::: "memory");
+#pragma unroll
+    for (int g = 0; g < 2; ++g)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        red[j * kVecRedLd + 16 * g + 4 * kg + v] = static_cast<double>(tot[c][g][v]);
+    double sum = 0.0;
+    if (lane < 32)
+#pragma unroll
+      for (int jj = 0; jj < 16; ++jj) sum += red[jj * kVecRedLd + lane];
+    comp[c] = sum;
+    __asm__ volatile("" ::: "memory");
+  }
+  if (p < 0) return;
+  if (n_rough > 0) {
+    double x[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) x[k] = pos[k * ld + p];
+    double rt[D];
+    sum_modes_vec<D>(rough, n_rough, x, rt);
+#pragma unroll
+    for (int c = 0; c < D; ++c) comp[c] += rt[c];
+  }
+#pragma unroll
+  for (int c = 0; c < D; ++c) store(out + c * out_ld + p, comp[c] * scale);
+}
+
+// The form above with r0 evaluated in the kernel: per wave pair (the two
+// halves of one record group, one cell) an LDS table of (r0, amplitudes) for
+// kVecChunk modes, r0 entering the product as the accumulator C.
+template <int D, typename OutT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4)))
+k_direct_local_vec_eval(const BinParams* __restrict__ prm, const double* __restrict__ centre,
                    const int* __restrict__ wave_cell, const float4* __restrict__ recs,
                    const uint4* __restrict__ btpl, const float4* __restrict__ ampt,
                    const SmoothRec* __restrict__ sm, const ModeRec* __restrict__ rough,
@@ -1235,7 +1442,10 @@ hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table
   float4* ampt = at<float4>(bin_work, L.amp);
   SmoothRec* sm = at<SmoothRec>(bin_work, L.sm);
   ModeRec* rough = at<ModeRec>(bin_work, L.rough);
-  uint32_t* r0w = L.r0ld > 0 ? at<uint32_t>(bin_work, L.r0w) : nullptr;
+  uint32_t* r0w = L.r0ld > 0 || L.r0vld > 0 ? at<uint32_t>(bin_work, L.r0w) : nullptr;
+  // the r0 rows k_bin_waves writes: the scalar or the vector form, if any
+  uint32_t* r0rows = vec ? (L.r0vld > 0 ? r0w : nullptr) : (L.r0ld > 0 ? r0w : nullptr);
+  const int64_t r0rld = vec ? L.r0vld : L.r0ld;
   const ModeRec* modes = static_cast<const ModeRec*>(table);
   const int nblk = static_cast<int>(L.nblk);
   const unsigned cb = static_cast<unsigned>((L.max_cells + kColCells - 1) / kColCells);
@@ -1252,20 +1462,29 @@ hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table
     k_bin_hist<D><<<nblk, kSortThreads, 0, s>>>(pos, pos_ld, N, L.blk_pts, prm, bcount);         \
     k_col_scan<<<cb, 256, 0, s>>>(prm, nblk, bcount, counts);                                    \
     k_bin_offsets<<<1, kSortThreads, 0, s>>>(prm, counts, offsets);                              \
-    k_bin_waves<D><<<wb, 256, 0, s>>>(prm, counts, offsets, centre, wave_cell, recs, sm,         \
-                                      vec ? nullptr : r0w, L.r0ld);                              \
+    k_bin_waves<D><<<wb, 256, 0, s>>>(prm, counts, offsets, centre, wave_cell, recs, sm, r0rows, \
+                                      r0rld, vec);                                               \
     k_bin_scatter<D><<<nblk, kSortThreads, 0, s>>>(pos, pos_ld, N, L.blk_pts, prm, bcount,       \
                                                    offsets, recs);                               \
     if (split) (void)hipEventRecord(split, s);                                                   \
-    if (vec && f32)                                                                              \
-      k_direct_local_vec<D, float><<<vb, 256, 0, s>>>(prm, centre, wave_cell, recs, btpl, ampt,  \
-                                                      sm, rough, pos, pos_ld,                    \
-                                                      static_cast<float*>(out), out_ld, scale);  \
+    if (vec && r0rows != nullptr && f32)                                                         \
+      k_direct_local_vec<D, float><<<vb, 256, 0, s>>>(                                           \
+          prm, wave_cell, recs, btpl, ampt, reinterpret_cast<const uint2*>(r0rows), r0rld, rough, \
+          pos, pos_ld, static_cast<float*>(out), out_ld, scale);                                 \
+    else if (vec && r0rows != nullptr)                                                           \
+      k_direct_local_vec<D, double><<<vb, 256, 0, s>>>(                                          \
+          prm, wave_cell, recs, btpl, ampt, reinterpret_cast<const uint2*>(r0rows), r0rld, rough, \
+          pos, pos_ld, static_cast<double*>(out), out_ld, scale);                                \
+    else if (vec && f32)                                                                         \
+      k_direct_local_vec_eval<D, float><<<vb, 256, 0, s>>>(prm, centre, wave_cell, recs, btpl,   \
+                                                           ampt, sm, rough, pos, pos_ld,         \
+                                                           static_cast<float*>(out), out_ld,     \
+                                                           scale);                               \
     else if (vec)                                                                                \
-      k_direct_local_vec<D, double><<<vb, 256, 0, s>>>(prm, centre, wave_cell, recs, btpl, ampt, \
-                                                       sm, rough, pos, pos_ld,                   \
-                                                       static_cast<double*>(out), out_ld,        \
-                                                       scale);                                   \
+      k_direct_local_vec_eval<D, double><<<vb, 256, 0, s>>>(prm, centre, wave_cell, recs, btpl,  \
+                                                            ampt, sm, rough, pos, pos_ld,        \
+                                                            static_cast<double*>(out), out_ld,   \
+                                                            scale);                              \
     else if (f32)                                                                                \
       k_direct_local32<D, float><<<mb, 256, 0, s>>>(prm, centre, wave_cell, recs, btpl32, ampt,  \
                                                     sm, r0w, L.r0ld, rough, pos, pos_ld,         \

@@ -99,6 +99,17 @@ struct EnsReal {
   int path;           // GSC_PATH_DIRECT or GSC_PATH_FACTORED
   int mp;             // factored: modes padded to 32
   int trans;          // factored 2-D: rows are y and columns x (the padding is smaller)
+  // Product form (factored 2-D on uniform axes): row r = brows rh + a and
+  // column c = tcols ch + b of tile (rh, ch) have the phase
+  //   th + k_r (org_r + brows rh step_r) + k_c (org_c + tcols ch step_c)
+  //      + k_r a step_r + k_c b step_c,
+  // so S holds the first line per tile (na = row_tiles col_tiles rows,
+  // amplitude folded in), A the row offsets a < brows and C the column
+  // offsets b < ncp (one tile each), and the GEMM runs every tile as a 3-D
+  // block (B = S(tile) C).  The tables shrink from the whole grid to a tile.
+  int prod;
+  int tcols;          // product form: GEMM tile columns
+  double org[2], step[2];  // product form: origin and spacing of the row and column axes
   int64_t na, nrow, arows, nb_out, ncols, ncp, row_tiles, col_tiles;  // factored geometry
   int64_t s0, s1, s2, npts;                     // grid shape
   double scale;
@@ -107,18 +118,22 @@ struct EnsReal {
 // Launch sequence of an ensemble (all pointers device-resident).  fact / dir
 // list the realisations on each path, task_start / blk_start their prefix
 // sums of table tasks and direct work items.  The factored realisations run in
-// up to two persistent GEMM launches, one per tile shape: each has its own
-// realisation list, tile prefix sums, XCD groups and counters.  max_tiles
-// bounds the batched prep grid; the *_blocks are the persistent grid sizes.
+// up to kEnsGemmGroups persistent GEMM launches, one per tile shape and
+// operand form (tables of the whole grid, or the 2-D product form): each has
+// its own realisation list, tile prefix sums, XCD groups and counters.
+// max_tiles bounds the batched prep grid; the *_blocks are the persistent
+// grid sizes.
 constexpr int kXcdGroups = 8;     // ensemble GEMM work groups (one per XCD of an MI355X)
+constexpr int kEnsGemmGroups = 4; // GEMM launches: tile shape (256, 128 rows) x operand form
 struct EnsGemmGroup {
-  const int* fact;             // realisations of this tile shape, longest first per XCD group
+  const int* fact;             // realisations of this group, longest first per XCD group
   int nfact;
   const int64_t* tile_start;   // nfact + 1 prefix sums of their GEMM tiles
   const int64_t* group_start;  // tile ranges of the kXcdGroups groups (kXcdGroups + 1)
   int* counters;               // kXcdGroups work counters
   int gemm_blocks;
   int brows;                   // block rows of the tile shape (256 or 128)
+  bool kr;                     // B built as S(block) C: 3-D, or the 2-D product form
 };
 struct EnsLaunch {
   const EnsReal* d;
@@ -127,7 +142,7 @@ struct EnsLaunch {
   int nfact;
   const int64_t* task_start;   // table tasks: 3 nfact + 1 prefix sums (roles S, A, C)
   int64_t n_table_tasks;
-  EnsGemmGroup gemm[2];
+  EnsGemmGroup gemm[kEnsGemmGroups];
   int ngemm;
   const int* dir;
   int ndir;
@@ -139,8 +154,8 @@ struct EnsLaunch {
   int direct_blocks;
 };
 constexpr int kEnsPoints = 2048;  // points per direct work item
-// counters: kXcdGroups per GEMM launch (two), then the direct-kernel counter
-constexpr int kEnsCounters = 2 * kXcdGroups + 1;
+// counters: kXcdGroups per GEMM launch, then the direct-kernel counter
+constexpr int kEnsCounters = kEnsGemmGroups * kXcdGroups + 1;
 hipError_t launch_ensemble(const EnsLaunch& e, int precision, int out_dtype, int accum,
                            hipStream_t stream);
 

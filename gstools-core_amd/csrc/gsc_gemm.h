@@ -656,13 +656,17 @@ k_factored_gemm(GemmArgs g) {
 // and the modes' data for the role's axis are staged in LDS, kTabStage modes
 // at a time, so the chunk loop's only global accesses are its stores (the
 // vector memory counter counts stores too: a global load in the loop would
-// wait for the stores before it).
+// wait for the stores before it).  Product-form realisations (EnsReal.prod)
+// take their coordinates from origin and spacing: S rows are tiles (phase
+// th + k_r x_r + k_c x_c at the tile's corner), A rows and C columns the
+// offsets within a tile.
 constexpr int kTabStage = 512;
 template <typename T, int NS>
 __global__ void __launch_bounds__(256)
 k_ens_tables(const EnsReal* __restrict__ d, const int* __restrict__ fact,
              const int64_t* __restrict__ task_start, int nfact) {
   __shared__ double2 s_kt[kTabStage];  // (slope along the role's axis, phase offset)
+  __shared__ double s_k2[kTabStage];   // product-form S: slope along the column axis
   __shared__ float s_am[kTabStage];    // amplitude
   const int64_t task = blockIdx.x;
   const int k = find_item(task_start, 3 * nfact, task);
@@ -673,32 +677,45 @@ k_ens_tables(const EnsReal* __restrict__ d, const int* __restrict__ fact,
   const int nch = r.mp / 16;
   const int g = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
   const int64_t row = (task - task_start[k]) * 64 + (threadIdx.x & 63);
+  const bool prod = r.prod != 0;
   int axis;
   bool amp;
   const double* xp;
   int64_t nvalid, nrows;
-  if (role == 0) {
-    axis = 0;
+  if (role == 0) {  // 3-D axis 0, or the product form's tile corners
+    axis = prod ? (r.trans ? 1 : 0) : 0;
     amp = true;
     xp = r.ax0;
     nvalid = r.na;
     nrows = r.na;
   } else if (role == 1) {  // a transposed 2-D realisation takes its rows from y
     axis = r.trans ? 1 : r.dim - 2;
-    amp = r.dim == 2;
+    amp = r.dim == 2 && !prod;
     xp = r.dim == 3 ? r.ax1 : (r.trans ? r.ax1 : r.ax0);
-    nvalid = r.nrow;
+    nvalid = prod ? r.arows : r.nrow;
     nrows = r.arows;
   } else {
     axis = r.trans ? 0 : r.dim - 1;
     amp = false;
     xp = r.dim == 3 ? r.ax2 : (r.trans ? r.ax0 : r.ax1);
-    nvalid = r.ncols;
+    nvalid = prod ? r.ncp : r.ncols;
     nrows = r.ncp;
   }
+  const int axis2 = r.trans ? 0 : 1;  // product-form S: the column axis
   const bool active = row < nrows;  // rows of the last S group past na store nothing
   const bool valid = row < nvalid;  // padding rows of A and C hold zeros
-  const double x = valid ? gmem(xp)[row] : 0.0;
+  double x = 0.0, x2 = 0.0;
+  if (valid) {
+    if (!prod) {
+      x = gmem(xp)[row];
+    } else if (role == 0) {
+      const int64_t rh = row / r.col_tiles, ch = row - rh * r.col_tiles;
+      x = r.org[0] + static_cast<double>(rh * r.arows) * r.step[0];
+      x2 = r.org[1] + static_cast<double>(ch * r.tcols) * r.step[1];
+    } else {
+      x = static_cast<double>(row) * r.step[role - 1];
+    }
+  }
   const __attribute__((address_space(1))) ModeRec* gm = gmem(r.table);
   float2* S = r.S;
   uint32_t* A = r.A;
@@ -708,6 +725,7 @@ k_ens_tables(const EnsReal* __restrict__ d, const int* __restrict__ fact,
     for (int i = threadIdx.x; i < kTabStage; i += 256) {
       const int m = min(b0 + i, M - 1);
       s_kt[i] = double2{gm[m].kap[axis], gm[m].th};
+      s_k2[i] = gm[m].kap[axis2];
       s_am[i] = gm[m].amp[0];
     }
     __syncthreads();
@@ -720,7 +738,8 @@ k_ens_tables(const EnsReal* __restrict__ d, const int* __restrict__ fact,
       for (int e = 0; e < 4; ++e) {
         const double2 kt = s_kt[m0 - b0 + e];
         kx[e] = kt.x;
-        th[e] = kt.y;
+        // the product form's S: the column axis' part joins the offset
+        th[e] = prod && role == 0 ? __builtin_fma(s_k2[m0 - b0 + e], x2, kt.y) : kt.y;
         am[e] = s_am[m0 - b0 + e];
       }
       f2v v[4];
@@ -761,11 +780,20 @@ k_ens_gemm(const EnsReal* __restrict__ d, const int* __restrict__ fact,
       const int64_t rt = local / r.col_tiles, ct = local - rt * r.col_tiles;
       // a transposed realisation (2-D): rows are y, columns x of the C-order
       // (x, y) output, so the row stride is 1 and the column stride nrow
-      const TileJob jb = make_job<KR, JN, RB>(r.A, r.arows, r.mp / 16, r.S, r.na, r.C, r.ncp,
-                                              r.nrow, r.nb_out, r.out, out_f64,
-                                              r.trans ? 1 : r.ncols, r.ncols, r.scale, rt, ct,
-                                              r.trans ? r.nrow : 1, r.col_tiles, r.ncp, 0);
-      if (!KR && r.trans) gemm_tile<T, NS, KR, ACC, JN, RB, true>(sm, jb);
+      TileJob jb = make_job<KR, JN, RB>(r.A, r.arows, r.mp / 16, r.S, r.na, r.C, r.ncp, r.nrow,
+                                        r.nb_out, r.out, out_f64, r.trans ? 1 : r.ncols,
+                                        r.ncols, r.scale, rt, ct, r.trans ? r.nrow : 1,
+                                        r.col_tiles, r.ncp, 0);
+      if (KR && r.prod) {
+        // product form: S row = tile, A and C tables one tile each
+        jb.a = local;
+        jb.arow0 = 0;
+        jb.ccol0 = 0;
+        jb.col0 = ct * (32 * JN);
+        jb.out_row0 = rt * r.arows;
+        jb.nvalid = r.nrow - rt * r.arows;
+      }
+      if (r.trans) gemm_tile<T, NS, KR, ACC, JN, RB, true>(sm, jb);
       else gemm_tile<T, NS, KR, ACC, JN, RB, false>(sm, jb);
     }
   }
@@ -825,7 +853,7 @@ hipError_t ens_launch(const EnsLaunch& e, int accum, bool out_f64, hipStream_t s
     if (g.brows == 128) GSC_ER(KR, ACC, 1);     \
     else GSC_ER(KR, ACC, 2);                    \
   } while (0)
-    if (e.dim == 3) {
+    if (g.kr) {
       if (accum == ACC_F32X2) GSC_E(true, ACC_F32X2);
       else if (accum == ACC_F32) GSC_E(true, ACC_F32);
       else GSC_E(true, ACC_F64);

@@ -320,9 +320,10 @@ def test_local_extreme_coordinate_scales(scale):
 
 
 def test_local_without_cell_r0_table():
-    # 16384 cells x (4000 + 1024) modes of r0 words exceed the 256 MB table
-    # cap, so every wave evaluates its cell's r0 itself; 1000 modes keep the
-    # precomputed table.  Both against the FP64 VALU kernel.
+    # 16384 cells x (4000 + 1024) modes of r0 words (scalar) and x (4000 +
+    # 64) modes of r0 parts (incompressible) exceed the 256 MB table cap, so
+    # every wave evaluates its cell's r0 itself; 1000 modes keep the
+    # precomputed tables.  Both against the FP64 VALU kernels.
     rng = np.random.default_rng(77)
     pos = rng.uniform(0, 500, size=(3, 1_100_000))
     for m in (4000, 1000):
@@ -332,6 +333,12 @@ def test_local_without_cell_r0_table():
         b = gstools_core.summate(cov, z1, z2, pos, variant="valu")
         err = _err(a, b) / np.sqrt(m)
         print(f"local vs valu, m={m}: {err:.3e} sigma")
+        assert err <= 1e-5
+        ai = gstools_core.summate_incompr(cov, z1, z2, pos, variant="local")
+        bi = gstools_core.summate_incompr(cov, z1, z2, pos, variant="valu")
+        assert np.all(np.isfinite(ai))
+        err = _err(ai, bi) / np.sqrt(m)
+        print(f"incompressible local vs valu, m={m}: {err:.3e} sigma")
         assert err <= 1e-5
 
 
