@@ -268,7 +268,8 @@ k_tables(const ModeRec* __restrict__ modes, int M, int dim, int nch,
 // (output rows out_row0 + r for r < nvalid), columns col0 .. col0 + 32 JN - 1
 // (stored for col < ncols):  out[(out_row0 + r) * ldo + col] = scale * u.
 //
-// Software pipeline, one barrier per chunk: in the step for chunk c a wave
+// Software pipeline (128-row blocks), one barrier per chunk: in the step for
+// chunk c a wave
 //   1. requests chunk c + 1's A fragments into the other register set and
 //      reads chunk c's ks = 1 B fragments from LDS buffer c % 2,
 //   2. issues chunk c's ks = 0 MFMAs (fragments read in the previous step)
@@ -277,10 +278,10 @@ k_tables(const ModeRec* __restrict__ modes, int M, int dim, int nch,
 //   3. waits at the barrier, requests the C (and S) values of chunk c + 2
 //      and reads chunk c + 1's ks = 0 B fragments,
 //   4. issues chunk c's ks = 1 MFMAs.
+// 256-row blocks (RB = 2) build the B tile two chunks ahead into four slots
+// and synchronise once per two chunks; their C (and S) values are loaded a
+// step and a half ahead, into two register sets.
 // ---------------------------------------------------------------------------
-// B tile slots: 256-row blocks (RB = 2) build the tile two chunks ahead into
-// four slots and synchronise once per two chunks; 128-row blocks (two per CU,
-// so the LDS is shared) one chunk ahead into two slots, one barrier per chunk.
 template <int RB>
 constexpr int b_slots() {
   return RB == 2 ? 4 : 2;
@@ -384,34 +385,37 @@ __device__ __forceinline__ void gemm_tile(GemmSmem<T, NS, JN, b_slots<RB>()>& sm
       KR ? gmem(reinterpret_cast<const f2v*>(jb.S)) + jb.a * 16 + (lane & 15) : nullptr;
   const int64_t s_chunk = jb.na * 16;
 
-  // C values (and, in 3-D, the wave's four S values) of one chunk, loaded a
-  // step before they are used
-  f4v cr[GC][2];
-  float2 sl;
-  auto gload = [&](int c) {
+  // C values (and, in 3-D, the wave's four S values) of one chunk: with
+  // RB = 1 loaded a step before they are used, with RB = 2 into two register
+  // sets (chunk parity) a step and a half before (an L2 hit outlasts half a
+  // step of MFMAs)
+  constexpr int CB = RB == 2 ? 2 : 1;
+  f4v cr[CB][GC][2];
+  float2 sl[CB];
+  auto gload = [&](int c, const int b) {
 #pragma unroll
     for (int g = 0; g < GC; ++g) {
       const __attribute__((address_space(1))) f4v* p = cbase + c * c_chunk + 64 * g;
-      cr[g][0] = p[0];
-      cr[g][1] = p[c_pair];
+      cr[b][g][0] = p[0];
+      cr[b][g][1] = p[c_pair];
     }
     if constexpr (KR) {
       // one 8-byte load, then plain float members for the readlanes below
       // (bit-casting elements of a vector-typed sl lost its .y half)
       const f2v t = sbase[c * s_chunk];
-      sl.x = t[0];
-      sl.y = t[1];
+      sl[b].x = t[0];
+      sl[b].y = t[1];
     }
   };
-  auto gstore = [&](const int buf) {
+  auto gstore = [&](const int buf, const int b) {
     float2 sv[4];
     if constexpr (KR) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         sv[q].x = __builtin_bit_cast(float, __builtin_amdgcn_readlane(
-                                                __builtin_bit_cast(int, sl.x), 4 * wave + q));
+                                                __builtin_bit_cast(int, sl[b].x), 4 * wave + q));
         sv[q].y = __builtin_bit_cast(float, __builtin_amdgcn_readlane(
-                                                __builtin_bit_cast(int, sl.y), 4 * wave + q));
+                                                __builtin_bit_cast(int, sl[b].y), 4 * wave + q));
       }
     }
 #pragma unroll
@@ -419,7 +423,7 @@ __device__ __forceinline__ void gemm_tile(GemmSmem<T, NS, JN, b_slots<RB>()>& sm
       uint32_t w[NS][4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const f4v v = cr[g][q >> 1];
+        const f4v v = cr[b][g][q >> 1];
         float re = (q & 1) ? v.z : v.x, im = (q & 1) ? v.w : v.y;
         if constexpr (KR) {
           const float r2 = sv[q].x * re - sv[q].y * im;
@@ -509,11 +513,11 @@ __device__ __forceinline__ void gemm_tile(GemmSmem<T, NS, JN, b_slots<RB>()>& sm
     aload(set ^ 1, cn);
     bread(0, set);
     mfma_ks(set, 0);
-    gstore(set ^ 1);
+    gstore(set ^ 1, 0);
     bread(1, set);
     mfma_ks(set, 1);
     __syncthreads();
-    gload(c + 2 < nch ? c + 2 : cn);
+    gload(c + 2 < nch ? c + 2 : cn, 0);
   };
   // RB = 2: chunk c's fragments come from slot c % 4; the step builds chunk
   // c + 2 into slot (c + 2) % 4, and odd steps end with the barrier that
@@ -532,7 +536,7 @@ __device__ __forceinline__ void gemm_tile(GemmSmem<T, NS, JN, b_slots<RB>()>& sm
     mfma_ks(set, 0);
     // past the last chunk this rebuilds the last one into a slot that no
     // wave reads again
-    gstore((c + 2) & 3);
+    gstore((c + 2) & 3, set);
 #pragma unroll
     for (int k = 0; k < kM; ++k) {
       __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
@@ -543,7 +547,7 @@ __device__ __forceinline__ void gemm_tile(GemmSmem<T, NS, JN, b_slots<RB>()>& sm
     }
     __builtin_amdgcn_sched_barrier(0);
     if (set == 1) __syncthreads();  // set = c % 2, fixed by the unrolled pair below
-    gload(c + 3 < nch ? c + 3 : nch - 1);
+    gload(c + 4 < nch ? c + 4 : nch - 1, set);
     bread(0, (c + 1) & 3);
     mfma_ks(set, 1);
 #pragma unroll
@@ -556,12 +560,15 @@ __device__ __forceinline__ void gemm_tile(GemmSmem<T, NS, JN, b_slots<RB>()>& sm
     __builtin_amdgcn_sched_barrier(0);
   };
 
-  gload(0);
-  gstore(0);
-  gload(nch > 1 ? 1 : 0);
+  gload(0, 0);
+  gstore(0, 0);
   if constexpr (RB == 2) {
-    gstore(1);
-    gload(nch > 2 ? 2 : nch - 1);
+    gload(1, 1);  // nch is even, at least 2
+    gstore(1, 1);
+    gload(nch > 2 ? 2 : nch - 1, 0);
+    gload(nch > 3 ? 3 : nch - 1, 1);
+  } else {
+    gload(nch > 1 ? 1 : 0, 0);
   }
   aload(0, 0);
   __syncthreads();
