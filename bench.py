@@ -27,6 +27,28 @@ sys.path.insert(0, os.path.join(ROOT, "gstools-core_amd"))
 sys.path.insert(0, ROOT)
 
 METRIC = "point·modes/s: direct vs factored-MFMA summator at ≤1e-5·σ err, 1 GPU"
+# max|err|/sigma of every timed grid format against the FP64 oracle on the
+# seeded 10^6-point subsets, as printed by tests/test_gpu_grid.py (the bench
+# itself does not evaluate the oracle); the bar is 1e-5 sigma
+GRID_ERRORS = {
+    "structured_grid": {"factored_fp16x3_f32x2": 2.201e-06, "factored_fp16x3_f64": 2.201e-06,
+                        "factored_fp16x3_f32": 6.077e-06, "factored_bf16x6_f32x2": 3.036e-06,
+                        "factored_fp16x1_f32": 1.253e-03, "direct": 7.951e-07},
+    "heavy_tail_grid": {"factored_fp16x3": 1.735e-06, "factored_bf16x3": 1.872e-05,
+                        "factored_bf16x6": 2.486e-06, "direct": 7.464e-07},
+}
+
+
+def label_errors(res: dict, leg: str) -> None:
+    """Attaches the measured error and the 1e-5 sigma verdict to each timed format."""
+    errs = GRID_ERRORS[leg]
+    for key, r in res.items():
+        if not isinstance(r, dict):
+            continue
+        base = key.replace("_tile128", "")
+        if base in errs and errs[base] is not None:
+            r["max_err_sigma"] = errs[base]
+            r["meets_1e-5_sigma"] = errs[base] <= 1e-5
 # gfx950 FP32 transcendental issue peak: 256 CUs x 32 lanes/clk (half rate) x 2.4 GHz
 TRANS_PEAK = 256 * 32 * 2.4e9
 
@@ -171,6 +193,7 @@ def grid_leg(steps: int, warmup: int) -> dict:
         del work
     res["factored_fp16x3_over_direct"] = (res["direct"]["ms_per_step"] /
                                           res["factored_fp16x3_f32x2"]["ms_per_step"])
+    label_errors(res, "structured_grid")
     return res
 
 
@@ -178,7 +201,8 @@ def c5_leg(steps: int, warmup: int) -> dict:
     """Heavy-tailed 512^3 grid, M = 4000 exponential-model modes (workload c5),
     FP32 output: the factored path per operand format vs the direct kernel on
     the grid.  Accuracy against the FP64 oracle is checked by the GPU tests
-    (tests/test_gpu_grid.py), not here."""
+    (tests/test_gpu_grid.py), not here; each format carries the error they
+    measure (bf16x3 misses the 1e-5 sigma bar and is labelled so)."""
     import numpy as np
     import torch
 
@@ -224,6 +248,7 @@ def c5_leg(steps: int, warmup: int) -> dict:
         del work
     res["factored_fp16x3_over_direct"] = (res["direct"]["ms_per_step"] /
                                           res["factored_fp16x3"]["ms_per_step"])
+    label_errors(res, "heavy_tail_grid")
     return res
 
 
@@ -318,10 +343,15 @@ def ensemble_leg(steps: int, warmup: int) -> dict:
     return res
 
 
-def incompr_leg(steps: int, warmup: int, n: int = 4_000_000, m: int = 1000) -> dict:
+def incompr_leg(steps: int, warmup: int, n: int = 4_000_000, m: int = 1000,
+                sine: dict | None = None) -> dict:
     """summate_incompr on the 3-D unstructured workload at N = 4e6, M = 1000:
     the cell-local kernel (one sine per point-mode shared by the three
-    components) and the FP64-phase VALU kernel, device-resident, FP64 out."""
+    components) and the FP64-phase VALU kernel, device-resident, FP64 out.
+    The local kernel's launches are also timed alone (HIP events around the
+    summation kernel, gsc_direct_phase_times) against the sine peak: one
+    v_sin_f32 and three FP32 FMAs per point-mode."""
+    import ctypes
     import numpy as np
     import torch
 
@@ -350,6 +380,19 @@ def incompr_leg(steps: int, warmup: int, n: int = 4_000_000, m: int = 1000) -> d
         _, tot = time_steps(step, steps, warmup, stream)
         ms = tot / steps
         res[f"{vname}_phase"] = {"ms_per_step": ms, "point_modes_per_s": n * m / (ms * 1e-3)}
+    from gstools_core import _native
+
+    ph = (ctypes.c_double * 3)()
+    _native.check(_native.lib().gsc_direct_phase_times(
+        _native.GSC_FIELD_INCOMPR, d, m, n, None, cov.data_ptr(), z1.data_ptr(), z2.data_ptr(),
+        pos.data_ptr(), n, out.data_ptr(), n, work.data_ptr(), nb,
+        ctypes.byref(_native.make_opts()), ctypes.c_void_p(stream.cuda_stream), steps, ph))
+    tsin = n * m / (ph[1] * 1e-3) / 1e12
+    r = {"kernel": "k_direct_local_vec<3,double>", "sort_ms": ph[0], "kernel_ms": ph[1],
+         "call_ms": ph[2], "achieved_tsin": tsin, "frac_of_rated": tsin / (TRANS_PEAK / 1e12)}
+    if sine is not None:
+        r["frac_of_calibrated"] = tsin / (sine["sin_per_s"] / 1e12)
+    res["local_phase"]["kernel"] = r
     return res
 
 
@@ -493,7 +536,7 @@ def main() -> int:
     if "direct" not in legs:
         res = {"sine_calibration": sine} if sine else {}
         if "incompr" in legs:
-            res["incompr"] = incompr_leg(max(3, args.steps // 2), 1)
+            res["incompr"] = incompr_leg(max(3, args.steps // 2), 1, sine=sine)
         if "fourier" in legs:
             res["fourier"] = fourier_leg(max(3, args.steps // 2), 1)
         if "grid" in legs:
@@ -609,7 +652,7 @@ def main() -> int:
     inc_res = None
     if "incompr" in legs:
         log("incompressible leg (N = 4e6)")
-        inc_res = incompr_leg(max(3, args.steps // 2), 1)
+        inc_res = incompr_leg(max(3, args.steps // 2), 1, sine=sine)
 
     fourier_res = None
     if "fourier" in legs:

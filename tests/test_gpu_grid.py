@@ -110,13 +110,15 @@ def test_c3_grid_256_factored_and_direct():
     idx = workloads.subset_indices(w.n_points, 1_000_000, seed=3)
     pts = workloads.grid_points(w.axes, idx)
     ref = oracle.summate(md.cov_samples, md.z1, md.z2, pts)
-    for prec in ("fp16x3", "bf16x6", "bf16x3"):
-        for acc in ("f64", "f32x2"):
-            g = grid.summate_grid(md.cov_samples, md.z1, md.z2, w.axes, path="factored",
-                                  precision=prec, out_dtype=np.float32, accum=acc)
-            err = _scaled_err(g.ravel()[idx], ref, md.m)
-            print(f"c3 factored {prec} accum {acc}: max|err|/sigma = {err:.3e}")
-            assert err <= TOL[prec]
+    # every format and accumulation the benchmark times at this size
+    cases = [(p, a) for p in ("fp16x3", "bf16x6", "bf16x3") for a in ("f64", "f32x2")]
+    cases += [("fp16x3", "f32"), ("fp16x1", "f32")]
+    for prec, acc in cases:
+        g = grid.summate_grid(md.cov_samples, md.z1, md.z2, w.axes, path="factored",
+                              precision=prec, out_dtype=np.float32, accum=acc)
+        err = _scaled_err(g.ravel()[idx], ref, md.m)
+        print(f"c3 factored {prec} accum {acc}: max|err|/sigma = {err:.3e}")
+        assert err <= TOL[prec] * (2 if acc == "f32" and prec == "fp16x3" else 1)
     g = grid.summate_grid(md.cov_samples, md.z1, md.z2, w.axes, path="direct",
                           out_dtype=np.float32)
     err = _scaled_err(g.ravel()[idx], ref, md.m)
