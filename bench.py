@@ -252,8 +252,9 @@ def c5_leg(steps: int, warmup: int) -> dict:
     return res
 
 
-def ensemble_leg(steps: int, warmup: int) -> dict:
-    """512 seeded 2-D realisations (workload c4) as one graph-captured plan."""
+def ensemble_leg(steps: int, warmup: int, plan_only: bool = False) -> dict:
+    """512 seeded 2-D realisations (workload c4) as one graph-captured plan;
+    plan_only (leg c4plan) stops after the library-rule plan (for counter runs)."""
     import numpy as np
     import torch
 
@@ -274,6 +275,9 @@ def ensemble_leg(steps: int, warmup: int) -> dict:
     info = ens.info
     ens.close()
     del ens
+    if plan_only:
+        return {"point_modes": info["point_modes"], "workspace_gb": info["workspace_bytes"] / 1e9,
+                "ms_per_step": ms, "point_modes_per_s": info["point_modes"] / (ms * 1e-3)}
     # the autotuner's plan: path and tile shape per realisation, split format
     # and accumulation per plan, from per-class timings (cached)
     t0 = time.perf_counter()
@@ -290,9 +294,10 @@ def ensemble_leg(steps: int, warmup: int) -> dict:
     tuned_res = {"ms_per_step": ms_t, "point_modes_per_s": tinfo["point_modes"] / (ms_t * 1e-3),
                  "workspace_gb": tinfo["workspace_bytes"] / 1e9, "tune_and_plan_s": t_tune,
                  "classes_measured": tuner.measured, "precision": choice["precision"],
-                 "accum": choice["accum"],
-                 "choices": dict(Counter(f"{p}/{t}" if p == "factored" else p
-                                         for p, t in zip(choice["paths"], choice["tiles"])))}
+                 "accum": choice["accum"], "plan_ms": choice["plan_ms"], "kept": choice["kept"],
+                 "choices": (dict(Counter(f"{p}/{t}" if p == "factored" else p
+                                          for p, t in zip(choice["paths"], choice["tiles"])))
+                             if choice["paths"] is not None else "library rule")}
     res = {"workload": "512 2-D grids, sides uniform in {64..1024}, M=1000 Gaussian modes "
                        "(l uniform in [5,50]) per realisation, FP32 output, fp16x3",
            "point_modes": info["point_modes"], "factored": info["factored"],
@@ -499,7 +504,8 @@ def main() -> int:
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble leg")
     ap.add_argument("--legs", default="sine,direct,incompr,fourier,grid,c5,ensemble,krige",
                     help="comma list of legs: sine (v_sin_f32 issue-rate calibration), direct, "
-                         "incompr, fourier, grid, c5, ensemble, krige, copy "
+                         "incompr, fourier, grid, c5, ensemble, c4plan (the ensemble's plan "
+                         "alone), krige, copy "
                          "(a 1 GiB device copy, the reference for HBM byte counters)")
     args = ap.parse_args()
     legs = set(args.legs.split(","))
@@ -547,6 +553,8 @@ def main() -> int:
             res["krige"] = krige_leg(max(3, args.steps // 4), 1)
         if "ensemble" in legs:
             res["ensemble"] = ensemble_leg(max(3, args.steps // 2), 1)
+        if "c4plan" in legs:
+            res["ensemble_plan"] = ensemble_leg(max(3, args.steps // 2), 1, plan_only=True)
         print(json.dumps(res), flush=True)
         return 0
 

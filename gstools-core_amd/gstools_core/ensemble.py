@@ -59,14 +59,18 @@ class Ensemble:
         self._cov = _native.DeviceBuffer.from_array(cat(covs), device)
         self._z1 = _native.DeviceBuffer.from_array(cat(z1s), device)
         self._z2 = _native.DeviceBuffer.from_array(cat(z2s), device)
-        plan = ctypes.c_void_p()
         c_shapes, c_ms, path_codes, tile_codes, scale_arr, opts = args
-        rc = _native.lib().gsc_ensemble_create_tuned(
-            ctypes.byref(plan), self.count, self.dim, c_shapes, c_ms, path_codes, tile_codes,
-            scale_arr, self._cov.ptr, self._z1.ptr, self._z2.ptr, 0, None, None, self._ax.ptr,
-            self._out.ptr, ctypes.byref(opts))
-        _native.check(rc)
-        self._plan = plan
+
+        def create(pc, tc):
+            plan = ctypes.c_void_p()
+            rc = _native.lib().gsc_ensemble_create_tuned(
+                ctypes.byref(plan), self.count, self.dim, c_shapes, c_ms, pc, tc, scale_arr,
+                self._cov.ptr, self._z1.ptr, self._z2.ptr, 0, None, None, self._ax.ptr,
+                self._out.ptr, ctypes.byref(opts))
+            _native.check(rc)
+            return plan
+
+        self._plan = self._build(create, path_codes, tile_codes)
 
     @classmethod
     def seeded(cls, model: str, grids: Sequence, m_counts: Sequence[int],
@@ -95,14 +99,52 @@ class Ensemble:
             raise TypeError(f"unexpected keyword arguments {sorted(kw)}")
         c_len = (ctypes.c_double * n)(*[float(v) for v in len_scales])
         c_seeds = (ctypes.c_uint64 * n)(*[int(v) & 0xFFFFFFFFFFFFFFFF for v in seeds])
-        plan = ctypes.c_void_p()
-        rc = _native.lib().gsc_ensemble_create_tuned(
-            ctypes.byref(plan), self.count, self.dim, c_shapes, c_ms, path_codes, tile_codes,
-            scale_arr, None, None, None, MODELS[model], c_len, c_seeds, self._ax.ptr,
-            self._out.ptr, ctypes.byref(opts))
-        _native.check(rc)
-        self._plan = plan
+
+        def create(pc, tc):
+            plan = ctypes.c_void_p()
+            rc = _native.lib().gsc_ensemble_create_tuned(
+                ctypes.byref(plan), self.count, self.dim, c_shapes, c_ms, pc, tc, scale_arr,
+                None, None, None, MODELS[model], c_len, c_seeds, self._ax.ptr, self._out.ptr,
+                ctypes.byref(opts))
+            _native.check(rc)
+            return plan
+
+        self._plan = self._build(create, path_codes, tile_codes)
         return self
+
+    def _build(self, create, path_codes, tile_codes):
+        """The plan.  A tuned plan is checked as a whole: the per-class
+        choices can lose to simpler plans once the realisations share one
+        launch (a second GEMM launch for the other tile shape has its own
+        tail), so the tuned plan, the same paths with one tile shape for all
+        (256 or 128 rows) and the library rule are each built and timed
+        (gsc_ensemble_time) and the fastest is kept; ``tuned`` records the
+        times and the choice."""
+        if self.tuned is None:
+            return create(path_codes, tile_codes)
+        tall = (ctypes.c_int32 * self.count)(*[_native.TILES["256"]] * self.count)
+        tshort = (ctypes.c_int32 * self.count)(*[_native.TILES["128"]] * self.count)
+        cands = (("tuned", path_codes, tile_codes), ("tuned_256", path_codes, tall),
+                 ("tuned_128", path_codes, tshort), ("library", None, None))
+        best, times = None, {}
+        for name, pc, tc in cands:
+            plan = create(pc, tc)
+            ms = ctypes.c_double()
+            _native.check(_native.lib().gsc_ensemble_time(plan, 5, ctypes.byref(ms)))
+            times[name] = ms.value
+            if best is None or ms.value < best[1]:
+                if best is not None:
+                    _native.lib().gsc_ensemble_destroy(best[2])
+                best = (name, ms.value, plan)
+            else:
+                _native.lib().gsc_ensemble_destroy(plan)
+        self.tuned["plan_ms"] = times
+        self.tuned["kept"] = best[0]
+        if best[0] != "tuned":
+            self.tuned["tiles"] = ["128" if best[0] == "tuned_128" else "256"] * self.count
+            if best[0] == "library":
+                self.tuned["paths"] = None
+        return best[2]
 
     def reseed(self, seeds: Sequence[int], stream: int = 0) -> None:
         """New per-realisation seeds for the following runs of a seeded plan."""

@@ -27,7 +27,11 @@ in a persistent plan only the throughput per tile counts.  For a plan the
 tuner therefore times every candidate on a plan of k copies of the class
 grid (k chosen for about 2e10 point-modes per replay, at most 64; gsc_ensemble_time),
 per split format and accumulation, and picks the pair with the smallest
-summed estimate, then per realisation its path and tile shape.
+summed estimate, then per realisation its path and tile shape.  A tuned
+``Ensemble`` then times that plan as a whole against two simpler ones (the
+same paths with one tile shape for all, and the library rule) and keeps the
+fastest: per-class times do not see that a second tile shape adds a second
+persistent launch, with its own tail.
 
 Cache file: $GSC_TUNE_CACHE, else ~/.cache/gstools_core_amd/tuning.json.
 """

@@ -229,7 +229,9 @@ def test_tuner_measures_chooses_and_caches(tmp_path):
         assert padded_work((300, 400), 1000, key) <= padded_work((512, 512), 1024, key)
     reals = _reals(2, [(300, 400), (8, 9), (64, 70)], [1000, 50, 300], 13)
     ens = Ensemble(reals, tune=True, tuner=t2, out_dtype=np.float64)
-    assert set(ens.tuned) == {"precision", "accum", "paths", "tiles"}
+    assert {"precision", "accum", "paths", "tiles", "plan_ms", "kept"} <= set(ens.tuned)
+    # the kept plan is the fastest of the three timed as whole plans
+    assert ens.tuned["plan_ms"][ens.tuned["kept"]] == min(ens.tuned["plan_ms"].values())
     ens.run()
     for r, (cov, z1, z2, axes) in zip(ens.results(), reals):
         ref = oracle.summate(cov, z1, z2, oracle.grid_points(axes)).reshape(r.shape)
