@@ -452,19 +452,54 @@ size_t grid_work_bytes(int field, int dim, int64_t M, const int64_t* shape, cons
          direct_work_bytes(gsc::KIND_INCOMPR, dim, M, N, o);
 }
 
-// `work` starts with the mode table in every case.
+// True when x[i] = org + i step for every i < n within 8 units in the last
+// place of the axis' largest magnitude (np.arange and np.linspace axes): the
+// product form evaluates the coordinates from org and step, so a phase moves
+// by at most |k| 8 eps max|x| revolutions.
+bool uniform_axis(const double* x, int64_t n, double* org, double* step) {
+  if (n <= 0 || !std::isfinite(x[0]) || !std::isfinite(x[n - 1])) return false;
+  const double d = n > 1 ? (x[n - 1] - x[0]) / (double)(n - 1) : 0.0;
+  const double tol = 8.0 * DBL_EPSILON * std::max(std::fabs(x[0]), std::fabs(x[n - 1]));
+  for (int64_t i = 1; i + 1 < n; ++i)
+    if (!(std::fabs(x[i] - (x[0] + (double)i * d)) <= tol)) return false;
+  *org = x[0];
+  *step = d;
+  return std::isfinite(d);
+}
+
+// The plan of a factored grid call: the product form when the grid is 2-D,
+// one field component, both axes uniform (known on the host: haxes) and its
+// tables take no more room than the whole-grid ones (every grid of more than
+// one tile); the whole-grid tables otherwise.  Ensembles use the same rule,
+// so a realisation equals its single-grid call.
+gsc::FactoredPlan grid_plan(int field, int dim, int64_t M, const int64_t* shape,
+                            const gsc_opts* o, const double* const* haxes) {
+  const gsc::FactoredPlan p =
+      gsc::plan_factored(dim, M, shape, o->precision, field_comps(field, dim), grid_tile(o));
+  if (dim != 2 || p.ncomp != 1 || haxes == nullptr) return p;
+  const int r = p.trans ? 1 : 0;
+  double org[2], step[2];
+  if (!uniform_axis(haxes[r], shape[r], &org[0], &step[0]) ||
+      !uniform_axis(haxes[1 - r], shape[1 - r], &org[1], &step[1]))
+    return p;
+  const gsc::FactoredPlan q = gsc::plan_factored_product(
+      p, gsc::gemm_tile_cols(gsc::precision_splits(o->precision), o->accum), org, step);
+  return q.bytes <= p.bytes ? q : p;
+}
+
+// `work` starts with the mode table in every case.  haxes: the axes on the
+// host when the caller has them (host-array entry points), else null.
 int grid_launch(int field, int dim, int64_t M, const double* sf, const double* cov,
                 const double* z1, const double* z2, const double* const* axes,
                 const int64_t* shape, void* out, void* work, const gsc_opts* o, int path,
-                hipStream_t s) {
+                hipStream_t s, const double* const* haxes) {
   if (path == GSC_PATH_FACTORED) {
     // the factored tables take the amplitude A (sf A for the Fourier method)
     // from amp[0]; the projector of the incompressible field is applied to the
     // last-axis table
     const int kind = field == GSC_FIELD_FOURIER ? gsc::KIND_FOURIER : gsc::KIND_SCALAR;
     GSC_HIP(gsc::launch_prep(kind, dim, M, cov, z1, z2, sf, work, s));
-    const gsc::FactoredPlan p =
-        gsc::plan_factored(dim, M, shape, o->precision, field_comps(field, dim), grid_tile(o));
+    const gsc::FactoredPlan p = grid_plan(field, dim, M, shape, o, haxes);
     GSC_HIP(gsc::launch_factored(p, M, axes, work, out, o->out_dtype, o->scale, o->accum, s));
     return GSC_OK;
   }
@@ -866,14 +901,16 @@ int gsc_summate_grid_field(int field, int dim, int64_t M, const double* sf, int6
     GSC_HIP(hipMemcpyAsync(d_ax, ax.data(), sizeof(double) * ax.size(), hipMemcpyHostToDevice,
                            c.stream));
     const double* d_axes[3] = {nullptr, nullptr, nullptr};
+    const double* h_axes[3] = {nullptr, nullptr, nullptr};
     off = 0;
     for (int i = 0; i < dim; ++i) {
       d_axes[i] = d_ax + off;
+      h_axes[i] = ax.data() + off;
       off += shape[i];
     }
     if ((rc = grid_launch(field, dim, M, d_modes + (dim + 2) * M, d_modes, d_modes + dim * M,
                           d_modes + (dim + 1) * M, d_axes, shape, d_out, d_work, o, path,
-                          c.stream)))
+                          c.stream, h_axes)))
       return rc;
     GSC_HIP(hipMemcpyAsync(out, d_out, elt * (size_t)ncomp * (size_t)N, hipMemcpyDeviceToHost,
                            c.stream));
@@ -918,7 +955,7 @@ int gsc_summate_grid_field_dev(int field, int dim, int64_t M, const double* sf,
   for (int i = 0; i < dim; ++i)
     if (!axes[i]) return fail(GSC_ERR_INVALID, "null axis %d", i);
   return grid_launch(field, dim, M, sf, cov, z1, z2, axes, shape, out, work, o, path,
-                     static_cast<hipStream_t>(stream));
+                     static_cast<hipStream_t>(stream), nullptr);
 }
 
 int gsc_tune_grid_time(int field, int dim, int64_t M, const int64_t* shape, const gsc_opts* opts,
@@ -959,10 +996,11 @@ int gsc_tune_grid_time(int field, int dim, int64_t M, const int64_t* shape, cons
   for (int i = 0; i < 2 && e == hipSuccess; ++i) e = hipEventCreate(&ev[i]);
   double* modes = reinterpret_cast<double*>(mem);
   double* ax = reinterpret_cast<double*>(mem + b_modes);
+  // unit-spaced axes (kept on the host for the plan choice) and Gaussian
+  // modes drawn on the device (length scale an eighth of the largest extent);
+  // Fourier factors of one
+  std::vector<double> hax((size_t)std::max<int64_t>(ax_total, 1));
   if (e == hipSuccess) {
-    // unit-spaced axes and Gaussian modes drawn on the device (length scale
-    // an eighth of the largest extent); Fourier factors of one
-    std::vector<double> hax((size_t)std::max<int64_t>(ax_total, 1));
     std::vector<double> ones((size_t)Mm, 1.0);
     int64_t off = 0, ext = 1;
     for (int i = 0; i < dim; ++i) {
@@ -997,10 +1035,17 @@ int gsc_tune_grid_time(int field, int dim, int64_t M, const int64_t* shape, cons
     }
     void* out = mem + b_modes + b_ax;
     void* work = mem + b_modes + b_ax + b_out;
+    // the host copy of the unit axes: the plan host-array calls would take
+    const double* haxes[3] = {nullptr, nullptr, nullptr};
+    off = 0;
+    for (int i = 0; i < dim; ++i) {
+      haxes[i] = hax.data() + off;
+      off += shape[i];
+    }
     for (int r = 0; r <= repeats && rc == GSC_OK; ++r) {
       if (r == 1 && (e = hipEventRecord(ev[0], st)) != hipSuccess) break;
       rc = grid_launch(field, dim, M, modes + (dim + 2) * Mm, modes, modes + dim * M,
-                       modes + (dim + 1) * M, axes, shape, out, work, o, path, st);
+                       modes + (dim + 1) * M, axes, shape, out, work, o, path, st, haxes);
     }
     if (rc == GSC_OK && e == hipSuccess) e = hipEventRecord(ev[1], st);
     if (rc == GSC_OK && e == hipSuccess) e = hipEventSynchronize(ev[1]);
@@ -1054,21 +1099,6 @@ void ensemble_release(gsc_ensemble* p) {
   if (p->mem) (void)hipFree(p->mem);
   if (p->mode_mem) (void)hipFree(p->mode_mem);
   delete p;
-}
-
-// True when x[i] = org + i step for every i < n within 8 units in the last
-// place of the axis' largest magnitude (np.arange and np.linspace axes): the
-// product form evaluates the coordinates from org and step, so a phase moves
-// by at most |k| 8 eps max|x| revolutions.
-bool uniform_axis(const double* x, int64_t n, double* org, double* step) {
-  if (n <= 0 || !std::isfinite(x[0]) || !std::isfinite(x[n - 1])) return false;
-  const double d = n > 1 ? (x[n - 1] - x[0]) / (double)(n - 1) : 0.0;
-  const double tol = 8.0 * DBL_EPSILON * std::max(std::fabs(x[0]), std::fabs(x[n - 1]));
-  for (int64_t i = 1; i + 1 < n; ++i)
-    if (!(std::fabs(x[i] - (x[0] + (double)i * d)) <= tol)) return false;
-  *org = x[0];
-  *step = d;
-  return std::isfinite(d);
 }
 
 // sample: null, or the mode sampler that fills cov / z1 / z2 at the start
@@ -1146,25 +1176,32 @@ int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes
       e.row_tiles = fp.row_tiles;
       e.col_tiles = (fp.ncols + tile_cols - 1) / tile_cols;
       f_off[(size_t)r] = ws;  // S, A, C as laid out after the plan's own mode table
-      const double* rax = dim == 2 ? hax.data() + (e.ax0 - axes) + (fp.trans ? sh[0] : 0) : nullptr;
-      const double* cax = dim == 2 ? hax.data() + (e.ax0 - axes) + (fp.trans ? 0 : sh[0]) : nullptr;
-      if (dim == 2 && uniform_axis(rax, fp.nrow, &e.org[0], &e.step[0]) &&
-          uniform_axis(cax, fp.ncols, &e.org[1], &e.step[1])) {
+      // the single-call rule (grid_plan): the product form on uniform axes
+      // when its tables are no larger
+      gsc::FactoredPlan q = fp;
+      if (dim == 2) {
+        const double* hx[2] = {hax.data() + (e.ax0 - axes), hax.data() + (e.ax0 - axes) + sh[0]};
+        gsc_opts po2 = *o;
+        po2.tile = tile_of[(size_t)r];
+        q = grid_plan(GSC_FIELD_SCALAR, dim, M, sh, &po2, hx);
+      }
+      if (q.prod) {
         // product form: S per tile, A and C one tile
         e.prod = 1;
-        e.tcols = tile_cols;
-        e.row_tiles = fp.arows / fp.brows;
-        e.na = e.row_tiles * e.col_tiles;
-        e.arows = fp.brows;
-        e.ncp = gsc::kMaxTileCols;
-        const size_t sb = align256(sizeof(float2) * (size_t)e.na * (size_t)fp.mp);
-        const size_t ab =
-            align256(sizeof(uint32_t) * 16 * (size_t)fp.ns * (size_t)fp.nch * (size_t)e.arows);
-        const size_t cb = align256(sizeof(float2) * (size_t)e.ncp * (size_t)fp.mp);
+        e.tcols = q.tcols;
+        e.org[0] = q.org[0];
+        e.org[1] = q.org[1];
+        e.step[0] = q.step[0];
+        e.step[1] = q.step[1];
+        e.row_tiles = q.row_tiles;
+        e.col_tiles = q.col_tiles;
+        e.na = q.na;
+        e.arows = q.arows;
+        e.ncp = q.ncp;
         plans[(size_t)r].off_s = 0;  // offsets within the realisation's block
-        plans[(size_t)r].off_a = sb;
-        plans[(size_t)r].off_c = sb + ab;
-        ws += sb + ab + cb;
+        plans[(size_t)r].off_a = q.off_a - q.off_s;
+        plans[(size_t)r].off_c = q.off_c - q.off_s;
+        ws += align256(q.bytes - q.off_s);
       } else {
         plans[(size_t)r].off_a -= fp.off_s;
         plans[(size_t)r].off_c -= fp.off_s;

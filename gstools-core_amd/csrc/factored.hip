@@ -161,6 +161,33 @@ FactoredPlan plan_factored(int dim, int64_t M, const int64_t* shape, int precisi
   return p;
 }
 
+FactoredPlan plan_factored_product(const FactoredPlan& p, int tcols, const double* org,
+                                   const double* step) {
+  FactoredPlan q = p;
+  q.prod = 1;
+  q.tcols = tcols;
+  q.col_tiles = (p.ncols + tcols - 1) / tcols;
+  q.row_tiles = p.arows / p.brows;
+  q.na = q.row_tiles * q.col_tiles;
+  q.arows = p.brows;
+  q.ncp = kMaxTileCols;
+  q.ncpt = q.ncp;
+  for (int i = 0; i < 2; ++i) {
+    q.org[i] = org[i];
+    q.step[i] = step[i];
+  }
+  auto al = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
+  size_t off = p.off_s;  // after the mode table
+  q.off_s = off;
+  off += al(sizeof(float2) * static_cast<size_t>(q.na) * q.mp);
+  q.off_a = off;
+  off += al(sizeof(uint32_t) * 16 * static_cast<size_t>(q.ns) * q.nch * q.arows);
+  q.off_c = off;
+  off += al(sizeof(float2) * static_cast<size_t>(q.ncpt) * q.mp);
+  q.bytes = off;
+  return q;
+}
+
 // Dispatches a launcher template on the operand format of a plan.
 template <template <typename, int> class F, typename... Args>
 hipError_t by_format(bool bf16, int ns, Args&&... args) {
@@ -197,22 +224,25 @@ hipError_t launch_factored(const FactoredPlan& p, int64_t M, const double* const
   uint32_t* A = reinterpret_cast<uint32_t*>(w + p.off_a);
   float2* C = reinterpret_cast<float2*>(w + p.off_c);
   const int Mi = static_cast<int>(M);
-  const bool kr = p.dim == 3;
+  // 3-D, and the 2-D product form, build B = S(block) C
+  const bool kr = p.dim == 3 || p.prod;
   // tables (the mode table was written by launch_prep): one launch
   const int64_t nmax = std::max(std::max(kr ? p.na : 0, p.arows), p.ncpt);
   const dim3 tg(static_cast<unsigned>((nmax + 63) / 64), static_cast<unsigned>(p.nch), 3);
-  const double* xa = axes[kr ? 1 : (p.trans ? 1 : 0)];
-  const double* xc = axes[kr ? 2 : (p.trans ? 0 : 1)];
+  const double* xa = axes[p.dim == 3 ? 1 : (p.trans ? 1 : 0)];
+  const double* xc = axes[p.dim == 3 ? 2 : (p.trans ? 0 : 1)];
+  const ProdGeom pg{p.prod, p.tcols, p.col_tiles, static_cast<int64_t>(p.brows),
+                    p.org[0], p.org[1], p.step[0], p.step[1]};
   TableArgs ta{modes, Mi,      p.dim,   p.nch, axes[0], xa, xc, p.na, p.nrow,
-               p.arows, p.ncols, p.ncp, S,     A,       C,  p.trans, p.ncomp};
+               p.arows, p.ncols, p.ncp, S,     A,       C,  p.trans, p.ncomp, pg};
   hipError_t e = by_format<TablesF>(p.bf16, p.ns, ta, tg, s);
   if (e != hipSuccess) return e;
   // output: (ncomp, grid) dense; a transposed 2-D plan stores rows with
   // stride 1 and columns with stride nrow
-  const int64_t npts = (kr ? p.na : 1) * p.nrow * p.ncols;
+  const int64_t npts = (p.dim == 3 ? p.na : 1) * p.nrow * p.ncols;
   GemmArgs ga{A,       p.arows, p.nch, S,     p.na,  C,  p.ncp, p.nrow, p.nb_out, out,
               out_dtype != 1, p.trans ? 1 : p.ncols, p.ncols, p.trans ? p.nrow : 1, scale, kr,
-              accum, p.ncpt, npts, p.brows};
+              accum, p.ncpt, npts, p.brows, p.prod != 0, p.col_tiles};
   const int64_t tile_cols = gemm_tile_cols(p.ns, accum);
   const int64_t tpc = (p.ncols + tile_cols - 1) / tile_cols;
   const dim3 grid(static_cast<unsigned>(p.row_tiles), static_cast<unsigned>(tpc * p.ncomp));

@@ -64,6 +64,14 @@ struct FactoredPlan {
   int64_t row_tiles; // 3-D: na * arows / brows; 2-D: arows / brows
   int trans;         // 2-D with axis 1 as the A axis (it pads less); the output is stored transposed
   size_t off_s, off_a, off_c, bytes;
+  // 2-D product form (uniform axes, see plan_factored_product): S over the
+  // tiles (na = row_tiles col_tiles), A over the brows row offsets of a tile,
+  // C over the kMaxTileCols column offsets; org / step of the row (A) axis
+  // and the column axis
+  int prod;
+  int tcols;          // GEMM tile columns
+  int64_t col_tiles;
+  double org[2], step[2];
 };
 
 // Columns of a GEMM block tile for an operand format and accumulation mode:
@@ -77,6 +85,13 @@ constexpr int gemm_tile_cols(int ns, int accum) {
 
 FactoredPlan plan_factored(int dim, int64_t M, const int64_t* shape, int precision,
                            int ncomp = 1, int tile = 0);
+// The product form of a 2-D scalar plan p whose row (A) axis is org[0] +
+// i step[0] and column axis org[1] + j step[1] (phase of point (r, c) of tile
+// (rh, ch) = tile corner part + row offset part + column offset part): S per
+// tile with amplitude and theta folded in, A and C one tile each, the GEMM as
+// in 3-D (B = S(tile) C).  Its workspace follows the same mode table.
+FactoredPlan plan_factored_product(const FactoredPlan& p, int tcols, const double* org,
+                                   const double* step);
 
 // One realisation of an ensemble (device-resident descriptor).
 struct EnsReal {
@@ -160,6 +175,16 @@ hipError_t launch_ensemble(const EnsLaunch& e, int precision, int out_dtype, int
                            hipStream_t stream);
 
 
+// Coordinates of the product form's tables (FactoredPlan.prod): S rows are
+// tiles a = rh col_tiles + ch at (org0 + rh brows step0, org1 + ch tcols
+// step1), A rows and C columns the offsets i step0 and j step1.
+struct ProdGeom {
+  int prod;
+  int tcols;
+  int64_t col_tiles, brows;
+  double org0, org1, step0, step1;
+};
+
 // Arguments of the per-operand-format launchers (gsc_gemm.h, gemm_*.hip).
 struct TableArgs {
   const ModeRec* modes;
@@ -173,6 +198,7 @@ struct TableArgs {
   float2* C;
   int trans;  // 2-D: A from axis 1, C from axis 0
   int ncomp;  // C components (1, or dim with the projector folded in)
+  ProdGeom pg;  // product form (pg.prod != 0): coordinates from origin and spacing
 };
 struct GemmArgs {
   const uint32_t* A;
@@ -194,6 +220,8 @@ struct GemmArgs {
   int64_t ncpt;        // columns of the C table: ncomp blocks of ncp
   int64_t comp_stride; // output elements between field components
   int brows;           // block rows: 256 or 128 (the plan's tile shape)
+  bool prod;           // 2-D product form: tile (x, y) of the grid is S row x col_tiles + y
+  int64_t col_tiles;
 };
 
 // Expects the mode table at the start of `work` (launch_prep, KIND_SCALAR).

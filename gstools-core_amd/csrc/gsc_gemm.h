@@ -196,33 +196,43 @@ __device__ __forceinline__ void table_quad(const ModeRec* __restrict__ modes, in
                                            int64_t ncp, float2* __restrict__ S,
                                            uint32_t* __restrict__ A, float2* __restrict__ C,
                                            int role, int64_t row, int c, int g, int trans,
-                                           int ncomp) {
+                                           int ncomp, const ProdGeom& pg) {
   if (c >= nch || M <= 0) return;
   const int m0 = c * 16 + 4 * g;
   int axis, comp = 0;
   bool amp, valid;
-  double x = 0.0;
-  if (role == 0) {
-    if (dim != 3 || row >= na) return;
-    axis = 0;
+  double x = 0.0, x2 = 0.0;
+  const bool prod = pg.prod != 0;
+  if (role == 0) {  // 3-D axis 0, or the product form's tile corners
+    if ((dim != 3 && !prod) || row >= na) return;
+    axis = prod ? (trans ? 1 : 0) : 0;
     amp = true;
     valid = true;
-    x = gmem(xs)[row];
+    if (prod) {
+      const int64_t rh = row / pg.col_tiles, ch = row - rh * pg.col_tiles;
+      x = pg.org0 + static_cast<double>(rh * pg.brows) * pg.step0;
+      x2 = pg.org1 + static_cast<double>(ch * pg.tcols) * pg.step1;
+    } else {
+      x = gmem(xs)[row];
+    }
   } else if (role == 1) {  // a transposed 2-D plan takes its rows from axis 1
     if (row >= arows) return;
     axis = trans ? 1 : dim - 2;
-    amp = dim == 2;
-    valid = row < nrow;
-    if (valid) x = gmem(xa)[row];
+    amp = dim == 2 && !prod;
+    valid = prod || row < nrow;
+    if (prod) x = static_cast<double>(row) * pg.step0;
+    else if (valid) x = gmem(xa)[row];
   } else {
     if (row >= ncp * ncomp) return;
     comp = static_cast<int>(row / ncp);
     const int64_t col = row - comp * ncp;
     axis = trans ? 0 : dim - 1;
     amp = false;
-    valid = col < ncols;
-    if (valid) x = gmem(xc)[col];
+    valid = prod || col < ncols;
+    if (prod) x = static_cast<double>(col) * pg.step1;
+    else if (valid) x = gmem(xc)[col];
   }
+  const int axis2 = trans ? 0 : 1;  // product-form S: the column axis
   const __attribute__((address_space(1))) ModeRec* gm = gmem(modes);
   double kx[4], th[4];
   float am[4], pr[4];
@@ -230,7 +240,8 @@ __device__ __forceinline__ void table_quad(const ModeRec* __restrict__ modes, in
   for (int e = 0; e < 4; ++e) {
     const int m = min(m0 + e, M - 1);
     kx[e] = gm[m].kap[axis];
-    th[e] = gm[m].th;
+    // the product form's S: the column axis' part joins the offset
+    th[e] = prod && role == 0 ? __builtin_fma(gm[m].kap[axis2], x2, gm[m].th) : gm[m].th;
     am[e] = gm[m].amp[0];
     pr[e] = 1.0f;
     if (ncomp > 1 && role == 2) {
@@ -257,10 +268,10 @@ k_tables(const ModeRec* __restrict__ modes, int M, int dim, int nch,
          const double* __restrict__ xs, const double* __restrict__ xa,
          const double* __restrict__ xc, int64_t na, int64_t nrow, int64_t arows, int64_t ncols,
          int64_t ncp, float2* __restrict__ S, uint32_t* __restrict__ A, float2* __restrict__ C,
-         int trans, int ncomp) {
+         int trans, int ncomp, ProdGeom pg) {
   table_quad<T, NS>(modes, M, dim, nch, xs, xa, xc, na, nrow, arows, ncols, ncp, S, A, C,
                     blockIdx.z, static_cast<int64_t>(blockIdx.x) * 64 + (threadIdx.x & 63),
-                    blockIdx.y, threadIdx.x >> 6, trans, ncomp);
+                    blockIdx.y, threadIdx.x >> 6, trans, ncomp, pg);
 }
 
 // ---------------------------------------------------------------------------
@@ -649,10 +660,18 @@ k_factored_gemm(GemmArgs g) {
   constexpr int JN = cols_blocks<NS, ACC>();
   __shared__ __attribute__((aligned(16))) GemmSmem<T, NS, JN, b_slots<RB>()> sm;
   const int64_t tpc = (g.ncols + 32 * JN - 1) / (32 * JN);
-  const TileJob jb = make_job<KR, JN, RB>(g.A, g.arows, g.nch, g.S, g.na, g.C, g.ncpt, g.nrow,
-                                          g.nb_out, g.out, g.out_f64, g.ldo, g.ncols, g.scale,
-                                          blockIdx.x, blockIdx.y, g.ldc, tpc, g.ncp,
-                                          g.comp_stride);
+  TileJob jb = make_job<KR, JN, RB>(g.A, g.arows, g.nch, g.S, g.na, g.C, g.ncpt, g.nrow,
+                                    g.nb_out, g.out, g.out_f64, g.ldo, g.ncols, g.scale,
+                                    blockIdx.x, blockIdx.y, g.ldc, tpc, g.ncp, g.comp_stride);
+  if (KR && g.prod) {
+    // product form: S row = tile (blockIdx.x, blockIdx.y), A and C tables one tile each
+    jb.a = static_cast<int64_t>(blockIdx.x) * g.col_tiles + blockIdx.y;
+    jb.arow0 = 0;
+    jb.ccol0 = 0;
+    jb.col0 = static_cast<int64_t>(blockIdx.y) * (32 * JN);
+    jb.out_row0 = static_cast<int64_t>(blockIdx.x) * g.arows;
+    jb.nvalid = g.nrow - jb.out_row0;
+  }
   gemm_tile<T, NS, KR, ACC, JN, RB, TR>(sm, jb);
 }
 
@@ -812,7 +831,7 @@ template <typename T, int NS>
 hipError_t tables_launch(const TableArgs& a, dim3 grid, hipStream_t s) {
   gemm::k_tables<T, NS><<<grid, 256, 0, s>>>(a.modes, a.M, a.dim, a.nch, a.xs, a.xa, a.xc, a.na,
                                               a.nrow, a.arows, a.ncols, a.ncp, a.S, a.A, a.C,
-                                              a.trans, a.ncomp);
+                                              a.trans, a.ncomp, a.pg);
   return hipGetLastError();
 }
 
@@ -821,7 +840,7 @@ hipError_t factored_gemm_launch(const GemmArgs& a, dim3 grid, hipStream_t s) {
   using namespace gemm;
 #define GSC_GR(KR, ACC, RB)                                                                \
   do {                                                                                     \
-    if (!KR && a.ldc != 1) k_factored_gemm<T, NS, KR, ACC, true, RB><<<grid, 256, 0, s>>>(a); \
+    if (a.ldc != 1) k_factored_gemm<T, NS, KR, ACC, true, RB><<<grid, 256, 0, s>>>(a);         \
     else k_factored_gemm<T, NS, KR, ACC, false, RB><<<grid, 256, 0, s>>>(a);                  \
   } while (0)
 #define GSC_G(KR, ACC)                        \

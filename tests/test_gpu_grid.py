@@ -261,3 +261,40 @@ def test_c3_incompr_grid_256():
     err = _scaled_err(g.reshape(3, -1)[:, idx], ref, md.m)
     print(f"c3 incompr factored fp16x3: max|err|/sigma = {err:.3e} on {idx.size} points")
     assert err <= 1e-5
+
+
+def test_product_form_single_call_2d():
+    # a 2-D grid on uniform axes (np.arange / np.linspace) of several tiles
+    # takes the product form on the host-array entry point (S per tile, one
+    # tile of A and C); the device entry point, which does not see the axes on
+    # the host, keeps the whole-grid tables.  Both meet the bar, and they
+    # differ (the product form rounds other table entries), which shows that
+    # the product path ran.  Irregular axes take the whole-grid tables on both.
+    import torch
+    from gstools_core import device  # noqa: F401  (loads the library)
+    dev = torch.device("cuda:0")
+    for k, axes in enumerate(([np.arange(600.0) * 0.7 - 3.0, np.linspace(-5.0, 80.0, 700)],
+                              [np.sort(np.random.default_rng(5).uniform(0, 400, 600)),
+                               np.linspace(-5.0, 80.0, 700)])):
+        cov, z1, z2 = _modes(2, 500, 6.0, 21)
+        h = grid.summate_grid(cov, z1, z2, axes, path="factored", precision="fp16x3")
+        shape = [len(a) for a in axes]
+        nb = grid.grid_workspace_bytes(2, 500, shape, path="factored", precision="fp16x3")
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+        tc, t1, t2, tax = t(cov), t(z1), t(z2), [t(a) for a in axes]
+        out = torch.empty(shape, dtype=torch.float64, device=dev)
+        work = torch.empty(nb, dtype=torch.uint8, device=dev)
+        grid.summate_grid_dev(2, 500, tc.data_ptr(), t1.data_ptr(), t2.data_ptr(),
+                              [a.data_ptr() for a in tax], shape, out.data_ptr(),
+                              work.data_ptr(), nb, path="factored", precision="fp16x3")
+        torch.cuda.synchronize()
+        d = out.cpu().numpy()
+        idx = np.random.default_rng(9).choice(h.size, 20000, replace=False)
+        ref = oracle.summate(cov, z1, z2, oracle.grid_points(axes)[:, idx])
+        eh, ed = _scaled_err(h.ravel()[idx], ref, 500), _scaled_err(d.ravel()[idx], ref, 500)
+        print(f"2-D grid host {eh:.3e} device {ed:.3e} sigma")
+        assert eh <= 1e-5 and ed <= 1e-5
+        if k == 0:
+            assert not np.array_equal(h, d)
+        else:
+            np.testing.assert_array_equal(h, d)
