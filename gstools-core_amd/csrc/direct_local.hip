@@ -512,7 +512,10 @@ k_bin_params(const double* __restrict__ box, int64_t N, int64_t max_cells,
       P.n[k] = n[k];
       P.f[k] = f[k];
       P.C[k] = C[k];
-      P.h[k] = ext[k] > 0.0 ? ext[k] / n[k] : 1.0;
+      // a dimension without extent (a planar slice) gets width 0: every
+      // point sits at the cell centre there (xi = 0) and the dimension adds
+      // nothing to a mode's local-phase bound (half_cell_exp(0) scales by 1)
+      P.h[k] = ext[k] > 0.0 ? ext[k] / n[k] : 0.0;
       P.inv_h[k] = ext[k] > 0.0 ? 1.0 / P.h[k] : 0.0;
       P.inv_f[k] = 1.0 / f[k];
     }

@@ -378,6 +378,50 @@ def test_local_degenerate_point_sets():
         assert err <= 1e-5, name
 
 
+def test_local_planar_slice_keeps_modes_smooth():
+    # a constant-z slice of a 3-D model with a short length scale: the
+    # dimension without extent adds nothing to a mode's local-phase bound, so
+    # the modes stay on the matrix-core path.  (With a unit cell width there,
+    # every mode with |kappa_z| > 2 rev/unit, about half of them here, went to
+    # the FP64 fallback and the summation kernel ran ~1.4x longer than on a
+    # slab of tiny but non-zero thickness.)
+    import ctypes
+    import torch
+    from gstools_core import _native, device
+    rng = np.random.default_rng(17)
+    n, m = 1_000_000, 2000
+    xy = [rng.uniform(0, 20, n), rng.uniform(0, 20, n)]
+    planar = np.stack(xy + [np.full(n, 3.5)])
+    slab = np.stack(xy + [3.5 + rng.uniform(0, 1e-3, n)])
+    cov = rng.normal(0, 1 / 0.05, size=(3, m))
+    z1, z2 = rng.normal(size=m), rng.normal(size=m)
+    idx = rng.choice(n, 5000, replace=False)
+    ref = oracle.summate(cov, z1, z2, planar[:, idx])
+    g = gstools_core.summate(cov, z1, z2, planar)
+    assert _err(g[idx], ref) / np.sqrt(m) <= 1e-5
+    gi = gstools_core.summate_incompr(cov, z1, z2, planar)
+    refi = oracle.summate_incompr(cov, z1, z2, planar[:, idx])
+    assert _err(gi[:, idx], refi) / np.sqrt(m) <= 1e-5
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    tc, t1, t2 = t(cov), t(z1), t(z2)
+    out = torch.empty(n, dtype=torch.float64, device=dev)
+    nb = device.workspace_bytes(3, m, n)
+    work = torch.empty(nb, dtype=torch.uint8, device=dev)
+    kernel_ms = {}
+    for name, pos in (("planar", planar), ("slab", slab)):
+        tp = t(pos)
+        ph = (ctypes.c_double * 3)()
+        _native.check(_native.lib().gsc_direct_phase_times(
+            _native.GSC_FIELD_SCALAR, 3, m, n, None, tc.data_ptr(), t1.data_ptr(),
+            t2.data_ptr(), tp.data_ptr(), n, out.data_ptr(), n, work.data_ptr(), nb,
+            ctypes.byref(_native.make_opts()), None, 5, ph))
+        kernel_ms[name] = ph[1]
+    print(f"summation kernel: planar {kernel_ms['planar']:.3f} ms, "
+          f"slab {kernel_ms['slab']:.3f} ms")
+    assert kernel_ms["planar"] < 1.2 * kernel_ms["slab"]
+
+
 def test_local_non_finite_points_only_affect_themselves():
     rng = np.random.default_rng(32)
     cov, z1, z2 = _rand_modes(rng, 2, 200)
