@@ -106,10 +106,18 @@ def sine_leg(repeats: int = 10) -> dict:
     a, b = ctypes.c_double(), ctypes.c_double()
     _native.check(_native.lib().gsc_calibrate_sine(int(repeats), ctypes.byref(a),
                                                    ctypes.byref(b)))
+    ms, mv = ctypes.c_double(), ctypes.c_double()
+    _native.check(_native.lib().gsc_calibrate_local_mix(max(1, int(repeats) // 2),
+                                                        ctypes.byref(ms), ctypes.byref(mv)))
     return {"sin_per_s": a.value, "sin_fma_pairs_per_s": b.value,
             "rated_sin_per_s": TRANS_PEAK, "frac_of_rated": a.value / TRANS_PEAK,
             "kernel": "k_sine_rate: 2048 blocks x 256 threads, 8 independent chains x 4096 "
-                      "iterations per lane"}
+                      "iterations per lane",
+            "local_mix": {"scalar_pm_per_s": ms.value, "vector_pm_per_s": mv.value,
+                          "kernel": "k_local_mix: per 1024 point-modes of a wave 1 x "
+                                    "32x32x16 f16 MFMA + 16 sines + 16 FMAs (scalar) or 4 x "
+                                    "16x16x32 bf16 MFMA + 16 sines + 48 FMAs (vector), 4 "
+                                    "waves per SIMD, no memory traffic"}}
 
 
 def time_steps(step, steps: int, warmup: int, stream, settle_ms: float = 30.0):
@@ -397,6 +405,7 @@ def incompr_leg(steps: int, warmup: int, n: int = 4_000_000, m: int = 1000,
          "call_ms": ph[2], "achieved_tsin": tsin, "frac_of_rated": tsin / (TRANS_PEAK / 1e12)}
     if sine is not None:
         r["frac_of_calibrated"] = tsin / (sine["sin_per_s"] / 1e12)
+        r["frac_of_mix_ceiling"] = tsin / (sine["local_mix"]["vector_pm_per_s"] / 1e12)
     res["local_phase"]["kernel"] = r
     return res
 
@@ -655,6 +664,9 @@ def main() -> int:
         cal = sine["sin_per_s"] / 1e12
         roofline["calibrated_peak"] = round(cal, 4)
         roofline["frac_of_calibrated"] = round(achieved / cal, 4)
+        mix = sine["local_mix"]["scalar_pm_per_s"] / 1e12
+        roofline["mix_ceiling"] = round(mix, 4)
+        roofline["frac_of_mix_ceiling"] = round(achieved / mix, 4)
         roofline["step"]["frac_of_calibrated"] = round(step_tsin / cal, 4)
 
     inc_res = None

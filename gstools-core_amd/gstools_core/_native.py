@@ -126,6 +126,8 @@ SIGNATURES = {
                                       ctypes.POINTER(ctypes.c_double)]),
     "gsc_calibrate_sine": (_int, [_int, ctypes.POINTER(ctypes.c_double),
                                   ctypes.POINTER(ctypes.c_double)]),
+    "gsc_calibrate_local_mix": (_int, [_int, ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(ctypes.c_double)]),
     "gsc_device_alloc": (_vp, [_sz, _int]),
     "gsc_device_free": (None, [_vp]),
     "gsc_memcpy_to_device": (_int, [_vp, _vp, _sz]),

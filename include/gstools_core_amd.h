@@ -423,6 +423,15 @@ int gsc_krige_dev(int64_t n, int64_t N, const double* mat, const double* vecs, i
  * gsc_status (no message is set). */
 int gsc_calibrate_sine(int repeats, double* sin_per_s, double* sin_fma_per_s);
 
+/* Ceiling of the cell-local kernels' instruction mix (no counterpart in the
+ * reference): point-modes per second of a kernel that runs, per 1024
+ * point-modes of a wave, the scalar kernel's mix (one 32x32x16 fp16 matrix
+ * product, 16 sines, 16 FMAs; *scalar_pm_per_s) or the three-component
+ * vector kernel's (four 16x16x32 bf16 products, 16 sines, 48 FMAs;
+ * *vector_pm_per_s), four waves per SIMD and no memory traffic, from HIP
+ * events over `repeats` launches.  Returns a gsc_status (no message). */
+int gsc_calibrate_local_mix(int repeats, double* scalar_pm_per_s, double* vector_pm_per_s);
+
 /* Device memory plumbing for hosts without their own allocator (e.g. plain
  * numpy callers of the device-resident entry points). */
 void* gsc_device_alloc(size_t bytes, int device);

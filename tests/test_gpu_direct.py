@@ -511,3 +511,20 @@ def test_incompr_output_layouts_through_the_c_abi(n):
     assert not big[:, 0::2].any()
     np.testing.assert_array_equal(gstools_core.summate_incompr(md.cov_samples, md.z1, md.z2,
                                                                w.pos), f)
+
+
+def test_roofline_calibrations_are_consistent():
+    # The sine issue rate and the local kernels' instruction-mix ceilings:
+    # rates near the rated 19.66 T sines/s (any clock from 1 GHz), the mixes
+    # below the bare sine rate, the three-component mix below the scalar one.
+    import ctypes
+    from gstools_core import _native
+    lib = _native.lib()
+    s, sf = ctypes.c_double(), ctypes.c_double()
+    _native.check(lib.gsc_calibrate_sine(1, ctypes.byref(s), ctypes.byref(sf)))
+    ms, mv = ctypes.c_double(), ctypes.c_double()
+    _native.check(lib.gsc_calibrate_local_mix(1, ctypes.byref(ms), ctypes.byref(mv)))
+    assert 8e12 < s.value < 2.0e13
+    assert sf.value < s.value
+    assert 2e12 < mv.value < ms.value < s.value
+    assert lib.gsc_calibrate_local_mix(0, ctypes.byref(ms), ctypes.byref(mv)) != 0
