@@ -312,6 +312,17 @@ def ensemble_leg(steps: int, warmup: int, plan_only: bool = False) -> dict:
            "direct": info["direct"], "workspace_gb": info["workspace_bytes"] / 1e9,
            "plan_s": t_plan, "ms_per_step": ms,
            "point_modes_per_s": info["point_modes"] / (ms * 1e-3), "tuned": tuned_res}
+    # the library rule with at most 1 GiB of factored tables held at once
+    # (table batches that reuse one region)
+    bounded = Ensemble(reals, scales=scales, out_dtype=np.float32, precision="fp16x3",
+                       table_bytes=1 << 30)
+    _, total_b = time_steps(lambda: bounded.run(stream.cuda_stream), steps, warmup, stream)
+    ms_b = total_b / steps
+    res["bounded_tables"] = {"table_bytes": 1 << 30, "ms_per_step": ms_b,
+                             "point_modes_per_s": info["point_modes"] / (ms_b * 1e-3),
+                             "workspace_gb": bounded.info["workspace_bytes"] / 1e9}
+    bounded.close()
+    del bounded
     # the same grids with their modes drawn on the device from per-realisation
     # seeds inside the graph (sampler + tables + GEMM per replay)
     seeded = Ensemble.seeded("gaussian", [w.axes for w in ens_w], [w.modes.m for w in ens_w],

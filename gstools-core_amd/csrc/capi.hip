@@ -127,6 +127,7 @@ int validate_opts(const gsc_opts* o) {
     return fail(GSC_ERR_INVALID, "blocks_per_cu must be in 0..64");
   if (o->tile < GSC_TILE_AUTO || o->tile > GSC_TILE_128)
     return fail(GSC_ERR_INVALID, "unknown tile shape %d", o->tile);
+  if (o->table_bytes < 0) return fail(GSC_ERR_INVALID, "table_bytes must be >= 0");
   return GSC_OK;
 }
 
@@ -1096,6 +1097,7 @@ void ensemble_release(gsc_ensemble* p) {
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   if (p->cap) (void)hipStreamDestroy(p->cap);
+
   if (p->mem) (void)hipFree(p->mem);
   if (p->mode_mem) (void)hipFree(p->mode_mem);
   delete p;
@@ -1113,7 +1115,7 @@ int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes
   std::vector<gsc::EnsReal> rs((size_t)count);
   std::vector<int> fact, dir;
   std::vector<int> tile_of((size_t)count, GSC_TILE_256);
-  std::vector<size_t> tab_off((size_t)count), f_off((size_t)count);
+  std::vector<size_t> tab_off((size_t)count), f_off((size_t)count), fbytes((size_t)count, 0);
   std::vector<gsc::FactoredPlan> plans((size_t)count);
   const int tile_cols = gsc::gemm_tile_cols(gsc::precision_splits(o->precision), o->accum);
   // 2-D: the axes on the host, to find the uniform ones (product form)
@@ -1175,7 +1177,7 @@ int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes
       e.ncp = fp.ncp;
       e.row_tiles = fp.row_tiles;
       e.col_tiles = (fp.ncols + tile_cols - 1) / tile_cols;
-      f_off[(size_t)r] = ws;  // S, A, C as laid out after the plan's own mode table
+      // S, A, C as laid out after the plan's own mode table, placed below
       // the single-call rule (grid_plan): the product form on uniform axes
       // when its tables are no larger
       gsc::FactoredPlan q = fp;
@@ -1201,12 +1203,12 @@ int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes
         plans[(size_t)r].off_s = 0;  // offsets within the realisation's block
         plans[(size_t)r].off_a = q.off_a - q.off_s;
         plans[(size_t)r].off_c = q.off_c - q.off_s;
-        ws += align256(q.bytes - q.off_s);
+        fbytes[(size_t)r] = align256(q.bytes - q.off_s);
       } else {
         plans[(size_t)r].off_a -= fp.off_s;
         plans[(size_t)r].off_c -= fp.off_s;
         plans[(size_t)r].off_s = 0;
-        ws += align256(fp.bytes - fp.off_s);
+        fbytes[(size_t)r] = align256(fp.bytes - fp.off_s);
       }
       fact.push_back(r);
     } else {
@@ -1214,63 +1216,107 @@ int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes
     }
     plan->point_modes += (double)e.npts * (double)M;
   }
-  // GEMM groups by tile shape (256-row tiles first) and operand form (tables
-  // of the whole grid, then the product form); in each, longest first (most
-  // modes, then most tiles), then a longest-processing-time split into
-  // kXcdGroups groups of whole realisations (one per XCD), each keeping that
-  // order
+  // table batches: a plan whose factored tables exceed the bound
+  // (opts->table_bytes, default kEnsTableBytes) builds and consumes them in
+  // batches of realisations that reuse one region of the workspace (the
+  // batches run one after the other: tables, then GEMM); realisations go to
+  // the batches by a longest-processing-time split of their GEMM work
+  auto gemm_work = [&](int r) {
+    const gsc::EnsReal& e = rs[(size_t)r];
+    return (double)(e.row_tiles * e.col_tiles) * (double)e.mp;
+  };
+  size_t total_fb = 0;
+  for (int r : fact) total_fb += fbytes[(size_t)r];
+  const size_t cap = o->table_bytes > 0 ? (size_t)o->table_bytes : gsc::kEnsTableBytes;
+  const int nb = (int)std::max<size_t>(1, (total_fb + cap - 1) / cap);
+  std::vector<std::vector<int>> bfact((size_t)nb);
+  {
+    std::vector<int> lst = fact;
+    std::stable_sort(lst.begin(), lst.end(), [&](int a, int b) { return gemm_work(a) > gemm_work(b); });
+    std::vector<double> load((size_t)nb, 0.0);
+    for (int r : lst) {
+      const size_t bi = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
+      bfact[bi].push_back(r);
+      load[bi] += gemm_work(r);
+    }
+  }
+  size_t region_bytes = 0;
+  for (auto& bl : bfact) {
+    std::sort(bl.begin(), bl.end());
+    size_t off = 0;
+    for (int r : bl) off += fbytes[(size_t)r];
+    region_bytes = std::max(region_bytes, off);
+  }
+  for (int bi = 0; bi < nb; ++bi) {
+    size_t off = ws;
+    for (int r : bfact[(size_t)bi]) {
+      f_off[(size_t)r] = off;
+      off += fbytes[(size_t)r];
+    }
+  }
+  ws += region_bytes;
+  // per batch: GEMM groups by tile shape (256-row tiles first) and operand
+  // form (tables of the whole grid, then the product form); in each, longest
+  // first (most modes, then most tiles), then a longest-processing-time split
+  // into kXcdGroups groups of whole realisations (one per XCD), each keeping
+  // that order; and the table tasks of every factored realisation, groups of
+  // 64 rows of S (na, 3-D or product form), A (arows) and C (ncp)
   constexpr int NG = gsc::kEnsGemmGroups;
   const int tile_ids[2] = {GSC_TILE_256, GSC_TILE_128};
-  std::vector<int> gemm_order[NG];
-  std::vector<int64_t> tile_start[NG], group_start[NG];
-  int ngemm = 0;
-  for (int q = 0; q < NG; ++q) {
-    std::vector<int> lst;
-    for (int r : fact)
-      if (tile_of[(size_t)r] == tile_ids[q / 2] && rs[(size_t)r].prod == q % 2) lst.push_back(r);
-    std::stable_sort(lst.begin(), lst.end(), [&](int a, int b) {
-      const gsc::EnsReal &ea = rs[(size_t)a], &eb = rs[(size_t)b];
-      if (ea.mp != eb.mp) return ea.mp > eb.mp;
-      return ea.row_tiles * ea.col_tiles > eb.row_tiles * eb.col_tiles;
-    });
-    std::vector<std::vector<int>> groups(gsc::kXcdGroups);
-    std::vector<double> load(gsc::kXcdGroups, 0.0);
-    for (int r : lst) {
-      const gsc::EnsReal& e = rs[(size_t)r];
-      const double w = (double)(e.row_tiles * e.col_tiles) * e.mp;
-      const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-      groups[(size_t)g].push_back(r);
-      load[(size_t)g] += w;
+  struct Batch {
+    std::vector<int> gemm_order[NG];
+    std::vector<int64_t> tile_start[NG], group_start[NG];
+    int ngemm = 0;
+    std::vector<int> all_fact;
+    std::vector<int64_t> task_start;
+  };
+  std::vector<Batch> bt((size_t)nb);
+  for (int bi = 0; bi < nb; ++bi) {
+    Batch& B = bt[(size_t)bi];
+    for (int q = 0; q < NG; ++q) {
+      std::vector<int> lst;
+      for (int r : bfact[(size_t)bi])
+        if (tile_of[(size_t)r] == tile_ids[q / 2] && rs[(size_t)r].prod == q % 2) lst.push_back(r);
+      std::stable_sort(lst.begin(), lst.end(), [&](int a, int b) {
+        const gsc::EnsReal &ea = rs[(size_t)a], &eb = rs[(size_t)b];
+        if (ea.mp != eb.mp) return ea.mp > eb.mp;
+        return ea.row_tiles * ea.col_tiles > eb.row_tiles * eb.col_tiles;
+      });
+      std::vector<std::vector<int>> groups(gsc::kXcdGroups);
+      std::vector<double> load(gsc::kXcdGroups, 0.0);
+      for (int r : lst) {
+        const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        groups[(size_t)g].push_back(r);
+        load[(size_t)g] += gemm_work(r);
+      }
+      std::vector<int>& ord = B.gemm_order[q];
+      std::vector<int> glen;
+      for (auto& g : groups) {
+        ord.insert(ord.end(), g.begin(), g.end());
+        glen.push_back((int)g.size());
+      }
+      B.tile_start[q].assign(ord.size() + 1, 0);
+      for (size_t i = 0; i < ord.size(); ++i) {
+        const gsc::EnsReal& e = rs[(size_t)ord[i]];
+        B.tile_start[q][i + 1] = B.tile_start[q][i] + e.row_tiles * e.col_tiles;
+      }
+      B.group_start[q].assign(gsc::kXcdGroups + 1, 0);
+      size_t k = 0;
+      for (int g = 0; g < gsc::kXcdGroups; ++g) {
+        k += (size_t)glen[(size_t)g];
+        B.group_start[q][(size_t)g + 1] = B.tile_start[q][k];
+      }
+      if (!ord.empty()) B.ngemm = q + 1;
     }
-    std::vector<int>& ord = gemm_order[q];
-    std::vector<int> glen;
-    for (auto& g : groups) {
-      ord.insert(ord.end(), g.begin(), g.end());
-      glen.push_back((int)g.size());
+    for (int q = 0; q < NG; ++q)
+      B.all_fact.insert(B.all_fact.end(), B.gemm_order[q].begin(), B.gemm_order[q].end());
+    B.task_start.assign(3 * B.all_fact.size() + 1, 0);
+    for (size_t i = 0; i < B.all_fact.size(); ++i) {
+      const gsc::EnsReal& e = rs[(size_t)B.all_fact[i]];
+      B.task_start[3 * i + 1] = B.task_start[3 * i] + (dim == 3 || e.prod ? (e.na + 63) / 64 : 0);
+      B.task_start[3 * i + 2] = B.task_start[3 * i + 1] + e.arows / 64;
+      B.task_start[3 * i + 3] = B.task_start[3 * i + 2] + e.ncp / 64;
     }
-    tile_start[q].assign(ord.size() + 1, 0);
-    for (size_t i = 0; i < ord.size(); ++i) {
-      const gsc::EnsReal& e = rs[(size_t)ord[i]];
-      tile_start[q][i + 1] = tile_start[q][i] + e.row_tiles * e.col_tiles;
-    }
-    group_start[q].assign(gsc::kXcdGroups + 1, 0);
-    size_t k = 0;
-    for (int g = 0; g < gsc::kXcdGroups; ++g) {
-      k += (size_t)glen[(size_t)g];
-      group_start[q][(size_t)g + 1] = tile_start[q][k];
-    }
-    if (!ord.empty()) ngemm = q + 1;
-  }
-  // table tasks: every factored realisation, groups of 64 rows of S (na,
-  // 3-D only), A (arows) and C (ncp)
-  std::vector<int> all_fact;
-  for (int q = 0; q < NG; ++q) all_fact.insert(all_fact.end(), gemm_order[q].begin(), gemm_order[q].end());
-  std::vector<int64_t> task_start(3 * all_fact.size() + 1, 0);
-  for (size_t i = 0; i < all_fact.size(); ++i) {
-    const gsc::EnsReal& e = rs[(size_t)all_fact[i]];
-    task_start[3 * i + 1] = task_start[3 * i] + (dim == 3 || e.prod ? (e.na + 63) / 64 : 0);
-    task_start[3 * i + 2] = task_start[3 * i + 1] + e.arows / 64;
-    task_start[3 * i + 3] = task_start[3 * i + 2] + e.ncp / 64;
   }
   std::stable_sort(dir.begin(), dir.end(), [&](int a, int b) {
     return rs[(size_t)a].M * rs[(size_t)a].npts > rs[(size_t)b].M * rs[(size_t)b].npts;
@@ -1280,7 +1326,7 @@ int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes
     const gsc::EnsReal& e = rs[(size_t)dir[i]];
     blk_start[i + 1] = blk_start[i] + (e.npts + gsc::kEnsPoints - 1) / gsc::kEnsPoints;
   }
-  plan->n_fact = (int64_t)all_fact.size();
+  plan->n_fact = (int64_t)fact.size();
   plan->n_dir = (int64_t)dir.size();
   // control block: descriptors, lists, prefix sums, counters (each piece
   // 256-byte aligned; offsets collected first, then one upload)
@@ -1293,17 +1339,23 @@ int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes
     return off;
   };
   const size_t c_desc = put(rs.data(), sizeof(gsc::EnsReal) * rs.size());
-  const size_t c_fact = put(all_fact.data(), sizeof(int) * all_fact.size());
-  const size_t c_tk = put(task_start.data(), sizeof(int64_t) * task_start.size());
   const size_t c_dir = put(dir.data(), sizeof(int) * dir.size());
   const size_t c_bs = put(blk_start.data(), sizeof(int64_t) * blk_start.size());
-  size_t c_go[NG], c_ts[NG], c_gs[NG];
-  for (int q = 0; q < NG; ++q) {
-    c_go[q] = put(gemm_order[q].data(), sizeof(int) * gemm_order[q].size());
-    c_ts[q] = put(tile_start[q].data(), sizeof(int64_t) * tile_start[q].size());
-    c_gs[q] = put(group_start[q].data(), sizeof(int64_t) * group_start[q].size());
+  std::vector<size_t> c_fact((size_t)nb), c_tk((size_t)nb), c_go((size_t)nb * NG),
+      c_ts((size_t)nb * NG), c_gs((size_t)nb * NG);
+  for (int bi = 0; bi < nb; ++bi) {
+    const Batch& B = bt[(size_t)bi];
+    c_fact[(size_t)bi] = put(B.all_fact.data(), sizeof(int) * B.all_fact.size());
+    c_tk[(size_t)bi] = put(B.task_start.data(), sizeof(int64_t) * B.task_start.size());
+    for (int q = 0; q < NG; ++q) {
+      const size_t i = (size_t)bi * NG + (size_t)q;
+      c_go[i] = put(B.gemm_order[q].data(), sizeof(int) * B.gemm_order[q].size());
+      c_ts[i] = put(B.tile_start[q].data(), sizeof(int64_t) * B.tile_start[q].size());
+      c_gs[i] = put(B.group_start[q].data(), sizeof(int64_t) * B.group_start[q].size());
+    }
   }
-  const size_t c_cnt = put(nullptr, sizeof(int) * gsc::kEnsCounters);
+  const int ncounters = nb * gsc::kEnsBatchCounters + 1;
+  const size_t c_cnt = put(nullptr, sizeof(int) * (size_t)ncounters);
   const size_t ctrl = hctrl.size();
   plan->mem_bytes = ctrl + ws;
   if (hipMalloc(&plan->mem, plan->mem_bytes) != hipSuccess) {
@@ -1333,43 +1385,52 @@ int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes
 
   int cus = 256;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, plan->device);
-  gsc::EnsLaunch el{};
-  el.d = reinterpret_cast<const gsc::EnsReal*>(base + c_desc);
-  el.count = count;
-  el.fact = reinterpret_cast<const int*>(base + c_fact);
-  el.nfact = (int)all_fact.size();
-  el.task_start = reinterpret_cast<const int64_t*>(base + c_tk);
-  el.n_table_tasks = task_start.back();
-  el.dir = reinterpret_cast<const int*>(base + c_dir);
-  el.ndir = (int)dir.size();
-  el.blk_start = reinterpret_cast<const int64_t*>(base + c_bs);
-  el.counters = reinterpret_cast<int*>(base + c_cnt);
-  el.dir_counter = el.counters + NG * gsc::kXcdGroups;
-  el.dim = dim;
-  el.max_tiles = max_tiles;
-  el.ngemm = ngemm;
-  for (int q = 0; q < NG; ++q) {
-    gsc::EnsGemmGroup& g = el.gemm[q];
-    g.fact = reinterpret_cast<const int*>(base + c_go[q]);
-    g.nfact = (int)gemm_order[q].size();
-    g.tile_start = reinterpret_cast<const int64_t*>(base + c_ts[q]);
-    g.group_start = reinterpret_cast<const int64_t*>(base + c_gs[q]);
-    g.counters = el.counters + q * gsc::kXcdGroups;
-    g.brows = gsc::tile_block_rows(tile_ids[q / 2]);
-    g.kr = dim == 3 || q % 2 == 1;
-    // blocks per CU the GEMM is built for: 256-row tiles one (two with the
-    // FP64 flush, which fits 256 registers), 128-row tiles two (four)
-    const int64_t bpc_auto = (o->accum == GSC_ACCUM_F64 ? 2 : 1) * (q / 2 == 1 ? 2 : 1);
-    const int64_t bpc = o->blocks_per_cu > 0 ? o->blocks_per_cu : bpc_auto;
-    g.gemm_blocks = (int)std::max<int64_t>(1, std::min<int64_t>(tile_start[q].back(), bpc * cus));
+  std::vector<gsc::EnsLaunch> els((size_t)nb);
+  for (int bi = 0; bi < nb; ++bi) {
+    const Batch& B = bt[(size_t)bi];
+    gsc::EnsLaunch& el = els[(size_t)bi];
+    el = gsc::EnsLaunch{};
+    el.d = reinterpret_cast<const gsc::EnsReal*>(base + c_desc);
+    el.count = count;
+    el.fact = reinterpret_cast<const int*>(base + c_fact[(size_t)bi]);
+    el.nfact = (int)B.all_fact.size();
+    el.task_start = reinterpret_cast<const int64_t*>(base + c_tk[(size_t)bi]);
+    el.n_table_tasks = B.task_start.back();
+    el.dir = reinterpret_cast<const int*>(base + c_dir);
+    el.ndir = (int)dir.size();
+    el.blk_start = reinterpret_cast<const int64_t*>(base + c_bs);
+    el.counters = reinterpret_cast<int*>(base + c_cnt);
+    el.ncounters = ncounters;
+    el.dir_counter = el.counters + nb * gsc::kEnsBatchCounters;
+    el.dim = dim;
+    el.max_tiles = max_tiles;
+    el.ngemm = B.ngemm;
+    for (int q = 0; q < NG; ++q) {
+      const size_t i = (size_t)bi * NG + (size_t)q;
+      gsc::EnsGemmGroup& g = el.gemm[q];
+      g.fact = reinterpret_cast<const int*>(base + c_go[i]);
+      g.nfact = (int)B.gemm_order[q].size();
+      g.tile_start = reinterpret_cast<const int64_t*>(base + c_ts[i]);
+      g.group_start = reinterpret_cast<const int64_t*>(base + c_gs[i]);
+      g.counters = el.counters + (bi * NG + q) * gsc::kXcdGroups;
+      g.brows = gsc::tile_block_rows(tile_ids[q / 2]);
+      g.kr = dim == 3 || q % 2 == 1;
+      // blocks per CU the GEMM is built for: 256-row tiles one (two with the
+      // FP64 flush, which fits 256 registers), 128-row tiles two (four)
+      const int64_t bpc_auto = (o->accum == GSC_ACCUM_F64 ? 2 : 1) * (q / 2 == 1 ? 2 : 1);
+      const int64_t bpc = o->blocks_per_cu > 0 ? o->blocks_per_cu : bpc_auto;
+      g.gemm_blocks =
+          (int)std::max<int64_t>(1, std::min<int64_t>(B.tile_start[q].back(), bpc * cus));
+    }
+    el.direct_blocks =
+        (int)std::max<int64_t>(1, std::min<int64_t>(blk_start.back(), 8 * (int64_t)cus));
   }
-  el.direct_blocks = (int)std::max<int64_t>(1, std::min<int64_t>(blk_start.back(), 8 * (int64_t)cus));
 
   GSC_HIP(hipStreamCreateWithFlags(&plan->cap, hipStreamNonBlocking));
   GSC_HIP(hipStreamBeginCapture(plan->cap, hipStreamCaptureModeThreadLocal));
   hipError_t le = sample ? gsc::launch_sample_modes(*sample, plan->cap) : hipSuccess;
   if (le == hipSuccess)
-    le = gsc::launch_ensemble(el, o->precision, o->out_dtype, o->accum, plan->cap);
+    le = gsc::launch_ensemble(els.data(), nb, o->precision, o->out_dtype, o->accum, plan->cap);
   hipGraph_t g = nullptr;
   const hipError_t ce = hipStreamEndCapture(plan->cap, &g);
   plan->graph = g;

@@ -19,8 +19,8 @@ namespace {
 // GEMM block tiles, direct ones blocks of kEnsPts points, to two flat work
 // lists consumed by persistent kernels through atomic counters.
 // ---------------------------------------------------------------------------
-__global__ void k_ens_reset(int* __restrict__ counters) {
-  if (threadIdx.x < kEnsCounters) counters[threadIdx.x] = 0;
+__global__ void k_ens_reset(int* __restrict__ counters, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) counters[i] = 0;
 }
 
 __global__ void __launch_bounds__(256)
@@ -209,8 +209,8 @@ template <typename T, int NS> struct GemmF {
   }
 };
 template <typename T, int NS> struct EnsF {
-  static hipError_t run(const EnsLaunch& e, int accum, bool out_f64, hipStream_t s) {
-    return ens_launch<T, NS>(e, accum, out_f64, s);
+  static hipError_t run(const EnsLaunch& e, int part, int accum, bool out_f64, hipStream_t s) {
+    return ens_launch<T, NS>(e, part, accum, out_f64, s);
   }
 };
 
@@ -249,9 +249,10 @@ hipError_t launch_factored(const FactoredPlan& p, int64_t M, const double* const
   return by_format<GemmF>(p.bf16, p.ns, ga, grid, s);
 }
 
-hipError_t launch_ensemble(const EnsLaunch& e, int precision, int out_dtype, int accum,
-                           hipStream_t s) {
-  k_ens_reset<<<1, 64, 0, s>>>(e.counters);
+hipError_t launch_ensemble(const EnsLaunch* batches, int nbatch, int precision, int out_dtype,
+                           int accum, hipStream_t s) {
+  const EnsLaunch& e = batches[0];
+  k_ens_reset<<<1, 256, 0, s>>>(e.counters, e.ncounters);
   if (e.max_tiles > 0)
     k_ens_prep<<<dim3(static_cast<unsigned>((static_cast<int64_t>(e.max_tiles) * 16 + 255) / 256),
                       static_cast<unsigned>(e.count)),
@@ -259,9 +260,14 @@ hipError_t launch_ensemble(const EnsLaunch& e, int precision, int out_dtype, int
   const bool f32 = out_dtype == 1;
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return err;
-  if (e.nfact > 0) {
-    err = by_format<EnsF>(precision_is_bf16(precision), precision_splits(precision), e, accum,
-                          !f32, s);
+  // the batches one after the other on this stream: each one's tables, then
+  // its GEMM, which finishes before the next batch rewrites the shared region
+  const bool bf = precision_is_bf16(precision);
+  const int ns = precision_splits(precision);
+  for (int b = 0; b < nbatch; ++b) {
+    if (batches[b].nfact == 0) continue;
+    err = by_format<EnsF>(bf, ns, batches[b], 0, accum, !f32, s);
+    if (err == hipSuccess) err = by_format<EnsF>(bf, ns, batches[b], 1, accum, !f32, s);
     if (err != hipSuccess) return err;
   }
   if (e.ndir > 0) {

@@ -6,5 +6,5 @@
 namespace gsc {
 template hipError_t tables_launch<__bf16, 2>(const TableArgs&, dim3, hipStream_t);
 template hipError_t factored_gemm_launch<__bf16, 2>(const GemmArgs&, dim3, hipStream_t);
-template hipError_t ens_launch<__bf16, 2>(const EnsLaunch&, int, bool, hipStream_t);
+template hipError_t ens_launch<__bf16, 2>(const EnsLaunch&, int, int, bool, hipStream_t);
 }  // namespace gsc

@@ -6,5 +6,5 @@
 namespace gsc {
 template hipError_t tables_launch<_Float16, 1>(const TableArgs&, dim3, hipStream_t);
 template hipError_t factored_gemm_launch<_Float16, 1>(const GemmArgs&, dim3, hipStream_t);
-template hipError_t ens_launch<_Float16, 1>(const EnsLaunch&, int, bool, hipStream_t);
+template hipError_t ens_launch<_Float16, 1>(const EnsLaunch&, int, int, bool, hipStream_t);
 }  // namespace gsc

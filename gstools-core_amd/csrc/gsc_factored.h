@@ -163,16 +163,25 @@ struct EnsLaunch {
   int ndir;
   const int64_t* blk_start;
   int* dir_counter;
-  int* counters;               // all kEnsCounters counters (reset at the start of a run)
+  int* counters;               // all counters of the plan (reset at the start of a run)
+  int ncounters;
   int dim;
   int max_tiles;
   int direct_blocks;
 };
 constexpr int kEnsPoints = 2048;  // points per direct work item
-// counters: kXcdGroups per GEMM launch, then the direct-kernel counter
-constexpr int kEnsCounters = kEnsGemmGroups * kXcdGroups + 1;
-hipError_t launch_ensemble(const EnsLaunch& e, int precision, int out_dtype, int accum,
-                           hipStream_t stream);
+// counters: kXcdGroups per GEMM launch of each table batch, then the
+// direct-kernel counter
+constexpr int kEnsBatchCounters = kEnsGemmGroups * kXcdGroups;
+// Default bound on an ensemble's factored-table region (gsc_opts.table_bytes
+// = 0): plans whose tables exceed the bound build and consume them in
+// batches of realisations that reuse one region.
+constexpr size_t kEnsTableBytes = size_t(4) << 30;
+// batches[0] holds the plan-wide fields (descriptors, mode preparation,
+// direct realisations, all counters); every batch its table tasks and GEMM
+// groups; the batches run in order on `stream`.
+hipError_t launch_ensemble(const EnsLaunch* batches, int nbatch, int precision, int out_dtype,
+                           int accum, hipStream_t stream);
 
 
 // Coordinates of the product form's tables (FactoredPlan.prod): S rows are
@@ -248,6 +257,6 @@ hipError_t tables_launch(const TableArgs& a, dim3 grid, hipStream_t s);
 template <typename T, int NS>
 hipError_t factored_gemm_launch(const GemmArgs& a, dim3 grid, hipStream_t s);
 template <typename T, int NS>
-hipError_t ens_launch(const EnsLaunch& e, int accum, bool out_f64, hipStream_t s);
+hipError_t ens_launch(const EnsLaunch& e, int part, int accum, bool out_f64, hipStream_t s);
 
 }  // namespace gsc

@@ -863,10 +863,13 @@ hipError_t factored_gemm_launch(const GemmArgs& a, dim3 grid, hipStream_t s) {
 }
 
 template <typename T, int NS>
-hipError_t ens_launch(const EnsLaunch& e, int accum, bool out_f64, hipStream_t s) {
+hipError_t ens_launch(const EnsLaunch& e, int part, int accum, bool out_f64, hipStream_t s) {
   using namespace gemm;
-  k_ens_tables<T, NS><<<static_cast<unsigned>(e.n_table_tasks), 256, 0, s>>>(
-      e.d, e.fact, e.task_start, e.nfact);
+  if (part == 0) {
+    k_ens_tables<T, NS><<<static_cast<unsigned>(e.n_table_tasks), 256, 0, s>>>(
+        e.d, e.fact, e.task_start, e.nfact);
+    return hipGetLastError();
+  }
   for (int q = 0; q < e.ngemm; ++q) {
     const EnsGemmGroup& g = e.gemm[q];
     if (g.nfact == 0) continue;

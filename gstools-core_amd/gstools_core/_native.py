@@ -48,6 +48,7 @@ class GscOpts(ctypes.Structure):
         ("blocks_per_cu", ctypes.c_int32),
         ("scale", ctypes.c_double),
         ("tile", ctypes.c_int32),
+        ("table_bytes", ctypes.c_int64),
     ]
 
 

@@ -12,6 +12,11 @@ contraction, and the GEMM tile shape of each factored realisation, come from
 the library rule, explicit ``paths`` / ``tiles`` lists, or the autotuner
 (``tune=True``, see gstools_core.tuner), which also picks the plan's split
 format and accumulation when they are left to it.
+
+``table_bytes`` bounds the factored tables the plan holds at once: above it
+the realisations are split into batches that build and use their tables in
+turn in one region (0 = the library's 4 GiB), trading a little time per
+batch for workspace.
 """
 from __future__ import annotations
 
@@ -38,7 +43,7 @@ class Ensemble:
                  paths: Optional[Sequence[str]] = None,
                  scales: Optional[Sequence[float]] = None, tune: bool = False, tuner=None,
                  device: int = -1, blocks_per_cu: int = 0, tile: str = "auto",
-                 tiles: Optional[Sequence[str]] = None):
+                 tiles: Optional[Sequence[str]] = None, table_bytes: int = 0):
         if len(realisations) == 0:
             raise ValueError("an ensemble needs at least one realisation")
         covs, z1s, z2s, grids, ms = [], [], [], [], []
@@ -54,7 +59,7 @@ class Ensemble:
             grids.append(axes)
             ms.append(cov.shape[1])
         args = self._setup(grids, ms, precision, out_dtype, accum, paths, scales, tune, tuner,
-                           device, blocks_per_cu, tile, tiles)
+                           device, blocks_per_cu, tile, tiles, table_bytes)
         cat = lambda xs: np.concatenate(xs) if xs else np.zeros(0)  # noqa: E731
         self._cov = _native.DeviceBuffer.from_array(cat(covs), device)
         self._z1 = _native.DeviceBuffer.from_array(cat(z1s), device)
@@ -94,7 +99,7 @@ class Ensemble:
             kw.pop("out_dtype", np.float32), kw.pop("accum", None), kw.pop("paths", None),
             kw.pop("scales", None), kw.pop("tune", False), kw.pop("tuner", None),
             kw.pop("device", -1), kw.pop("blocks_per_cu", 0), kw.pop("tile", "auto"),
-            kw.pop("tiles", None))
+            kw.pop("tiles", None), kw.pop("table_bytes", 0))
         if kw:
             raise TypeError(f"unexpected keyword arguments {sorted(kw)}")
         c_len = (ctypes.c_double * n)(*[float(v) for v in len_scales])
@@ -155,7 +160,7 @@ class Ensemble:
                                                         ctypes.c_void_p(stream or None)))
 
     def _setup(self, grids, ms, precision, out_dtype, accum, paths, scales, tune, tuner, device,
-               blocks_per_cu, tile="auto", tiles=None):
+               blocks_per_cu, tile="auto", tiles=None, table_bytes=0):
         """Geometry, output buffer and options shared by both constructors."""
         dims = {len(g) for g in grids}
         if len(dims) != 1:
@@ -204,6 +209,9 @@ class Ensemble:
             precision=_native.PRECISIONS[precision], accum=_native.ACCUM[accum], device=device,
             tile=_native.TILES[str(tile)])
         opts.blocks_per_cu = int(blocks_per_cu)
+        if int(table_bytes) < 0:
+            raise ValueError("table_bytes must be >= 0")
+        opts.table_bytes = int(table_bytes)
         c_shapes = (ctypes.c_int64 * (self.count * self.dim))(*[v for s in shapes for v in s])
         c_ms = (ctypes.c_int64 * self.count)(*ms)
         return c_shapes, c_ms, path_codes, tile_codes, scale_arr, opts

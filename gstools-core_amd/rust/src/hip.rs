@@ -25,6 +25,7 @@ pub struct GscOpts {
     pub blocks_per_cu: i32,
     pub scale: f64,
     pub tile: i32,
+    pub table_bytes: i64,
 }
 
 extern "C" {

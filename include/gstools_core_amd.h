@@ -116,6 +116,9 @@ typedef struct gsc_opts {
                               (2 with GSC_ACCUM_F64, else 1) */
   double scale;            /* multiplies every output value (1.0 = reference) */
   int32_t tile;            /* gsc_tile, factored path only */
+  int64_t table_bytes;     /* ensembles: bound on the factored tables held at once; plans
+                              whose tables exceed it build and use them in batches of
+                              realisations that reuse one region (0 = 4 GiB) */
 } gsc_opts;
 
 /* Fills *opts with the defaults: f64 output, auto path/precision, device -1, scale 1. */

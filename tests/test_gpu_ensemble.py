@@ -145,6 +145,41 @@ def test_many_realisations_match_single_grid_calls(dim, jitter):
     ens.close()
 
 
+@pytest.mark.parametrize("dim,jitter", [(2, False), (2, True), (3, False)])
+def test_table_batches_equal_one_region(dim, jitter):
+    # a bound on the tables held at once (table_bytes) splits the factored
+    # realisations into batches that build and use their tables in turn in
+    # one region: a smaller workspace, the same values bit for bit, on every
+    # replay, with direct realisations in the same plan
+    rng = np.random.default_rng(70 + dim)
+    count = 40
+    if dim == 2:
+        shapes = [tuple(int(v) for v in rng.integers(17, 300, size=2)) for _ in range(count)]
+    else:
+        shapes = [tuple(int(v) for v in rng.integers(17, 48, size=3)) for _ in range(count)]
+    ms = [int(v) for v in rng.integers(16, 400, size=count)]
+    paths = ["direct" if i % 7 == 3 else "factored" for i in range(count)]
+    reals = _reals(dim, shapes, ms, 80 + dim, jitter)
+    kw = dict(paths=paths, out_dtype=np.float64, precision="fp16x3")
+    whole = Ensemble(reals, **kw)
+    whole.run()
+    ref = [x.copy() for x in whole.results()]
+    ws_whole = whole.info["workspace_bytes"]
+    whole.close()
+    bounded = Ensemble(reals, table_bytes=ws_whole // 5, **kw)
+    assert bounded.info["workspace_bytes"] < ws_whole // 2
+    for rep in range(3):
+        bounded.run()
+        for i, (x, y) in enumerate(zip(bounded.results(), ref)):
+            np.testing.assert_array_equal(x, y, err_msg=f"replay {rep} realisation {i}")
+        grid.summate_grid(reals[0][0], reals[0][1], reals[0][2], reals[0][3])
+    print(f"dim {dim}: workspace {ws_whole / 1e6:.1f} MB -> "
+          f"{bounded.info['workspace_bytes'] / 1e6:.1f} MB")
+    bounded.close()
+    with pytest.raises(ValueError):
+        Ensemble(reals, table_bytes=-1, **kw)
+
+
 def test_fp64_flush_plan_two_blocks_per_cu():
     # the FP64-flush GEMM runs two persistent blocks per CU by default; every
     # realisation equals its single-grid evaluation with the same accumulation
