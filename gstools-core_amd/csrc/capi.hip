@@ -1219,8 +1219,11 @@ int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes
   // table batches: a plan whose factored tables exceed the bound
   // (opts->table_bytes, default kEnsTableBytes) builds and consumes them in
   // batches of realisations that reuse one region of the workspace (the
-  // batches run one after the other: tables, then GEMM); realisations go to
-  // the batches by a longest-processing-time split of their GEMM work
+  // batches run one after the other: tables, then GEMM).  The batch count
+  // starts at total / bound and grows until a longest-first split of the
+  // table sizes keeps every batch within the bound (a realisation larger than
+  // the bound forms a batch of its own); the batches come out of about equal
+  // size, so the region is about total / count.
   auto gemm_work = [&](int r) {
     const gsc::EnsReal& e = rs[(size_t)r];
     return (double)(e.row_tiles * e.col_tiles) * (double)e.mp;
@@ -1228,18 +1231,27 @@ int ensemble_build(gsc_ensemble* plan, int count, int dim, const int64_t* shapes
   size_t total_fb = 0;
   for (int r : fact) total_fb += fbytes[(size_t)r];
   const size_t cap = o->table_bytes > 0 ? (size_t)o->table_bytes : gsc::kEnsTableBytes;
-  const int nb = (int)std::max<size_t>(1, (total_fb + cap - 1) / cap);
-  std::vector<std::vector<int>> bfact((size_t)nb);
-  {
+  std::vector<std::vector<int>> bfact;
+  if (total_fb <= cap) {
+    bfact.push_back(fact);
+  } else {
     std::vector<int> lst = fact;
-    std::stable_sort(lst.begin(), lst.end(), [&](int a, int b) { return gemm_work(a) > gemm_work(b); });
-    std::vector<double> load((size_t)nb, 0.0);
-    for (int r : lst) {
-      const size_t bi = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
-      bfact[bi].push_back(r);
-      load[bi] += gemm_work(r);
+    std::stable_sort(lst.begin(), lst.end(),
+                     [&](int a, int b) { return fbytes[(size_t)a] > fbytes[(size_t)b]; });
+    for (size_t n = std::min((total_fb + cap - 1) / cap, lst.size());; ++n) {
+      bfact.assign(n, {});
+      std::vector<size_t> used(n, 0);
+      for (int r : lst) {
+        const size_t bi = (size_t)(std::min_element(used.begin(), used.end()) - used.begin());
+        used[bi] += fbytes[(size_t)r];
+        bfact[bi].push_back(r);
+      }
+      bool fits = true;
+      for (size_t bi = 0; bi < n; ++bi) fits = fits && (used[bi] <= cap || bfact[bi].size() == 1);
+      if (fits || n >= lst.size()) break;
     }
   }
+  const int nb = (int)bfact.size();
   size_t region_bytes = 0;
   for (auto& bl : bfact) {
     std::sort(bl.begin(), bl.end());
