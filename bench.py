@@ -40,7 +40,7 @@ GRID_ERRORS = {
 
 
 def label_errors(res: dict, leg: str) -> None:
-    """Attaches the measured error and the 1e-5 sigma verdict to each timed format."""
+    """Attaches the measured error and whether it meets 1e-5 sigma to each timed format."""
     errs = GRID_ERRORS[leg]
     for key, r in res.items():
         if not isinstance(r, dict):
