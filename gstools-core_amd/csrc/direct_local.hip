@@ -16,17 +16,19 @@
 //   v_mfma_f32_16x16x32_bf16, with xi and kap split into three bf16 parts
 //   each (24 significant bits) and K holding, per dimension d, the six
 //   products of parts whose orders sum to at most 2 (h h, h m, m h, h l,
-//   m m, l h) in slots 8 d .. 8 d + 5; r0 enters as the FP32 accumulator C.
+//   m m, l h) in slots 8 d .. 8 d + 5; r0 enters through three more K slots
+//   (from the per-cell table) or as the FP32 accumulator C (evaluated per
+//   wave pair when the table would be too large).
 // The dropped products are below 2^-22 (fp16) or 2^-24 (bf16) of the phase.
 // The mode-only parts of the B fragments are read from tables shared by all
-// waves; a wave keeps (r0, amplitude) of its cell for one chunk of modes in
-// LDS.  The VALU is left with one v_sin_f32 (argument in revolutions) and one
-// FMA per point-mode (D for the vector form); on gfx950 the matrix
-// instructions hardly overlap that VALU stream, which is why the scalar form
-// uses the instruction with the fewest matrix cycles per phase.  FP32 partial
-// sums over kModeChunk modes are added into a second FP32 set (one add per
-// chunk and point-mode column); the mode columns of a point are summed in
-// FP64 (the scalar form first adds groups of four columns in FP32).
+// waves; r0 comes from per-cell rows (or per-wave LDS tables) and the
+// amplitudes from LDS.  The VALU is left with one v_sin_f32 (argument in
+// revolutions) and one FMA per point-mode (D for the vector form); on gfx950
+// the matrix instructions hardly overlap that VALU stream, which is why the
+// scalar form uses the instruction with the fewest matrix cycles per phase.
+// FP32 partial sums over a chunk of modes go into FP32 (scalar) or FP64
+// (vector) totals; the mode columns of a point are summed in FP64 (the scalar
+// form first adds groups of four columns in FP32).
 //
 // Modes whose local phase could exceed kRoughRev revolutions in some cell
 // (sum_k |kap_k| h_k / 2 > kRoughRev, e.g. the far tail of a heavy-tailed
@@ -1074,10 +1076,11 @@ __device__ __forceinline__ void sum_modes_vec(const ModeRec* __restrict__ modes,
 // Vector-valued form (incompressible fields): component c of point x is
 // sum_m amp_c(m) sin(2 pi t_m(x)), one sine per point-mode shared by the D
 // components.  The phases come from 16 x 16 x 32 bf16 products (three-part
-// splits, 24 significant bits), for 32 slots per wave (half a record group,
-// two MFMA groups), which leaves registers for D accumulator sets.  Wave w
-// works on slots 32 (w & 1) .. + 31 of record group w >> 1.  FP32 partial
-// sums over kVecChunk modes are added into a second FP32 set.
+// splits, 24 significant bits).  Wave w takes record group w (64 points,
+// four MFMA row groups on every 16-mode B fragment), so each tile's B read
+// and amplitude read serve 1024 point-modes; its FP32 accumulators (D x 16
+// per lane) go into FP64 per point every kVecChunk modes through the wave's
+// LDS rows.
 //
 // K slots 24 .. 26 (lane group kg = 3) carry r0: A holds 1 there and B the
 // three bf16 parts of the cell's r0, read from the per-cell table that
@@ -1085,10 +1088,12 @@ __device__ __forceinline__ void sum_modes_vec(const ModeRec* __restrict__ modes,
 // phase.  The amplitudes of kVecAmpChunk modes at a time sit in the block's
 // LDS.  Without a per-cell table (too large) k_direct_local_vec_eval below
 // evaluates r0 per wave pair and chunk instead.
-constexpr int kVecChunk = 512;      // modes per FP32 accumulation interval
+constexpr int kVecChunk = 1024;     // modes per FP32 accumulation interval
+constexpr int kVecEvalChunk = 512;  // modes per (r0, amplitudes) table of a wave pair (eval form)
 constexpr int kVecAmpChunk = 1024;  // modes per amplitude table of a block (16 KB)
 constexpr int kVecRedLd = 33;       // LDS row of the cross-lane sums (doubles)
-static_assert(kVecChunk <= kModeChunk, "the mode tables are zero-padded by kModeChunk entries");
+static_assert(kVecChunk <= kModeChunk && kVecEvalChunk <= kModeChunk,
+              "the mode tables are zero-padded by kModeChunk entries");
 static_assert(kVecAmpChunk <= kModeChunk, "the amplitude table reads stay in the padding");
 static_assert(kVecAmpChunk % kVecChunk == 0, "accumulation intervals split the table");
 // B fragment reads run two tiles (32 modes) past the last tile issued, and the
@@ -1107,35 +1112,35 @@ __device__ __forceinline__ bf8 a_fragment_r0(const float* rec, int kg, float sc)
   return a_fragment<D>(rec, kg, sc);
 }
 
+constexpr int kVecSumLd = 65;  // LDS row (floats) of the vector kernel's cross-lane sums
 template <int D, typename OutT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4)))
 k_direct_local_vec(const BinParams* __restrict__ prm, const int* __restrict__ wave_cell,
-                   const float4* __restrict__ recs, const uint4* __restrict__ btpl,
-                   const float4* __restrict__ ampt, const uint2* __restrict__ r0v,
-                   int64_t r0ld, const ModeRec* __restrict__ rough,
-                   const double* __restrict__ pos, int64_t ld, OutT* __restrict__ out,
-                   int64_t out_ld, double scale) {
+                     const float4* __restrict__ recs, const uint4* __restrict__ btpl,
+                     const float4* __restrict__ ampt, const uint2* __restrict__ r0v,
+                     int64_t r0ld, const ModeRec* __restrict__ rough,
+                     const double* __restrict__ pos, int64_t ld, OutT* __restrict__ out,
+                     int64_t out_ld, double scale) {
   __shared__ __attribute__((aligned(16))) f4 s_amp[kVecAmpChunk];
-  __shared__ double s_red[4][16 * kVecRedLd];
+  __shared__ float s_red[4][16 * kVecSumLd];
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int w = blockIdx.x * 4 + wv;
   const BinParams P = *prm;
-  if (blockIdx.x * 4 >= 2 * P.nwaves) return;  // the whole block is past the list
+  if (blockIdx.x * 4 >= P.nwaves) return;  // the whole block is past the list
   // a wave past the list (in the last block) runs the loops on the last
   // record group, for the block's barriers, and stores nothing
-  const bool live = w < 2 * P.nwaves;
+  const bool live = w < P.nwaves;
   const int n_smooth = P.n_smooth, n_rough = P.n_rough;
-  const int rg = live ? (w >> 1) : P.nwaves - 1;
+  const int rg = live ? w : P.nwaves - 1;
   const int kg = lane >> 4, j = lane & 15;
-  const float* rf =
-      reinterpret_cast<const float*>(recs + static_cast<int64_t>(rg) * kWavePts + 32 * (w & 1));
-  bf8 a[2];
+  const float* rf = reinterpret_cast<const float*>(recs + static_cast<int64_t>(rg) * kWavePts);
+  bf8 a[4];
   // (a select chain, not P.h[kg]: a lane-dependent index would put P in scratch)
   const double hk = kg == 0 ? P.h[0] : (kg == 1 ? P.h[1] : P.h[2]);
   const float sck = kg < D ? static_cast<float>(ldexp(1.0, -half_cell_exp(hk))) : 0.0f;
 #pragma unroll
-  for (int g = 0; g < 2; ++g) a[g] = a_fragment_r0<D>(rf + (16 * g + j) * 4, kg, sck);
+  for (int g = 0; g < 4; ++g) a[g] = a_fragment_r0<D>(rf + (16 * g + j) * 4, kg, sck);
   // B fragment source of this lane, mode j of a tile: the mode template
   // (lane groups 0..2) or the cell's r0 parts (group 3; the 16-byte read
   // also takes the next entry, which meets zero slots of A)
@@ -1144,16 +1149,38 @@ k_direct_local_vec(const BinParams* __restrict__ prm, const int* __restrict__ wa
              : reinterpret_cast<const char*>(r0v + static_cast<int64_t>(wave_cell[rg]) * r0ld + j);
   const int64_t bstep = kg < 3 ? 16 * 4 * sizeof(uint4) : 16 * sizeof(uint2);  // per tile
   const f4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
-  float acc[D][2][4], tot[D][2][4];
+  float acc[D][4][4];
+  double comp[D];
 #pragma unroll
-  for (int c = 0; c < D; ++c)
+  for (int c = 0; c < D; ++c) {
+    comp[c] = 0.0;
 #pragma unroll
-    for (int g = 0; g < 2; ++g)
+    for (int g = 0; g < 4; ++g)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        acc[c][g][v] = 0.0f;
-        tot[c][g][v] = 0.0f;
-      }
+      for (int v = 0; v < 4; ++v) acc[c][g][v] = 0.0f;
+  }
+  // acc[c][g][v] is slot 16 g + 4 (lane / 16) + v of the record group, mode
+  // column j; lane l sums slot l's 16 columns in FP64
+  float* red = s_red[wv];
+  auto flush = [&]() {
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      __asm__ volatile("" ::: "memory");
+#pragma unroll
+      for (int g = 0; g < 4; ++g)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          red[j * kVecSumLd + 16 * g + 4 * kg + v] = acc[c][g][v];
+          acc[c][g][v] = 0.0f;
+        }
+      __asm__ volatile("" ::: "memory");
+      double sum = 0.0;
+#pragma unroll
+      for (int jj = 0; jj < 16; ++jj) sum += static_cast<double>(red[jj * kVecSumLd + lane]);
+      comp[c] += sum;
+    }
+    __asm__ volatile("" ::: "memory");
+  };
   for (int a0 = 0; a0 < n_smooth; a0 += kVecAmpChunk) {
     __syncthreads();  // the previous amplitude table is read
     // zero amplitudes past n_smooth (the padding of ampt)
@@ -1170,21 +1197,21 @@ k_direct_local_vec(const BinParams* __restrict__ prm, const int* __restrict__ wa
       uint4 ba = *reinterpret_cast<const uint4*>(bp);
       uint4 bb = *reinterpret_cast<const uint4*>(bp + bstep);
       bp += 2 * bstep;
-      f4 ta[2], tb[2];
+      f4 ta[4], tb[4];
       f4 ama, amb;
-      auto issue = [&](f4 (&t)[2], f4& am, uint4& bq) {
+      auto issue = [&](f4 (&t)[4], f4& am, uint4& bq) {
         am = *mp;
         mp += 16;
 #pragma unroll
-        for (int g = 0; g < 2; ++g)
+        for (int g = 0; g < 4; ++g)
           t[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[g], __builtin_bit_cast(bf8, bq), zero,
                                                          0, 0, 0);
         bq = *reinterpret_cast<const uint4*>(bp);
         bp += bstep;
       };
-      auto consume = [&](const f4 (&t)[2], const f4& am) {
+      auto consume = [&](const f4 (&t)[4], const f4& am) {
 #pragma unroll
-        for (int g = 0; g < 2; ++g)
+        for (int g = 0; g < 4; ++g)
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
             const float sv = __builtin_amdgcn_sinf(t[g][v]);
@@ -1207,37 +1234,10 @@ k_direct_local_vec(const BinParams* __restrict__ prm, const int* __restrict__ wa
       } else {
         consume(ta, ama);
       }
-#pragma unroll
-      for (int c = 0; c < D; ++c)
-#pragma unroll
-        for (int g = 0; g < 2; ++g)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            tot[c][g][v] += acc[c][g][v];
-            acc[c][g][v] = 0.0f;
-          }
+      flush();
     }
   }
-  // cross-lane sums, one component at a time: tot[c][g][v] is slot
-  // 16 g + 4 (lane / 16) + v of the half, mode column j
-  const int p = live && lane < 32 ? __builtin_bit_cast(int, rf[lane * 4 + 3]) : -1;
-  double comp[D];
-  double* red = s_red[wv];
-#pragma unroll
-  for (int c = 0; c < D; ++c) {
-    __asm__ volatile("" ::: "memory");
-#pragma unroll
-    for (int g = 0; g < 2; ++g)
-#pragma unroll
-      for (int v = 0; v < 4; ++v)
-        red[j * kVecRedLd + 16 * g + 4 * kg + v] = static_cast<double>(tot[c][g][v]);
-    double sum = 0.0;
-    if (lane < 32)
-#pragma unroll
-      for (int jj = 0; jj < 16; ++jj) sum += red[jj * kVecRedLd + lane];
-    comp[c] = sum;
-    __asm__ volatile("" ::: "memory");
-  }
+  const int p = live ? __builtin_bit_cast(int, rf[lane * 4 + 3]) : -1;
   if (p < 0) return;
   if (n_rough > 0) {
     double x[D];
@@ -1252,9 +1252,10 @@ k_direct_local_vec(const BinParams* __restrict__ prm, const int* __restrict__ wa
   for (int c = 0; c < D; ++c) store(out + c * out_ld + p, comp[c] * scale);
 }
 
-// The form above with r0 evaluated in the kernel: per wave pair (the two
-// halves of one record group, one cell) an LDS table of (r0, amplitudes) for
-// kVecChunk modes, r0 entering the product as the accumulator C.
+// The vector form with r0 evaluated in the kernel: per wave pair (the two
+// halves of one record group, 32 slots per wave, one cell) an LDS table of
+// (r0, amplitudes) for kVecEvalChunk modes, r0 entering the product as the
+// accumulator C.
 template <int D, typename OutT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4)))
 k_direct_local_vec_eval(const BinParams* __restrict__ prm, const double* __restrict__ centre,
@@ -1266,7 +1267,7 @@ k_direct_local_vec_eval(const BinParams* __restrict__ prm, const double* __restr
   // per wave pair (the two halves of one record group, one cell): the
   // (r0, amplitude) table of one chunk of modes, each wave filling half of
   // it, and afterwards the pair's cross-lane sums
-  constexpr int kPairLds = kVecChunk * 32 > 2 * 16 * kVecRedLd * 8 ? kVecChunk * 32
+  constexpr int kPairLds = kVecEvalChunk * 32 > 2 * 16 * kVecRedLd * 8 ? kVecEvalChunk * 32
                                                                     : 2 * 16 * kVecRedLd * 8;
   __shared__ __attribute__((aligned(16))) char lds[2][kPairLds];
   const int lane = threadIdx.x & 63;
@@ -1297,7 +1298,7 @@ k_direct_local_vec_eval(const BinParams* __restrict__ prm, const double* __restr
 #pragma unroll
   for (int g = 0; g < 2; ++g) a[g] = a_fragment<D>(rf + (16 * g + j) * 4, kg, sck);
   f4* rq = reinterpret_cast<f4*>(lds[wv >> 1]);
-  f4* ram = rq + kVecChunk;
+  f4* ram = rq + kVecEvalChunk;
   float acc[D][2][4], tot[D][2][4];
 #pragma unroll
   for (int c = 0; c < D; ++c)
@@ -1308,9 +1309,9 @@ k_direct_local_vec_eval(const BinParams* __restrict__ prm, const double* __restr
         acc[c][g][v] = 0.0f;
         tot[c][g][v] = 0.0f;
       }
-  for (int c0 = 0; c0 < n_smooth; c0 += kVecChunk) {
+  for (int c0 = 0; c0 < n_smooth; c0 += kVecEvalChunk) {
 #pragma unroll 2
-    for (int r = par; r < kVecChunk / 64; r += 2) {  // this wave's half of the table
+    for (int r = par; r < kVecEvalChunk / 64; r += 2) {  // this wave's half of the table
       const int i = c0 + 64 * r + lane;  // past n_smooth: zero padding
       const SmoothRec sr = sm[i];
       double t = __builtin_fma(sr.kap[0], X[0], sr.th);
@@ -1322,7 +1323,7 @@ k_direct_local_vec_eval(const BinParams* __restrict__ prm, const double* __restr
       ram[64 * r + lane] = f4{am.x, am.y, am.z, 0.0f};
     }
     __syncthreads();  // both halves of the table written
-    const int ntile = min(kVecChunk / 16, (n_smooth - c0 + 15) / 16);
+    const int ntile = min(kVecEvalChunk / 16, (n_smooth - c0 + 15) / 16);
     const uint4* bp = btpl + 4 * c0;
     const unsigned bo = 4u * j + kg;
     const f4* rp = rq + j;
@@ -1471,11 +1472,11 @@ hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table
                                                    offsets, recs);                               \
     if (split) (void)hipEventRecord(split, s);                                                   \
     if (vec && r0rows != nullptr && f32)                                                         \
-      k_direct_local_vec<D, float><<<vb, 256, 0, s>>>(                                           \
+      k_direct_local_vec<D, float><<<mb, 256, 0, s>>>(                                         \
           prm, wave_cell, recs, btpl, ampt, reinterpret_cast<const uint2*>(r0rows), r0rld, rough, \
           pos, pos_ld, static_cast<float*>(out), out_ld, scale);                                 \
     else if (vec && r0rows != nullptr)                                                           \
-      k_direct_local_vec<D, double><<<vb, 256, 0, s>>>(                                          \
+      k_direct_local_vec<D, double><<<mb, 256, 0, s>>>(                                        \
           prm, wave_cell, recs, btpl, ampt, reinterpret_cast<const uint2*>(r0rows), r0rld, rough, \
           pos, pos_ld, static_cast<double*>(out), out_ld, scale);                                \
     else if (vec && f32)                                                                         \
