@@ -121,7 +121,7 @@ int grid_tile(const gsc_opts* o) { return o->tile == GSC_TILE_AUTO ? GSC_TILE_25
 int validate_opts(const gsc_opts* o) {
   if (o->out_dtype != GSC_F64 && o->out_dtype != GSC_F32)
     return fail(GSC_ERR_INVALID, "out_dtype must be GSC_F64 or GSC_F32");
-  if (o->direct_variant < GSC_DIRECT_MFMA || o->direct_variant > GSC_DIRECT_LOCAL)
+  if (o->direct_variant < GSC_DIRECT_MFMA || o->direct_variant > GSC_DIRECT_AUTO)
     return fail(GSC_ERR_INVALID, "unknown direct_variant %d", o->direct_variant);
   if (o->blocks_per_cu < 0 || o->blocks_per_cu > 64)
     return fail(GSC_ERR_INVALID, "blocks_per_cu must be in 0..64");
@@ -164,14 +164,23 @@ size_t align256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
 //   (out_sd, out_sp), otherwise N values with stride out_s = out_sp.
 // The local-phase (binned) direct kernel serves all three summators in 2-D
 // and 3-D; other dimensions use the FP64-phase VALU kernel for that variant.
-bool use_binned(int kind, int dim, int64_t N, const gsc_opts* o) {
+// GSC_DIRECT_AUTO: the local variant from kAutoLocalModes modes up; below,
+// its cell sort (tens of microseconds plus ~33 ns per point) costs more than
+// the ~0.13 ns per point-mode it saves over the FP64-phase VALU kernel.
+constexpr int64_t kAutoLocalModes = 256;
+int resolve_variant(const gsc_opts* o, int64_t M) {
+  if (o->direct_variant != GSC_DIRECT_AUTO) return o->direct_variant;
+  return M >= kAutoLocalModes ? GSC_DIRECT_LOCAL : GSC_DIRECT_VALU;
+}
+
+bool use_binned(int kind, int dim, int64_t M, int64_t N, const gsc_opts* o) {
   (void)kind;
-  return o->direct_variant == GSC_DIRECT_LOCAL && gsc::binned_supported(dim, N);
+  return resolve_variant(o, M) == GSC_DIRECT_LOCAL && gsc::binned_supported(dim, N);
 }
 
 size_t direct_work_bytes(int kind, int dim, int64_t M, int64_t N, const gsc_opts* o) {
   const size_t table = align256(gsc::mode_table_bytes(dim, M));
-  return use_binned(kind, dim, N, o) ? table + gsc::binned_work_bytes(dim, M, N) : table;
+  return use_binned(kind, dim, M, N, o) ? table + gsc::binned_work_bytes(dim, M, N) : table;
 }
 
 // Launches the direct kernel of `kind` on device buffers; `work` holds the
@@ -181,7 +190,7 @@ size_t direct_work_bytes(int kind, int dim, int64_t M, int64_t N, const gsc_opts
 hipError_t launch_direct(int kind, int dim, int64_t M, int64_t N, void* work, const double* pos,
                          int64_t pos_ld, void* out, int64_t out_ld, const gsc_opts* o,
                          hipStream_t s, hipEvent_t split = nullptr) {
-  if (use_binned(kind, dim, N, o)) {
+  if (use_binned(kind, dim, M, N, o)) {
     void* bin = static_cast<char*>(work) + align256(gsc::mode_table_bytes(dim, M));
     return gsc::launch_direct_binned(dim, M, N, work, pos, pos_ld, out, out_ld,
                                      kind == gsc::KIND_INCOMPR, o->out_dtype, o->scale, bin, s,
@@ -507,7 +516,7 @@ int grid_launch(int field, int dim, int64_t M, const double* sf, const double* c
   if (field != GSC_FIELD_INCOMPR) {
     GSC_HIP(gsc::launch_prep(field_kind(field), dim, M, cov, z1, z2, sf, work, s));
     GSC_HIP(gsc::launch_direct_grid(dim, M, shape, axes, work, out, o->out_dtype, o->scale,
-                                    o->direct_variant, s));
+                                    resolve_variant(o, M), s));
     return GSC_OK;
   }
   const int64_t N = grid_points(dim, shape);
@@ -675,7 +684,7 @@ void gsc_opts_init(gsc_opts* o) {
   o->device = -1;
   o->num_threads = 0;
   o->accum = GSC_ACCUM_F32X2;
-  o->direct_variant = GSC_DIRECT_LOCAL;
+  o->direct_variant = GSC_DIRECT_AUTO;
   o->scale = 1.0;
   o->tile = GSC_TILE_AUTO;
 }

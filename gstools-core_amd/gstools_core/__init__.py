@@ -15,9 +15,10 @@ no CPU fallback.  Deliberate differences from the reference:
   * the GIL is released while the device works (ctypes drops it);
   * keyword-only extensions: out_dtype (np.float64 / np.float32), scale
     (multiplies the result, e.g. sqrt(var / M)), device (HIP ordinal),
-    variant ("local": points sorted into spatial cells, FP64 phase at each
-    cell centre and split-fp16 (scalar fields) or split-bf16 (incompressible
-    fields) matrix-core phases inside the cell, the default in 2-D and 3-D;
+    variant ("auto", the default: "local" from 256 modes up, "valu" below;
+    "local": points sorted into spatial cells, FP64 phase at each cell centre
+    and split-fp16 (scalar fields) or split-bf16 (incompressible fields)
+    matrix-core phases inside the cell, 2-D and 3-D;
     "valu": FP64 phases on the vector ALUs;
     "mfma": FP64 phases in matrix-core tiles, d <= 3).
 num_threads is accepted with the reference's meaning (worker threads); here
@@ -105,7 +106,7 @@ def _check_modes(cov, z1, z2, pos):
 
 
 def summate(cov_samples, z1, z2, pos, num_threads=None, *, out_dtype=np.float64, scale=1.0,
-            device=-1, variant="local"):
+            device=-1, variant="auto"):
     """Scalar randomization-method field (field.rs:37-65, lib.rs:33-48).
 
     u(x_j) = sum_m z1[m] cos(k_m . x_j) + z2[m] sin(k_m . x_j), k_m = cov_samples[:, m].
@@ -132,7 +133,7 @@ def summate(cov_samples, z1, z2, pos, num_threads=None, *, out_dtype=np.float64,
 
 
 def summate_incompr(cov_samples, z1, z2, pos, num_threads=None, *, out_dtype=np.float64,
-                    scale=1.0, device=-1, variant="local"):
+                    scale=1.0, device=-1, variant="auto"):
     """Incompressible vector field (field.rs:97-182, lib.rs:50-65); d must be 2 or 3.
 
     Returns a (d, N) array in Fortran order, like the reference (field.rs:166-174).
@@ -162,7 +163,7 @@ def summate_incompr(cov_samples, z1, z2, pos, num_threads=None, *, out_dtype=np.
 
 
 def summate_fourier(spectrum_factor, modes, z1, z2, pos, num_threads=None, *,
-                    out_dtype=np.float64, scale=1.0, device=-1, variant="local"):
+                    out_dtype=np.float64, scale=1.0, device=-1, variant="auto"):
     """Fourier-method field (field.rs:219-249, lib.rs:67-84).
 
     u(x_j) = sum_m sf[m] (z1[m] cos(k_m . x_j) + z2[m] sin(k_m . x_j)), k_m = modes[:, m].

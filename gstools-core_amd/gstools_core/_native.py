@@ -200,11 +200,11 @@ def lib():
 
 ACCUM = {"f64": 0, "f32x2": 1, "f32": 2}
 TILES = {"auto": 0, "256": 1, "128": 2}
-DIRECT_VARIANTS = {"mfma": 0, "valu": 1, "local": 2}
+DIRECT_VARIANTS = {"mfma": 0, "valu": 1, "local": 2, "auto": 3}
 
 
 def make_opts(out_dtype=GSC_F64, scale=1.0, device=-1, num_threads=0, path=GSC_PATH_AUTO,
-              precision=0, accum=1, direct_variant=2, tile=0) -> GscOpts:
+              precision=0, accum=1, direct_variant=3, tile=0) -> GscOpts:
     o = GscOpts()
     lib().gsc_opts_init(ctypes.byref(o))
     o.out_dtype = out_dtype

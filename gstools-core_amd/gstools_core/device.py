@@ -13,20 +13,20 @@ import ctypes
 from . import _native
 
 
-def workspace_bytes(dim: int, m: int, n: int, variant: str = "local") -> int:
+def workspace_bytes(dim: int, m: int, n: int, variant: str = "auto") -> int:
     """Device scratch (bytes) the *_dev summators need for `m` modes and `n`
     points in `dim` dimensions with direct kernel `variant`."""
     o = _opts(False, 1.0, variant)
     return int(_native.lib().gsc_workspace_bytes(int(dim), int(m), int(n), ctypes.byref(o)))
 
 
-def _opts(out_f32, scale, variant="local"):
+def _opts(out_f32, scale, variant="auto"):
     return _native.make_opts(out_dtype=_native.GSC_F32 if out_f32 else _native.GSC_F64,
                              scale=scale, direct_variant=_native.DIRECT_VARIANTS[variant])
 
 
 def summate_dev(dim, m, n, cov, z1, z2, pos, pos_ld, out, work, work_bytes, *, out_f32=False,
-                scale=1.0, stream=0, variant="local"):
+                scale=1.0, stream=0, variant="auto"):
     o = _opts(out_f32, scale, variant)
     rc = _native.lib().gsc_summate_dev(dim, m, n, cov, z1, z2, pos, pos_ld, out, work, work_bytes,
                                        ctypes.byref(o), ctypes.c_void_p(stream or None))
@@ -34,7 +34,7 @@ def summate_dev(dim, m, n, cov, z1, z2, pos, pos_ld, out, work, work_bytes, *, o
 
 
 def summate_incompr_dev(dim, m, n, cov, z1, z2, pos, pos_ld, out, out_ld, work, work_bytes, *,
-                        out_f32=False, scale=1.0, stream=0, variant="local"):
+                        out_f32=False, scale=1.0, stream=0, variant="auto"):
     o = _opts(out_f32, scale, variant)
     rc = _native.lib().gsc_summate_incompr_dev(dim, m, n, cov, z1, z2, pos, pos_ld, out, out_ld,
                                                work, work_bytes, ctypes.byref(o),
@@ -43,7 +43,7 @@ def summate_incompr_dev(dim, m, n, cov, z1, z2, pos, pos_ld, out, out_ld, work, 
 
 
 def summate_fourier_dev(dim, m, n, sf, modes, z1, z2, pos, pos_ld, out, work, work_bytes, *,
-                        out_f32=False, scale=1.0, stream=0, variant="local"):
+                        out_f32=False, scale=1.0, stream=0, variant="auto"):
     o = _opts(out_f32, scale, variant)
     rc = _native.lib().gsc_summate_fourier_dev(dim, m, n, sf, modes, z1, z2, pos, pos_ld, out,
                                                work, work_bytes, ctypes.byref(o),

@@ -90,7 +90,9 @@ typedef enum gsc_accum {
 typedef enum gsc_direct_variant {
   GSC_DIRECT_MFMA = 0,  /* v_mfma_f64_16x16x4 tiles of 16 modes x 16 points (dim <= 3) */
   GSC_DIRECT_VALU = 1,  /* FP64 FMA chains on the vector ALUs */
-  GSC_DIRECT_LOCAL = 2  /* cell-local phases from split-bf16 matrix-core products (default) */
+  GSC_DIRECT_LOCAL = 2, /* cell-local phases from split-bf16 matrix-core products */
+  GSC_DIRECT_AUTO = 3   /* LOCAL from 256 modes up, VALU below, where the cell sort costs
+                           more than it saves (default) */
 } gsc_direct_variant;
 
 /* Block tile of the factored path's GEMM: 256 rows (4 waves of 64 rows, one

@@ -43,7 +43,7 @@ def test_version_and_defaults():
     o = _native.GscOpts()
     lib.gsc_opts_init(ctypes.byref(o))
     assert (o.out_dtype, o.path, o.precision, o.device, o.scale) == (0, 0, 0, -1, 1.0)
-    assert (o.accum, o.direct_variant, o.blocks_per_cu) == (1, 2, 0)
+    assert (o.accum, o.direct_variant, o.blocks_per_cu, o.table_bytes) == (1, 3, 0, 0)
     assert gstools_core.__version__ == "1.1.0"
 
 
@@ -66,6 +66,10 @@ def test_workspace_bytes():
     loc = device.workspace_bytes(3, 1000, 10**6)
     assert loc >= valu + 8 * 10**6
     assert device.workspace_bytes(2, 1000, 10**6) >= valu + 8 * 10**6
+    # the default variant ("auto") takes the local kernel from 256 modes up and
+    # the FP64-phase VALU kernel (mode table only) below
+    assert device.workspace_bytes(3, 255, 10**6) == device.workspace_bytes(3, 255, 10**6, "valu")
+    assert device.workspace_bytes(3, 256, 10**6) == device.workspace_bytes(3, 256, 10**6, "local")
     # dimensions the local kernel does not serve need only the mode table
     assert device.workspace_bytes(1, 1000, 10**6) == device.workspace_bytes(1, 1000, 10**6, "valu")
     assert device.workspace_bytes(5, 10, 100) >= 7 * 8 * 10

@@ -528,3 +528,19 @@ def test_roofline_calibrations_are_consistent():
     assert sf.value < s.value
     assert 2e12 < mv.value < ms.value < s.value
     assert lib.gsc_calibrate_local_mix(0, ctypes.byref(ms), ctypes.byref(mv)) != 0
+
+
+@pytest.mark.parametrize("m", [255, 256])
+def test_auto_variant_picks_by_mode_count(m):
+    # the default variant runs the FP64-phase VALU kernel below 256 modes and
+    # the local kernel from 256 up, bit for bit as when asked by name
+    rng = np.random.default_rng(90 + m)
+    for d in (2, 3):
+        cov = rng.normal(0, 0.2, size=(d, m))
+        z1, z2 = rng.normal(size=m), rng.normal(size=m)
+        pos = rng.uniform(0, 50, size=(d, 5000))
+        want = "local" if m >= 256 else "valu"
+        np.testing.assert_array_equal(gstools_core.summate(cov, z1, z2, pos),
+                                      gstools_core.summate(cov, z1, z2, pos, variant=want))
+        np.testing.assert_array_equal(gstools_core.summate_incompr(cov, z1, z2, pos),
+                                      gstools_core.summate_incompr(cov, z1, z2, pos, variant=want))
