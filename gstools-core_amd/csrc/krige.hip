@@ -7,11 +7,14 @@
 // field and error (calculator_field_krige_and_variance): W costs 2 n^2 N
 // flops and is computed tile by tile on the FP64 matrix cores
 // (v_mfma_f64_16x16x4f64); the two contractions with cond and vecs are fused
-// into the tile epilogue, so W never reaches memory.  A block owns 64 rows i
-// (four waves of 16) and 128 targets j (eight 16-column MFMA tiles per wave);
-// K runs over the n conditions in LDS-staged chunks of 32.  Each block writes
-// its partial sums over its 64 rows; a second kernel adds the partials of
-// all row blocks in a fixed order (results are bitwise reproducible).
+// into the tile epilogue, so W never reaches memory.  A block owns 128 rows i
+// (four waves of 32: two 16-row MFMA tiles each, so every B operand read from
+// LDS feeds two matrix instructions) and 128 targets j (eight 16-column MFMA
+// tiles per wave); K runs over the n conditions in LDS-staged chunks of 16
+// (the accumulators, 128 registers, and the next chunk's prefetch fit two
+// waves per SIMD).  Each block writes its partial sums over its rows; a
+// second kernel adds the partials of all row blocks in a fixed order
+// (results are bitwise reproducible).
 //
 // field only (calculator_field_krige): the same sum reassociated as
 // field[j] = sum_k vecs[k][j] w[k], w = mat cond (n^2 + 2 n N flops, bound
@@ -26,8 +29,9 @@ namespace {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kKI = 64;  // rows i per block (four waves of 16)
-constexpr int kKK = 32;  // conditions k per LDS chunk
+constexpr int kRS = 2;        // 16-row MFMA tiles per wave
+constexpr int kKI = 64 * kRS; // rows i per block (four waves of 16 kRS)
+constexpr int kKK = 16;       // conditions k per LDS chunk
 constexpr int kTJ = 8;   // 16-column MFMA tiles per wave: kKJ = 16 kTJ targets per block
 constexpr int kKJ = 16 * kTJ;
 constexpr int kXcds = 8; // blocks are dispatched to the XCDs round-robin
@@ -72,9 +76,11 @@ k_krige_tiles(int64_t n, int64_t N, const double* __restrict__ mat,
   if (!krige_block(nib, njb, ib, jb)) return;
   const int64_t i0 = static_cast<int64_t>(ib) * kKI, j0 = static_cast<int64_t>(jb) * kKJ;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, jl = lane & 15;
-  f64x4 acc[kTJ];
+  f64x4 acc[kRS][kTJ];
 #pragma unroll
-  for (int t = 0; t < kTJ; ++t) acc[t] = f64x4{0.0, 0.0, 0.0, 0.0};
+  for (int r = 0; r < kRS; ++r)
+#pragma unroll
+    for (int t = 0; t < kTJ; ++t) acc[r][t] = f64x4{0.0, 0.0, 0.0, 0.0};
   // this thread's share of a chunk: mat[k0 + tm_k + kStepM u][i0 + tm_i] and
   // vecs[k0 + tv_k + kStepV u][j0 + tv_j] (zero outside), i.e. element
   // e = threadIdx.x + 256 u of the row-major chunk tiles
@@ -124,27 +130,34 @@ k_krige_tiles(int64_t n, int64_t N, const double* __restrict__ mat,
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int kk = 0; kk < kKK; kk += 4) {
-      const double a = sm.st.m[kk + g][16 * w + jl];
+      double a[kRS];
 #pragma unroll
-      for (int t = 0; t < kTJ; ++t)
-        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sm.st.v[kk + g][16 * t + jl], acc[t], 0,
-                                                      0, 0);
+      for (int r = 0; r < kRS; ++r) a[r] = sm.st.m[kk + g][16 * (kRS * w + r) + jl];
+#pragma unroll
+      for (int t = 0; t < kTJ; ++t) {
+        const double b = sm.st.v[kk + g][16 * t + jl];
+#pragma unroll
+        for (int r = 0; r < kRS; ++r)
+          acc[r][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b, acc[r][t], 0, 0, 0);
+      }
     }
     __syncthreads();
   }
-  // epilogue: acc[t][it] = W[i0 + 16 w + 4 it + g][j0 + 16 t + jl]
+  // epilogue: acc[r][t][it] = W[i0 + 16 (kRS w + r) + 4 it + g][j0 + 16 t + jl]
 #pragma unroll
   for (int t = 0; t < kTJ; ++t) {
     const int64_t j = j0 + 16 * t + jl;
     double pf = 0.0, pe = 0.0;
 #pragma unroll
-    for (int it = 0; it < 4; ++it) {
-      const int64_t i = i0 + 16 * w + 4 * it + g;
-      const bool ok = i < n && j < N;
-      const double wv = acc[t][it];
-      pf = __builtin_fma(ok ? cond[i] : 0.0, wv, pf);
-      pe = __builtin_fma(ok ? vecs[i * vld + j] : 0.0, wv, pe);
-    }
+    for (int r = 0; r < kRS; ++r)
+#pragma unroll
+      for (int it = 0; it < 4; ++it) {
+        const int64_t i = i0 + 16 * (kRS * w + r) + 4 * it + g;
+        const bool ok = i < n && j < N;
+        const double wv = acc[r][t][it];
+        pf = __builtin_fma(ok ? cond[i] : 0.0, wv, pf);
+        pe = __builtin_fma(ok ? vecs[i * vld + j] : 0.0, wv, pe);
+      }
     sm.red.f[4 * w + g][16 * t + jl] = pf;
     sm.red.e[4 * w + g][16 * t + jl] = pe;
   }

@@ -49,6 +49,7 @@ def _scale(m, v, c):
 
 
 @pytest.mark.parametrize("n,N", [(1, 1), (3, 6), (63, 65), (64, 64), (65, 1000), (130, 63),
+                                 (127, 129), (128, 128), (129, 300), (257, 2000),
                                  (200, 5000), (1, 777)])
 def test_random_against_oracle(n, N):
     rng = np.random.default_rng(1000 * n + N)
