@@ -207,14 +207,26 @@ k_krige_w(int64_t n, const double* __restrict__ mat, const double* __restrict__ 
   if (lane == 0) w[k] = s;
 }
 
-// field[j] = sum_k vecs[k][j] w[k]: one thread per target, rows in order.
+// field[j] = sum_k vecs[k][j] w[k]: one thread per target, rows in order;
+// kFieldRows rows' loads are issued before their FMAs, so that many loads
+// per wave are in flight.
+constexpr int kFieldRows = 8;
 __global__ void __launch_bounds__(256)
 k_krige_field(int64_t n, int64_t N, const double* __restrict__ vecs, int64_t vld,
               const double* __restrict__ w, double* __restrict__ field, int64_t fs) {
   const int64_t j = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
   if (j >= N) return;
+  const double* p = vecs + j;
   double f = 0.0;
-  for (int64_t k = 0; k < n; ++k) f = __builtin_fma(vecs[k * vld + j], w[k], f);
+  int64_t k = 0;
+  for (; k + kFieldRows <= n; k += kFieldRows) {
+    double v[kFieldRows];
+#pragma unroll
+    for (int u = 0; u < kFieldRows; ++u) v[u] = p[(k + u) * vld];
+#pragma unroll
+    for (int u = 0; u < kFieldRows; ++u) f = __builtin_fma(v[u], w[k + u], f);
+  }
+  for (; k < n; ++k) f = __builtin_fma(p[k * vld], w[k], f);
   field[j * fs] = f;
 }
 

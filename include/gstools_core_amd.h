@@ -410,7 +410,7 @@ int gsc_krige(int64_t n, int64_t N, const double* mat, int64_t mat_s0, int64_t m
               const double* vecs, int64_t vecs_s0, int64_t vecs_s1, const double* cond,
               int64_t cond_s, double* field, int64_t field_s, double* error, int64_t error_s,
               const gsc_opts* opts);
-/* Workspace of gsc_krige_dev: 2 * 8 * ceil(n / 64) * N bytes with the error,
+/* Workspace of gsc_krige_dev: 2 * 8 * ceil(n / 128) * N bytes with the error,
  * 8 n bytes for the field alone. */
 size_t gsc_krige_workspace_bytes(int64_t n, int64_t N, int with_error);
 /* Device form: mat dense (n, n) row-major, vecs (n, N) with row pitch

@@ -76,6 +76,15 @@ def test_workspace_bytes():
     assert device.workspace_bytes(0, 10, 10) == 0
 
 
+def test_krige_workspace_bytes():
+    # the documented sizes: per-row-block partial sums of field and error
+    # (blocks of 128 conditions), or w = mat cond for the field alone
+    lib = _native.lib()
+    for n, N in ((1, 1), (128, 10), (129, 10), (1000, 10**6)):
+        assert lib.gsc_krige_workspace_bytes(n, N, 1) == 2 * 8 * (-(-n // 128)) * N
+        assert lib.gsc_krige_workspace_bytes(n, N, 0) == 8 * n
+
+
 def _ok_args(d=3, m=4, n=5):
     rng = np.random.default_rng(0)
     return rng.normal(size=(d, m)), rng.normal(size=m), rng.normal(size=m), rng.normal(size=(d, n))
