@@ -2,7 +2,7 @@
 golden values and the FP64 oracle.
 
 Tolerance: the reference's own tests allow 6 ulps (krige.rs:208, 220, 243);
-the GPU sums in another order (tiles of 64 conditions, FMA), so random cases
+the GPU sums in another order (tiles of 128 conditions, FMA), so random cases
 are held to 1e-12 of the largest term of the sum, i.e. FP64 rounding only.
 """
 from __future__ import annotations
