@@ -125,6 +125,39 @@ def test_device_entry_point_matches_host():
         b.free()
 
 
+@pytest.mark.parametrize("with_error", [True, False])
+def test_device_entry_point_row_pitch(with_error):
+    # krig_vecs rows with a pitch above N (a view of a wider array): the same
+    # results as the dense host call, bit for bit
+    import ctypes
+    rng = np.random.default_rng(10)
+    n, N, pitch = 140, 1000, 1037
+    m = rng.normal(size=(n, n))
+    wide = rng.normal(size=(n, pitch))
+    c = rng.normal(size=n)
+    f, e = gstools_core.calc_field_krige_and_variance(m, np.ascontiguousarray(wide[:, :N]), c)
+    if not with_error:
+        f = gstools_core.calc_field_krige(m, np.ascontiguousarray(wide[:, :N]), c)
+    lib = _native.lib()
+    dm = _native.DeviceBuffer.from_array(m)
+    dv = _native.DeviceBuffer.from_array(wide)
+    dc = _native.DeviceBuffer.from_array(c)
+    out = _native.DeviceBuffer(16 * N)
+    nb = lib.gsc_krige_workspace_bytes(n, N, 1 if with_error else 0)
+    work = _native.DeviceBuffer(nb)
+    opts = _native.make_opts()
+    _native.check(lib.gsc_krige_dev(n, N, dm.ptr, dv.ptr, pitch, dc.ptr, out.ptr,
+                                    out.ptr + 8 * N if with_error else None, work.ptr, nb,
+                                    ctypes.byref(opts), None))
+    _native.synchronize()
+    res = out.to_array(np.float64, 2 * N)
+    np.testing.assert_array_equal(res[:N], f)
+    if with_error:
+        np.testing.assert_array_equal(res[N:], e)
+    for b in (dm, dv, dc, out, work):
+        b.free()
+
+
 def test_empty_and_shape_errors():
     f, e = gstools_core.calc_field_krige_and_variance(np.zeros((0, 0)), np.zeros((0, 4)),
                                                       np.zeros(0))
