@@ -214,6 +214,41 @@ def test_device_entry_point_matches_host_entry_point():
     np.testing.assert_array_equal(outi.cpu().numpy(), hosti)
 
 
+@pytest.mark.parametrize("variant", ["local", "valu"])
+def test_device_entry_points_with_row_pitch(variant):
+    # positions and incompressible output rows with pitches above N (views of
+    # wider arrays): the same values as the dense host calls, bit for bit
+    torch = pytest.importorskip("torch")
+    from gstools_core import device
+    w = workloads.c2(n=30_000)
+    md = w.modes
+    n, ld, old = w.n_points, w.n_points + 301, w.n_points + 77
+    dev = torch.device("cuda:0")
+    cov = torch.from_numpy(np.ascontiguousarray(md.cov_samples)).to(dev)
+    z1 = torch.from_numpy(md.z1).to(dev)
+    z2 = torch.from_numpy(md.z2).to(dev)
+    wide = torch.full((3, ld), float("nan"), dtype=torch.float64, device=dev)
+    wide[:, :n] = torch.from_numpy(np.ascontiguousarray(w.pos)).to(dev)
+    nb = device.workspace_bytes(3, md.m, n, variant=variant)
+    work = torch.empty(nb, dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    out = torch.empty(n, dtype=torch.float64, device=dev)
+    device.summate_dev(3, md.m, n, cov.data_ptr(), z1.data_ptr(), z2.data_ptr(),
+                       wide.data_ptr(), ld, out.data_ptr(), work.data_ptr(), nb,
+                       stream=stream, variant=variant)
+    outi = torch.full((3, old), -7.0, dtype=torch.float64, device=dev)
+    device.summate_incompr_dev(3, md.m, n, cov.data_ptr(), z1.data_ptr(), z2.data_ptr(),
+                               wide.data_ptr(), ld, outi.data_ptr(), old, work.data_ptr(), nb,
+                               stream=stream, variant=variant)
+    torch.cuda.synchronize()
+    host = gstools_core.summate(md.cov_samples, md.z1, md.z2, w.pos, variant=variant)
+    hosti = gstools_core.summate_incompr(md.cov_samples, md.z1, md.z2, w.pos, variant=variant)
+    np.testing.assert_array_equal(out.cpu().numpy(), host)
+    oi = outi.cpu().numpy()
+    np.testing.assert_array_equal(oi[:, :n], hosti)
+    assert np.all(oi[:, n:] == -7.0)  # nothing written past N in a row
+
+
 def test_committed_oracle_vectors():
     import os
     from conftest import GOLDEN
