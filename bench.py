@@ -12,7 +12,10 @@ Prints ONE JSON line: metric, value (point*modes/s), roofline of the
 summation (transcendental-issue bound: one v_sin_f32 per point-mode), the
 CPU FP64 oracle timed on a bounded sample of the same workload on this host,
 the other direct variants, the structured-grid leg (256^3, factored MFMA vs
-direct) and the ensemble leg (512 realisations).
+direct), the ensemble leg (512 realisations), the other configurations' legs
+(2-D C1, incompressible, Fourier method, heavy-tailed 512^3 grid, kriging)
+and, per configuration, the CPU oracle on a bounded sample beside the GPU
+rate of the same workload.
 """
 from __future__ import annotations
 
@@ -421,6 +424,23 @@ def incompr_leg(steps: int, warmup: int, n: int = 4_000_000, m: int = 1000,
     return res
 
 
+def fourier_modes(side: int = 10):
+    """Modes of the Fourier-method workload: M = side^3 wave vectors on a
+    regular k-grid (spacing 2 pi / 200, centred on 0), spectrum factors of a
+    Gaussian spectrum (l = 20, sum of sf^2 = 1), seeded z1, z2."""
+    import numpy as np
+
+    m = side ** 3
+    dk = 2.0 * np.pi / 200.0
+    ax = (np.arange(side) - (side - 1) / 2.0) * dk
+    kg = np.ascontiguousarray(np.stack(np.meshgrid(ax, ax, ax, indexing="ij")).reshape(3, m))
+    sf = np.exp(-((kg ** 2).sum(axis=0)) * 20.0 ** 2 / 8.0)
+    sf = sf / np.sqrt((sf ** 2).sum())
+    rng = np.random.default_rng(20240607)
+    z1, z2 = rng.normal(size=m), rng.normal(size=m)
+    return sf, kg, z1, z2
+
+
 def fourier_leg(steps: int, warmup: int, n: int = 10_000_000, side: int = 10) -> dict:
     """summate_fourier (field.rs:219-249) on the 3-D unstructured positions of
     the direct workload (N = 1e7 uniform in [0,500]^3) with M = side^3 modes
@@ -434,13 +454,7 @@ def fourier_leg(steps: int, warmup: int, n: int = 10_000_000, side: int = 10) ->
 
     w = workloads.c2(n=n)
     d, m = 3, side ** 3
-    dk = 2.0 * np.pi / 200.0
-    ax = (np.arange(side) - (side - 1) / 2.0) * dk
-    kg = np.stack(np.meshgrid(ax, ax, ax, indexing="ij")).reshape(3, m)
-    sf = np.exp(-((kg ** 2).sum(axis=0)) * 20.0 ** 2 / 8.0)
-    sf = sf / np.sqrt((sf ** 2).sum())
-    rng = np.random.default_rng(20240607)
-    z1, z2 = rng.normal(size=m), rng.normal(size=m)
+    sf, kg, z1, z2 = fourier_modes(side)
     dev = torch.device("cuda:0")
     t_sf = torch.from_numpy(sf).to(dev)
     t_k = torch.from_numpy(np.ascontiguousarray(kg)).to(dev)
@@ -463,6 +477,100 @@ def fourier_leg(steps: int, warmup: int, n: int = 10_000_000, side: int = 10) ->
         _, tot = time_steps(step, steps, warmup, stream)
         ms = tot / steps
         res[f"{vname}_phase"] = {"ms_per_step": ms, "point_modes_per_s": n * m / (ms * 1e-3)}
+    return res
+
+
+def c1_leg(steps: int, warmup: int) -> dict:
+    """The 2-D configuration the reference runs on a CPU (N = 1e5 points
+    uniform in [0,100]^2, M = 1000 Gaussian modes, l = 10): one device call
+    (default variant), device-resident, FP64 out."""
+    import numpy as np
+    import torch
+
+    from gstools_core import device, workloads
+
+    w = workloads.c1()
+    md = w.modes
+    d, m, n = md.dim, md.m, w.n_points
+    dev = torch.device("cuda:0")
+    cov = torch.from_numpy(np.ascontiguousarray(md.cov_samples)).to(dev)
+    z1 = torch.from_numpy(md.z1).to(dev)
+    z2 = torch.from_numpy(md.z2).to(dev)
+    pos = torch.from_numpy(np.ascontiguousarray(w.pos)).to(dev)
+    out = torch.empty(n, dtype=torch.float64, device=dev)
+    nb = device.workspace_bytes(d, m, n)
+    work = torch.empty(nb, dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def step():
+        device.summate_dev(d, m, n, cov.data_ptr(), z1.data_ptr(), z2.data_ptr(), pos.data_ptr(),
+                           n, out.data_ptr(), work.data_ptr(), nb, stream=stream.cuda_stream)
+
+    _, tot = time_steps(step, steps, warmup, stream)
+    ms = tot / steps
+    return {"workload": f"2-D unstructured summate, N={n} uniform in [0,100]^2, M={m} Gaussian "
+                        "modes (l=10)", "ms_per_step": ms, "point_modes_per_s": n * m / (ms * 1e-3)}
+
+
+def config_cpu_baselines(gpu: dict, threads: int) -> dict:
+    """The FP64 oracle (the C restatement of the reference, OpenMP over this
+    host's threads) on bounded samples of every configuration, each beside
+    the GPU rate of the same workload measured in this run (gpu: config ->
+    point-modes/s): C1 in full; C2i in the reference's mode-outer chunked
+    order (field.rs:130-163) on its first 2.5e5 points; C3 and C5 on seeded
+    random subsets of their grids; C4 on its first four realisations in full;
+    the Fourier-method workload on its first 2.5e5 points.  threads: the
+    OpenMP threads of the main baseline (set explicitly on every call: the
+    oracle's thread count persists between calls)."""
+    import numpy as np
+
+    import oracle
+    from gstools_core import workloads
+
+    res = {}
+
+    def timed(name, fn, pm, sample):
+        t0 = time.perf_counter()
+        fn()
+        dt = time.perf_counter() - t0
+        r = {"value": pm / dt, "unit": "point·modes/s", "cores": threads, "kind": "port",
+             "sample": f"{sample}, {dt:.2f} s wall"}
+        g = gpu.get(name)
+        if g:
+            r["gpu_point_modes_per_s"] = g
+            r["gpu_over_cpu"] = g / r["value"]
+        res[name] = r
+
+    w = workloads.c1()
+    md = w.modes
+    timed("c1", lambda: oracle.summate(md.cov_samples, md.z1, md.z2, w.pos, num_threads=threads),
+          w.point_modes, f"all {w.n_points} points x {md.m} modes (2-D)")
+    ni = 250_000
+    w = workloads.c2(n=4_000_000)
+    md = w.modes
+    pos = np.ascontiguousarray(w.pos[:, :ni])
+    timed("c2i", lambda: oracle.summate_incompr_modechunk(md.cov_samples, md.z1, md.z2, pos,
+                                                           num_threads=threads),
+          float(ni) * md.m, f"summate_incompr, first {ni} points x {md.m} modes, mode-outer order")
+    for name, wl, k in (("c3", workloads.c3(), 250_000), ("c5", workloads.c5(), 100_000)):
+        md = wl.modes
+        sub = workloads.grid_points(wl.axes, workloads.subset_indices(wl.n_points, k))
+        timed(name, lambda md=md, sub=sub: oracle.summate(md.cov_samples, md.z1, md.z2, sub,
+                                                          num_threads=threads),
+              float(k) * md.m, f"seeded {k}-point subset of the grid x {md.m} modes")
+    ens = workloads.c4(count=4)
+    pts = [oracle.grid_points(r.axes) for r in ens]
+
+    def run_c4():
+        for r, p in zip(ens, pts):
+            oracle.summate(r.modes.cov_samples, r.modes.z1, r.modes.z2, p, num_threads=threads)
+
+    timed("c4", run_c4, sum(r.point_modes for r in ens),
+          f"first 4 realisations in full ({sum(r.n_points for r in ens)} points x 1000 modes)")
+    sf, kg, z1, z2 = fourier_modes()
+    pos = np.ascontiguousarray(workloads.c2(n=ni).pos)
+    timed("fourier", lambda: oracle.summate_fourier(sf, kg, z1, z2, pos, num_threads=threads),
+          float(ni) * sf.size, f"summate_fourier, first {ni} points x {sf.size} modes")
     return res
 
 
@@ -746,10 +854,27 @@ def main() -> int:
                                   "point_modes_per_s": ni * m / (hi_ms * 1e-3),
                                   "note": "summate_incompr, (3, N) Fortran-order result"}
 
+    log("c1 leg (2-D, N = 1e5)")
+    c1_res = c1_leg(args.steps, args.warmup)
+
     cpu = None
+    cpu_cfg = None
     if args.cpu_sample > 0:
         log(f"cpu baseline on {args.cpu_sample} points")
         cpu = cpu_baseline(w, min(args.cpu_sample, n))
+        log("cpu baselines of the other configurations")
+        gpu_rates = {"c1": c1_res["point_modes_per_s"]}
+        if inc_res:
+            gpu_rates["c2i"] = inc_res["local_phase"]["point_modes_per_s"]
+        if grid_res:
+            gpu_rates["c3"] = grid_res["factored_fp16x3_f32x2"]["point_modes_per_s"]
+        if c5_res:
+            gpu_rates["c5"] = c5_res["factored_fp16x3"]["point_modes_per_s"]
+        if ens_res:
+            gpu_rates["c4"] = ens_res["point_modes_per_s"]
+        if fourier_res:
+            gpu_rates["fourier"] = fourier_res["local_phase"]["point_modes_per_s"]
+        cpu_cfg = config_cpu_baselines(gpu_rates, cpu["cores"])
 
     line = {
         "metric": METRIC,
@@ -770,6 +895,8 @@ def main() -> int:
         "roofline": roofline,
         "sine_calibration": sine,
         "cpu_baseline": cpu,
+        "cpu_baselines_by_config": cpu_cfg,
+        "c1": c1_res,
         "direct_variants": direct_variants,
         "host_arrays": host_arrays,
         "incompr": inc_res,
