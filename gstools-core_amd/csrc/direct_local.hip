@@ -295,12 +295,41 @@ k_box_partial(const double* __restrict__ pos, int64_t ld, int64_t N, double* __r
     }
 }
 
+// Modes of a 256-thread block held in registers: mode 256 r + thread for
+// r < kPreModes (M <= 1024 in one load per thread, issued before anything
+// waits on it); later modes are read from the table where they are used.
+constexpr int kPreModes = 4;
+struct PreModes {
+  ModeRec r[kPreModes];
+};
+__device__ __forceinline__ void pre_load(const ModeRec* __restrict__ modes, int M, PreModes& p) {
+#pragma unroll
+  for (int r = 0; r < kPreModes; ++r) {
+    const int m = 256 * r + static_cast<int>(threadIdx.x);
+    p.r[r] = m < M ? modes[m] : ModeRec{};
+  }
+}
+// Record of mode 256 rr + thread (m < M); rr is uniform over the block, so
+// the register copy is chosen by a scalar branch.
+__device__ __forceinline__ ModeRec pre_get(const ModeRec* __restrict__ modes, const PreModes& p,
+                                           int rr, int m) {
+  switch (rr) {
+    case 0: return p.r[0];
+    case 1: return p.r[1];
+    case 2: return p.r[2];
+    case 3: return p.r[3];
+    default: return modes[m];
+  }
+}
+static_assert(kPreModes == 4, "pre_get names every register copy");
+
 // One block of 256 threads: modes in order, split into smooth (FP32 local
 // phase) and rough (FP64 phase) lists by a block scan, so both keep the
 // original mode order; h is the cell width.  Thread 0 receives the list
 // lengths.
 template <int D>
-__device__ void split_modes(const ModeRec* __restrict__ modes, int M, const double* h,
+__device__ void split_modes(const ModeRec* __restrict__ modes, const PreModes& pre, int M,
+                            const double* h,
                             uint4* __restrict__ btpl, uint4* __restrict__ btpl32,
                             float4* __restrict__ amp,
                             SmoothRec* __restrict__ sm, ModeRec* __restrict__ rough,
@@ -313,7 +342,7 @@ __device__ void split_modes(const ModeRec* __restrict__ modes, int M, const doub
     ModeRec r{};
     bool smooth = false;
     if (m < M) {
-      r = modes[m];
+      r = pre_get(modes, pre, base / 256, m);
       double b = 0.0;
 #pragma unroll
       for (int k = 0; k < D; ++k) b += fabs(r.kap[k]) * 0.5 * h[k];
@@ -411,18 +440,33 @@ k_bin_params(const double* __restrict__ box, int64_t N, int64_t max_cells,
   __shared__ BinParams Ps;
   __shared__ double red[2 * D][256];
   __shared__ int qh[kQBins];
+  // every global read this single block needs is issued first: the mode
+  // records (up to 1024) and the partial boxes
+  PreModes pre;
+  pre_load(modes, M, pre);
+  constexpr int kBoxPer = kBoxBlocks / 256;
+  static_assert(kBoxPer * 256 == kBoxBlocks, "partial boxes per thread");
+  double bx[kBoxPer][2 * D];
+#pragma unroll
+  for (int u = 0; u < kBoxPer; ++u)
+#pragma unroll
+    for (int c = 0; c < 2 * D; ++c) bx[u][c] = box[(threadIdx.x + 256 * u) * 6 + c];
   for (int b = threadIdx.x; b < kQBins; b += 256) qh[b] = 0;
   __syncthreads();
-  for (int m = threadIdx.x; m < M; m += 256) {
+  auto quant = [&](const ModeRec& r) {
     double sk = 0.0;
 #pragma unroll
-    for (int k = 0; k < D; ++k) sk += fabs(modes[m].kap[k]);
-    if (!isfinite(sk)) continue;
+    for (int k = 0; k < D; ++k) sk += fabs(r.kap[k]);
+    if (!isfinite(sk)) return;
     const int b = sk > 0.0 ? static_cast<int>(fmin(fmax(floor(4.0 * log2(sk)) + kQBins / 2, 0.0),
                                                    kQBins - 1.0))
                            : 0;
     atomicAdd(&qh[b], 1);
-  }
+  };
+#pragma unroll
+  for (int r = 0; r < kPreModes; ++r)
+    if (256 * r + static_cast<int>(threadIdx.x) < M) quant(pre.r[r]);
+  for (int m = 256 * kPreModes + threadIdx.x; m < M; m += 256) quant(modes[m]);
   __syncthreads();
   // the quantile's bin: inclusive scan of bin pairs, then the pair that
   // crosses the wanted count
@@ -450,9 +494,10 @@ k_bin_params(const double* __restrict__ box, int64_t N, int64_t max_cells,
 #pragma unroll
   for (int k = 0; k < D; ++k) {
     double a = INFINITY, b = -INFINITY;
-    for (int i = threadIdx.x; i < kBoxBlocks; i += 256) {
-      a = fmin(a, box[i * 6 + 2 * k]);
-      b = fmax(b, box[i * 6 + 2 * k + 1]);
+#pragma unroll
+    for (int u = 0; u < kBoxPer; ++u) {
+      a = fmin(a, bx[u][2 * k]);
+      b = fmax(b, bx[u][2 * k + 1]);
     }
     red[2 * k][threadIdx.x] = a;
     red[2 * k + 1][threadIdx.x] = b;
@@ -528,7 +573,7 @@ k_bin_params(const double* __restrict__ box, int64_t N, int64_t max_cells,
   }
   __syncthreads();
   // the modes' smooth / rough split depends on the cell width
-  split_modes<D>(modes, M, Ps.h, btpl, btpl32, amp, sm, rough, &Ps.n_smooth, &Ps.n_rough);
+  split_modes<D>(modes, pre, M, Ps.h, btpl, btpl32, amp, sm, rough, &Ps.n_smooth, &Ps.n_rough);
   __syncthreads();
   if (threadIdx.x == 0) *prm = Ps;
 }
