@@ -198,7 +198,8 @@ BinLayout bin_layout(int64_t M, int64_t N) {
   BinLayout L{};
   L.max_cells = std::min<int64_t>(kMaxCells, N / kMinCellPts + 1);
   L.max_waves = (N + kWavePts - 1) / kWavePts + L.max_cells;
-  L.nblk = std::max<int64_t>(1, std::min<int64_t>(kMaxSortBlocks, (N + 16383) / 16384));
+  // point blocks of at least one round (kSortThreads x kSortUnroll points)
+  L.nblk = std::max<int64_t>(1, std::min<int64_t>(kMaxSortBlocks, (N + 4095) / 4096));
   L.blk_pts = (N + L.nblk - 1) / L.nblk;
   const size_t m = static_cast<size_t>(M > 0 ? M : 1);
   size_t off = 0;
@@ -787,39 +788,48 @@ __device__ __forceinline__ void cell_centre(const BinParams& P, int e, double (&
   }
 }
 
-// One wave per cell: records its centre and its waves in the wave -> cell
-// map, marks the slots past its points as padding and, when r0w is given,
-// writes the cell's row of r0 (the reduced FP64 phase at the centre, rounded
-// to FP32) for every smooth mode, which the main kernels read instead of
-// evaluating r0 in every wave of the cell:
+// kWaveSlices waves per cell, over a grid-stride loop of (cell, slice) items
+// (a quarter of a row per wave keeps the dependent loads per wave short, and
+// small cell counts still spread over many waves).  Slice 0 records the
+// cell's centre and its waves in the wave -> cell map and marks the slots
+// past its points as padding; when r0w is given, every slice
+// writes its part of the cell's row of r0 (the reduced FP64 phase at the
+// centre, rounded to FP32) for every smooth mode, which the main kernels
+// read instead of evaluating r0 in every wave of the cell:
 // - scalar form (vec = false): words r0_hi | r0_lo << 16 (fp16 parts) up to
 //   the end of the last chunk, zero phase data past n_smooth;
 // - vector form: pairs (p0 | p1 << 16, p2) of the three bf16 parts, r0ld of
 //   them, zero past n_smooth.
+constexpr int kWaveSlices = 4;  // waves per cell in k_bin_waves
+constexpr int kWavesMax = 8192; // waves of the k_bin_waves grid at most
 template <int D>
-__global__ void __launch_bounds__(256)
-k_bin_waves(const BinParams* __restrict__ prm, const int* __restrict__ counts,
-            const int* __restrict__ offsets, double* __restrict__ centre,
-            int* __restrict__ wave_cell, float4* __restrict__ recs,
-            const SmoothRec* __restrict__ sm, uint32_t* __restrict__ r0w, int64_t r0ld,
-            bool vec) {
-  const int e = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const BinParams P = *prm;
-  if (e >= P.ncells) return;
+__device__ __forceinline__ void cell_slice(const BinParams& P, int e, int slice, int lane,
+                                           const int* __restrict__ counts,
+                                           const int* __restrict__ offsets,
+                                           double* __restrict__ centre,
+                                           int* __restrict__ wave_cell,
+                                           float4* __restrict__ recs,
+                                           const SmoothRec* __restrict__ sm,
+                                           uint32_t* __restrict__ r0w, int64_t r0ld, bool vec) {
   double X[3];
   cell_centre<D>(P, e, X);
-  if (lane == 0)
-#pragma unroll
-    for (int k = 0; k < 3; ++k) centre[4 * e + k] = X[k];
   const int s0 = offsets[e], s1 = offsets[e + 1];
-  for (int s = s0 + kWavePts * lane; s < s1; s += kWavePts * 64) wave_cell[s / kWavePts] = e;
-  const int sp = s0 + counts[e] + lane;  // padding: fewer than kWavePts slots
-  if (sp < s1) recs[sp] = make_rec(0.0f, 0.0f, 0.0f, -1);
+  if (slice == 0) {
+    if (lane == 0)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) centre[4 * e + k] = X[k];
+    for (int s = s0 + kWavePts * lane; s < s1; s += kWavePts * 64) wave_cell[s / kWavePts] = e;
+    const int sp = s0 + counts[e] + lane;  // padding: fewer than kWavePts slots
+    if (sp < s1) recs[sp] = make_rec(0.0f, 0.0f, 0.0f, -1);
+  }
   if (r0w == nullptr || s1 == s0) return;  // no table, or no wave reads this row
   if (vec) {
     uint2* row = reinterpret_cast<uint2*>(r0w) + e * r0ld;
+    // the slice's entries: a quarter of the row, whole 64-entry steps
+    const int64_t per = (r0ld + 64 * kWaveSlices - 1) / (64 * kWaveSlices) * 64;
+    const int64_t i1 = min(r0ld, per * (slice + 1));
 #pragma unroll 4
-    for (int i = lane; i < r0ld; i += 64) {
+    for (int64_t i = per * slice + lane; i < i1; i += 64) {
       uint2 v = make_uint2(0u, 0u);
       if (i < P.n_smooth) {
         const SmoothRec sr = sm[i];
@@ -838,8 +848,10 @@ k_bin_waves(const BinParams* __restrict__ prm, const int* __restrict__ counts,
   }
   uint32_t* row = r0w + e * r0ld;
   const int end = (P.n_smooth + kModeChunk - 1) / kModeChunk * kModeChunk;
+  const int per = (end + 64 * kWaveSlices - 1) / (64 * kWaveSlices) * 64;
+  const int i1 = min(end, per * (slice + 1));
 #pragma unroll 4
-  for (int i = lane; i < end; i += 64) {
+  for (int i = per * slice + lane; i < i1; i += 64) {
     const SmoothRec sr = sm[i];
     double t = __builtin_fma(sr.kap[0], X[0], sr.th);
     t = __builtin_fma(sr.kap[1], X[1], t);
@@ -850,6 +862,22 @@ k_bin_waves(const BinParams* __restrict__ prm, const int* __restrict__ counts,
     split2h(r0, rh, rl);
     row[i] = static_cast<uint32_t>(rh) | (static_cast<uint32_t>(rl) << 16);
   }
+}
+
+template <int D>
+__global__ void __launch_bounds__(256)
+k_bin_waves(const BinParams* __restrict__ prm, const int* __restrict__ counts,
+            const int* __restrict__ offsets, double* __restrict__ centre,
+            int* __restrict__ wave_cell, float4* __restrict__ recs,
+            const SmoothRec* __restrict__ sm, uint32_t* __restrict__ r0w, int64_t r0ld,
+            bool vec) {
+  const BinParams P = *prm;
+  const int lane = threadIdx.x & 63;
+  const int items = P.ncells * kWaveSlices;
+  const int nw = static_cast<int>(gridDim.x) * 4;
+  for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < items; it += nw)  // uniform per wave
+    cell_slice<D>(P, it / kWaveSlices, it % kWaveSlices, lane, counts, offsets, centre,
+                  wave_cell, recs, sm, r0w, r0ld, vec);
 }
 
 // Point block b again (same threads and rounds as k_bin_hist): ranks its
@@ -1498,7 +1526,9 @@ hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table
   const ModeRec* modes = static_cast<const ModeRec*>(table);
   const int nblk = static_cast<int>(L.nblk);
   const unsigned cb = static_cast<unsigned>((L.max_cells + kColCells - 1) / kColCells);
-  const unsigned wb = static_cast<unsigned>((L.max_cells + 3) / 4);  // a wave per cell
+  // k_bin_waves: kWaveSlices waves per cell, at most kWavesMax waves
+  const unsigned wb = static_cast<unsigned>(
+      (std::min<int64_t>(kWavesMax, L.max_cells * kWaveSlices) + 3) / 4);
   const unsigned mb = static_cast<unsigned>((L.max_waves + 3) / 4);
   const unsigned vb = static_cast<unsigned>((2 * L.max_waves + 3) / 4);
   const int Mi = static_cast<int>(M);
