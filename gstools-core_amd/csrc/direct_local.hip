@@ -788,9 +788,10 @@ __device__ __forceinline__ void cell_centre(const BinParams& P, int e, double (&
   }
 }
 
-// kWaveSlices waves per cell, over a grid-stride loop of (cell, slice) items
-// (a quarter of a row per wave keeps the dependent loads per wave short, and
-// small cell counts still spread over many waves).  Slice 0 records the
+// Up to kWaveSlices waves per cell (as many as the grid holds for the cell
+// count: a part of a row per wave keeps the dependent loads per wave short,
+// and small cell counts still spread over many waves), over a grid-stride
+// loop of (cell, slice) items.  Slice 0 records the
 // cell's centre and its waves in the wave -> cell map and marks the slots
 // past its points as padding; when r0w is given, every slice
 // writes its part of the cell's row of r0 (the reduced FP64 phase at the
@@ -800,10 +801,11 @@ __device__ __forceinline__ void cell_centre(const BinParams& P, int e, double (&
 //   the end of the last chunk, zero phase data past n_smooth;
 // - vector form: pairs (p0 | p1 << 16, p2) of the three bf16 parts, r0ld of
 //   them, zero past n_smooth.
-constexpr int kWaveSlices = 4;  // waves per cell in k_bin_waves
-constexpr int kWavesMax = 8192; // waves of the k_bin_waves grid at most
+constexpr int kWaveSlices = 4;  // waves per cell in k_bin_waves, at most
+constexpr int kWavesMax = 8192; // waves of the k_bin_waves grid for extra slices
 template <int D>
-__device__ __forceinline__ void cell_slice(const BinParams& P, int e, int slice, int lane,
+__device__ __forceinline__ void cell_slice(const BinParams& P, int e, int slice, int slices,
+                                           int lane,
                                            const int* __restrict__ counts,
                                            const int* __restrict__ offsets,
                                            double* __restrict__ centre,
@@ -826,7 +828,7 @@ __device__ __forceinline__ void cell_slice(const BinParams& P, int e, int slice,
   if (vec) {
     uint2* row = reinterpret_cast<uint2*>(r0w) + e * r0ld;
     // the slice's entries: a quarter of the row, whole 64-entry steps
-    const int64_t per = (r0ld + 64 * kWaveSlices - 1) / (64 * kWaveSlices) * 64;
+    const int64_t per = (r0ld + 64 * slices - 1) / (64 * slices) * 64;
     const int64_t i1 = min(r0ld, per * (slice + 1));
 #pragma unroll 4
     for (int64_t i = per * slice + lane; i < i1; i += 64) {
@@ -848,7 +850,7 @@ __device__ __forceinline__ void cell_slice(const BinParams& P, int e, int slice,
   }
   uint32_t* row = r0w + e * r0ld;
   const int end = (P.n_smooth + kModeChunk - 1) / kModeChunk * kModeChunk;
-  const int per = (end + 64 * kWaveSlices - 1) / (64 * kWaveSlices) * 64;
+  const int per = (end + 64 * slices - 1) / (64 * slices) * 64;
   const int i1 = min(end, per * (slice + 1));
 #pragma unroll 4
   for (int i = per * slice + lane; i < i1; i += 64) {
@@ -873,11 +875,12 @@ k_bin_waves(const BinParams* __restrict__ prm, const int* __restrict__ counts,
             bool vec) {
   const BinParams P = *prm;
   const int lane = threadIdx.x & 63;
-  const int items = P.ncells * kWaveSlices;
   const int nw = static_cast<int>(gridDim.x) * 4;
+  const int slices = max(1, min(kWaveSlices, nw / max(P.ncells, 1)));
+  const int items = P.ncells * slices;
   for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < items; it += nw)  // uniform per wave
-    cell_slice<D>(P, it / kWaveSlices, it % kWaveSlices, lane, counts, offsets, centre,
-                  wave_cell, recs, sm, r0w, r0ld, vec);
+    cell_slice<D>(P, it / slices, it % slices, slices, lane, counts, offsets, centre, wave_cell,
+                  recs, sm, r0w, r0ld, vec);
 }
 
 // Point block b again (same threads and rounds as k_bin_hist): ranks its
@@ -1526,9 +1529,11 @@ hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table
   const ModeRec* modes = static_cast<const ModeRec*>(table);
   const int nblk = static_cast<int>(L.nblk);
   const unsigned cb = static_cast<unsigned>((L.max_cells + kColCells - 1) / kColCells);
-  // k_bin_waves: kWaveSlices waves per cell, at most kWavesMax waves
+  // k_bin_waves: a wave per cell at least, up to kWaveSlices while the grid
+  // stays within kWavesMax waves
   const unsigned wb = static_cast<unsigned>(
-      (std::min<int64_t>(kWavesMax, L.max_cells * kWaveSlices) + 3) / 4);
+      (std::max<int64_t>(L.max_cells, std::min<int64_t>(kWavesMax, L.max_cells * kWaveSlices)) +
+       3) / 4);
   const unsigned mb = static_cast<unsigned>((L.max_waves + 3) / 4);
   const unsigned vb = static_cast<unsigned>((2 * L.max_waves + 3) / 4);
   const int Mi = static_cast<int>(M);
