@@ -630,9 +630,10 @@ def main() -> int:
     ap.add_argument("--m", type=int, default=1000)
     ap.add_argument("--no-grid", action="store_true", help="skip the structured-grid leg")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble leg")
-    ap.add_argument("--legs", default="sine,direct,incompr,fourier,grid,c5,ensemble,krige",
+    ap.add_argument("--legs", default="sine,direct,c1,incompr,fourier,grid,c5,ensemble,krige",
                     help="comma list of legs: sine (v_sin_f32 issue-rate calibration), direct, "
-                         "incompr, fourier, grid, c5, ensemble, c4plan (the ensemble's plan "
+                         "c1 (2-D, N = 1e5), incompr, fourier, grid, c5, ensemble, c4plan (the "
+                         "ensemble's plan "
                          "alone), krige, copy "
                          "(a 1 GiB device copy, the reference for HBM byte counters)")
     args = ap.parse_args()
@@ -669,6 +670,8 @@ def main() -> int:
             f"({sine['frac_of_rated']:.3f} of rated)")
     if "direct" not in legs:
         res = {"sine_calibration": sine} if sine else {}
+        if "c1" in legs:
+            res["c1"] = c1_leg(args.steps, args.warmup)
         if "incompr" in legs:
             res["incompr"] = incompr_leg(max(3, args.steps // 2), 1, sine=sine)
         if "fourier" in legs:
@@ -854,8 +857,10 @@ def main() -> int:
                                   "point_modes_per_s": ni * m / (hi_ms * 1e-3),
                                   "note": "summate_incompr, (3, N) Fortran-order result"}
 
-    log("c1 leg (2-D, N = 1e5)")
-    c1_res = c1_leg(args.steps, args.warmup)
+    c1_res = None
+    if "c1" in legs:
+        log("c1 leg (2-D, N = 1e5)")
+        c1_res = c1_leg(args.steps, args.warmup)
 
     cpu = None
     cpu_cfg = None
@@ -863,7 +868,9 @@ def main() -> int:
         log(f"cpu baseline on {args.cpu_sample} points")
         cpu = cpu_baseline(w, min(args.cpu_sample, n))
         log("cpu baselines of the other configurations")
-        gpu_rates = {"c1": c1_res["point_modes_per_s"]}
+        gpu_rates = {}
+        if c1_res:
+            gpu_rates["c1"] = c1_res["point_modes_per_s"]
         if inc_res:
             gpu_rates["c2i"] = inc_res["local_phase"]["point_modes_per_s"]
         if grid_res:

@@ -788,10 +788,11 @@ __device__ __forceinline__ void cell_centre(const BinParams& P, int e, double (&
   }
 }
 
-// Up to kWaveSlices waves per cell (as many as the grid holds for the cell
-// count: a part of a row per wave keeps the dependent loads per wave short,
-// and small cell counts still spread over many waves), over a grid-stride
-// loop of (cell, slice) items.  Slice 0 records the
+// Up to kWaveSlices waves per cell while the cells are few (a part of a row
+// per wave keeps the dependent loads per wave short, so a few cells still
+// spread over many waves; with thousands of cells one wave per cell is
+// faster, every slice repeating the cell's set-up), over a grid-stride loop
+// of (cell, slice) items.  Slice 0 records the
 // cell's centre and its waves in the wave -> cell map and marks the slots
 // past its points as padding; when r0w is given, every slice
 // writes its part of the cell's row of r0 (the reduced FP64 phase at the
@@ -803,6 +804,7 @@ __device__ __forceinline__ void cell_centre(const BinParams& P, int e, double (&
 //   them, zero past n_smooth.
 constexpr int kWaveSlices = 4;  // waves per cell in k_bin_waves, at most
 constexpr int kWavesMax = 8192; // waves of the k_bin_waves grid for extra slices
+constexpr int kSliceWaves = 2048;  // cells x slices up to which cells are sliced
 template <int D>
 __device__ __forceinline__ void cell_slice(const BinParams& P, int e, int slice, int slices,
                                            int lane,
@@ -876,7 +878,7 @@ k_bin_waves(const BinParams* __restrict__ prm, const int* __restrict__ counts,
   const BinParams P = *prm;
   const int lane = threadIdx.x & 63;
   const int nw = static_cast<int>(gridDim.x) * 4;
-  const int slices = max(1, min(kWaveSlices, nw / max(P.ncells, 1)));
+  const int slices = max(1, min(kWaveSlices, min(nw, kSliceWaves) / max(P.ncells, 1)));
   const int items = P.ncells * slices;
   for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < items; it += nw)  // uniform per wave
     cell_slice<D>(P, it / slices, it % slices, slices, lane, counts, offsets, centre, wave_cell,
