@@ -64,7 +64,7 @@ struct KrigeSmem {
 // MFMA operand / result layouts (v_mfma_f64_16x16x4f64, verified on gfx950):
 //   A: lane l holds A[l % 16][l / 16], B: lane l holds B[l / 16][l % 16],
 //   D: lane l, item it holds D[4 it + l / 16][l % 16].
-// Here A[i][k] = mat[k][i] (16 rows i of the wave), B[k][j] = vecs[k][j].
+// Here A[i][k] = mat[k][i] (16 rows i of one of the wave's kRS tiles), B[k][j] = vecs[k][j].
 // Every block reads all of mat once per kKJ targets, so the target width
 // sets mat's share of the traffic (n^2 8 / kKJ bytes per target).
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
