@@ -37,6 +37,26 @@ def test_library_exports_every_header_symbol():
     assert set(_native.SIGNATURES) == set(names)
 
 
+def test_option_struct_mirrors_agree():
+    # gsc_opts in the header, its ctypes mirror and the Rust crate-side mirror
+    # (not compiled here) list the same fields, in order, with the same widths
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    body = re.search(r"typedef struct gsc_opts \{(.*?)\} gsc_opts;", text, flags=re.S).group(1)
+    c_fields = re.findall(r"\b(int32_t|int64_t|double)\s+(\w+)\s*;", body)
+    width = {"int32_t": 4, "int64_t": 8, "double": 8, "i32": 4, "i64": 8, "f64": 8}
+    assert [n for _, n in c_fields] == [n for n, _ in _native.GscOpts._fields_]
+    assert [width[t] for t, _ in c_fields] == [ctypes.sizeof(t) for _, t in
+                                               _native.GscOpts._fields_]
+    rust = open(os.path.join(ROOT, "gstools-core_amd", "rust", "src", "hip.rs")).read()
+    rbody = re.search(r"pub struct GscOpts \{(.*?)\}", rust, flags=re.S).group(1)
+    r_fields = re.findall(r"pub (\w+):\s*(\w+)", rbody)
+    assert [n for n, _ in r_fields] == [n for _, n in c_fields]
+    assert [width[t] for _, t in r_fields] == [width[t] for t, _ in c_fields]
+    # every function the Rust extern block binds is declared in the header
+    r_fns = set(re.findall(r"\bfn (gsc_\w+)\s*\(", rust))
+    assert r_fns and r_fns <= set(header_functions())
+
+
 def test_version_and_defaults():
     lib = _native.lib()
     assert lib.gsc_version().decode().startswith("1.1.0")
