@@ -308,7 +308,8 @@ def ensemble_leg(steps: int, warmup: int, plan_only: bool = False) -> dict:
                  "accum": choice["accum"], "plan_ms": choice["plan_ms"], "kept": choice["kept"],
                  "choices": (dict(Counter(f"{p}/{t}" if p == "factored" else p
                                           for p, t in zip(choice["paths"], choice["tiles"])))
-                             if choice["paths"] is not None else "library rule")}
+                             if choice["paths"] is not None
+                             else "library rule's paths and tiles, tuned format")}
     res = {"workload": "512 2-D grids, sides uniform in {64..1024}, M=1000 Gaussian modes "
                        "(l uniform in [5,50]) per realisation, FP32 output, fp16x3",
            "point_modes": info["point_modes"], "factored": info["factored"],

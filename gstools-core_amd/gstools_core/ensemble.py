@@ -122,9 +122,10 @@ class Ensemble:
         choices can lose to simpler plans once the realisations share one
         launch (a second GEMM launch for the other tile shape has its own
         tail), so the tuned plan, the same paths with one tile shape for all
-        (256 or 128 rows) and the library rule are each built and timed
-        (gsc_ensemble_time) and the fastest is kept; ``tuned`` records the
-        times and the choice."""
+        (256 or 128 rows) and the library rule's paths and tiles ("library"),
+        all in the tuned split format and accumulation, are each built and
+        timed (gsc_ensemble_time) and the fastest is kept; ``tuned`` records
+        the times and the choice."""
         if self.tuned is None:
             return create(path_codes, tile_codes)
         tall = (ctypes.c_int32 * self.count)(*[_native.TILES["256"]] * self.count)

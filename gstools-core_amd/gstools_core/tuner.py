@@ -10,6 +10,8 @@ output) and on M.  The candidates are only those that meet 1e-5 sigma:
   direct                                   (the per-point kernel on the grid)
   factored  x  {fp16x3, bf16x6}            (bf16x3 and the single products miss the bar)
             x  {f32x2, f64}                (two-level accumulation)
+               + f32 up to M = 1024        (one FP32 sum: 6.1e-6 sigma at M = 1000 on
+                                            a 256^3 grid, 1.04e-5 at M = 4000)
             x  {256, 128}                  (GEMM block rows; 128 runs two blocks per CU)
 
 A shape class is the grid with every axis rounded up to a power of two (at
@@ -50,11 +52,12 @@ _DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "gstools_core_a
 
 FORMATS = ("fp16x3", "bf16x6")  # the split formats that meet 1e-5 sigma
 ACCUMS = ("f32x2", "f64")
+F32_MAX_MODES = 1024  # plain FP32 accumulation is a candidate up to this M
 TILES = ("256", "128")
 
 _TILE_ROWS = {"256": 256, "128": 128}
 _TILE_COLS = {("fp16x3", "f32x2"): 128, ("bf16x6", "f32x2"): 64, ("fp16x3", "f64"): 32,
-              ("bf16x6", "f64"): 32}
+              ("bf16x6", "f64"): 32, ("fp16x3", "f32"): 128, ("bf16x6", "f32"): 64}
 
 
 def _pow2(n: int) -> int:
@@ -125,11 +128,17 @@ class Tuner:
         cls, mb = shape_class(shape, m)
         return f"{self.device_key}|{field}|{dim}|{'x'.join(map(str, cls))}|{mb}"
 
-    def candidates(self, dim: int) -> List[str]:
+    def accums_for(self, m: int) -> Tuple[str, ...]:
+        """Accumulations that meet 1e-5 sigma at M modes."""
+        extra = ("f32",) if "f32" not in self.accums and int(m) <= F32_MAX_MODES else ()
+        return self.accums + extra
+
+    def candidates(self, dim: int, m: int) -> List[str]:
+        """Configurations of a class with M (rounded) modes."""
         keys = ["direct"]
         if dim >= 2:
-            keys += [config_key("factored", p, a, t) for p in self.formats for a in self.accums
-                     for t in self.tiles]
+            keys += [config_key("factored", p, a, t) for p in self.formats
+                     for a in self.accums_for(m) for t in self.tiles]
         return keys
 
     def time_config(self, dim: int, shape: Sequence[int], m: int, key: str,
@@ -152,14 +161,15 @@ class Tuner:
     def measure(self, dim: int, shape: Sequence[int], m: int, field: str = "scalar") -> dict:
         """Times every candidate on the class of (shape, m): config -> ms."""
         cls, mb = shape_class(shape, m)
-        return {k: self.time_config(dim, cls, mb, k, field) for k in self.candidates(dim)}
+        return {k: self.time_config(dim, cls, mb, k, field) for k in self.candidates(dim, mb)}
 
     def class_times(self, dim: int, shape: Sequence[int], m: int,
                     field: str = "scalar") -> dict:
         """The class's table of times, measured on first use and cached."""
         key = self._key(dim, shape, m, field)
         hit = self.table.get(key)
-        missing = hit is None or any(k not in hit["ms"] for k in self.candidates(dim))
+        _, mb = shape_class(shape, m)
+        missing = hit is None or any(k not in hit["ms"] for k in self.candidates(dim, mb))
         if missing:
             ms = self.measure(dim, shape, m, field)
             cls, mb = shape_class(shape, m)
@@ -183,7 +193,7 @@ class Tuner:
         if dim < 2 or min(shape) == 0 or m == 0:
             return {"path": "direct", "ms": 0.0}
         best, best_ms = None, None
-        for k in self.candidates(dim):
+        for k in self.candidates(dim, shape_class(shape, m)[1]):
             c = parse_config(k)
             if c["path"] == "factored" and ((precision not in (None, "auto") and
                                              c["precision"] != precision) or
@@ -241,7 +251,9 @@ class Tuner:
         summed estimated time over the realisations (one pair per plan), and
         for every realisation its path and tile shape under that pair, from
         per-class times measured inside persistent plans."""
-        pairs = [(p, a) for p in self.formats for a in self.accums
+        m_top = max([int(m) for m in ms] + [0])
+        pairs = [(p, a) for p in self.formats
+                 for a in self.accums_for(shape_class((16,), m_top)[1])
                  if precision in (None, "auto", p) and accum in (None, a)]
         if not pairs:
             pairs = [(precision, accum or "f32x2")]

@@ -200,13 +200,16 @@ def test_fp64_flush_plan_two_blocks_per_cu():
     ens.close()
 
 
-def test_c4_all_realisations_in_one_plan():
+@pytest.mark.parametrize("accum", ["f32x2", "f32"])
+def test_c4_all_realisations_in_one_plan(accum):
     # the plan the benchmark times: all 512 realisations, fp16x3, FP32 output,
-    # each checked against the FP64 oracle on 10^4 seeded points
+    # each checked against the FP64 oracle on 10^4 seeded points; with the
+    # two-level accumulation (the library's) and with one FP32 sum (a tuner
+    # candidate at M <= 1024)
     ens_w = workloads.c4(count=512)
     reals = [(w.modes.cov_samples, w.modes.z1, w.modes.z2, w.axes) for w in ens_w]
     scales = [w.modes.scale for w in ens_w]
-    ens = Ensemble(reals, scales=scales, out_dtype=np.float32, precision="fp16x3")
+    ens = Ensemble(reals, scales=scales, out_dtype=np.float32, precision="fp16x3", accum=accum)
     assert ens.info["factored"] == 512
     ens.run()
     res = ens.results()
@@ -219,7 +222,8 @@ def test_c4_all_realisations_in_one_plan():
         assert e <= 1e-5, f"realisation {r} {out.shape}: {e:.3e}"
         worst = max(worst, e)
         n_checked += idx.size
-    print(f"c4 all 512 realisations ({n_checked} points) max|err|/sigma = {worst:.3e}")
+    print(f"c4 all 512 realisations, accumulation {accum} ({n_checked} points) "
+          f"max|err|/sigma = {worst:.3e}")
     ens.close()
 
 
@@ -252,7 +256,10 @@ def test_tuner_measures_chooses_and_caches(tmp_path):
     print("chosen", big, small)
     assert big["path"] == "factored"  # a 512 x 512 class at M = 1024 favours the MFMA path
     table = t.class_times(2, (300, 400), 1000)
-    assert len(table["ms"]) == 9 and table["best"] in table["ms"]
+    # direct + 2 formats x 3 accumulations (f32 joins at M <= 1024) x 2 tiles
+    assert len(table["ms"]) == 13 and table["best"] in table["ms"]
+    assert len(t.class_times(2, (8, 9), 50)["ms"]) == 13
+    assert "f32" not in t.accums_for(2048) and "f32" in t.accums_for(1024)
     # a fixed format is honoured; a second tuner reads the cache without measuring
     c = t.choose(2, (300, 400), 1000, precision="bf16x6", accum="f64")
     assert c["path"] == "direct" or (c["precision"], c["accum"]) == ("bf16x6", "f64")
