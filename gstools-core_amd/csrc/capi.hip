@@ -210,6 +210,25 @@ hipError_t launch_direct(int kind, int dim, int64_t M, int64_t N, void* work, co
                                    variant, s);
 }
 
+// The mode table (from cov, z1, z2 and, for the Fourier summator, sf) and the
+// direct kernel of `kind`: the local variant prepares the table in extra
+// blocks of its first sort kernel, the others launch the preparation first.
+hipError_t launch_prep_direct(int kind, int dim, int64_t M, int64_t N, const double* cov,
+                              const double* z1, const double* z2, const double* sf, void* work,
+                              const double* pos, int64_t pos_ld, void* out, int64_t out_ld,
+                              const gsc_opts* o, hipStream_t s, hipEvent_t split = nullptr) {
+  if (use_binned(kind, dim, M, N, o)) {
+    const gsc::PrepArgs pa = gsc::prep_args(kind, dim, M, cov, z1, z2, sf, work);
+    void* bin = static_cast<char*>(work) + align256(gsc::mode_table_bytes(dim, M));
+    return gsc::launch_direct_binned(dim, M, N, work, pos, pos_ld, out, out_ld,
+                                     kind == gsc::KIND_INCOMPR, o->out_dtype, o->scale, bin, s,
+                                     split, &pa);
+  }
+  const hipError_t e = gsc::launch_prep(kind, dim, M, cov, z1, z2, sf, work, s);
+  if (e != hipSuccess) return e;
+  return launch_direct(kind, dim, M, N, work, pos, pos_ld, out, out_ld, o, s, split);
+}
+
 int host_summate(int kind, int dim, int64_t M, int64_t N, const double* sf, int64_t sf_s,
                  const double* cov, int64_t cov_sd, int64_t cov_sm, const double* z1,
                  int64_t z1_s, const double* z2, int64_t z2_s, const double* pos, int64_t pos_sd,
@@ -290,10 +309,10 @@ int host_summate(int kind, int dim, int64_t M, int64_t N, const double* sf, int6
       GSC_HIP(hipMemcpyAsync(d_pos, pos_src, sizeof(double) * dim * N, hipMemcpyHostToDevice,
                              c.stream));
     }
-    GSC_HIP(gsc::launch_prep(kind, dim, M, d_modes, d_modes + dim * M, d_modes + (dim + 1) * M,
-                             kind == gsc::KIND_FOURIER ? d_modes + (dim + 2) * M : nullptr,
-                             d_table, c.stream));
-    GSC_HIP(launch_direct(kind, dim, M, N, d_table, d_pos, N, d_out, N, o, c.stream));
+    GSC_HIP(launch_prep_direct(kind, dim, M, N, d_modes, d_modes + dim * M,
+                               d_modes + (dim + 1) * M,
+                               kind == gsc::KIND_FOURIER ? d_modes + (dim + 2) * M : nullptr,
+                               d_table, d_pos, N, d_out, N, o, c.stream));
 
     // result: a flat (1, ncomp N) copy for dense C-order and Fortran-order
     // views, else the strided (ncomp, N) scatter
@@ -364,8 +383,8 @@ int dev_common(int kind, int dim, int64_t M, int64_t N, const double* sf, const 
                 need);
   if (N == 0) return GSC_OK;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  GSC_HIP(gsc::launch_prep(kind, dim, M, cov, z1, z2, sf, work, s));
-  GSC_HIP(launch_direct(kind, dim, M, N, work, pos, pos_ld, out, out_ld, o, s, split));
+  GSC_HIP(launch_prep_direct(kind, dim, M, N, cov, z1, z2, sf, work, pos, pos_ld, out, out_ld, o,
+                             s, split));
   return GSC_OK;
 }
 
@@ -524,8 +543,8 @@ int grid_launch(int field, int dim, int64_t M, const double* sf, const double* c
   double* pos = static_cast<double*>(work);
   void* dwork = static_cast<char*>(work) + b_pos;
   GSC_HIP(gsc::launch_expand_grid(dim, shape, axes, pos, s));
-  GSC_HIP(gsc::launch_prep(gsc::KIND_INCOMPR, dim, M, cov, z1, z2, nullptr, dwork, s));
-  GSC_HIP(launch_direct(gsc::KIND_INCOMPR, dim, M, N, dwork, pos, N, out, N, o, s));
+  GSC_HIP(launch_prep_direct(gsc::KIND_INCOMPR, dim, M, N, cov, z1, z2, nullptr, dwork, pos, N,
+                             out, N, o, s));
   return GSC_OK;
 }
 

@@ -388,6 +388,23 @@ size_t mode_table_bytes(int dim, int64_t M) {
   return rec_bytes(M) + atab_bytes(M) + amptab_bytes(M);
 }
 
+PrepArgs prep_args(int kind, int dim, int64_t M, const double* cov, const double* z1,
+                   const double* z2, const double* sf, void* table) {
+  PrepArgs a;
+  a.kind = kind;
+  a.dim = dim;
+  a.M = M;
+  a.ntiles = M > 0 ? mfma_tiles(M) : 0;
+  a.cov = cov;
+  a.z1 = z1;
+  a.z2 = z2;
+  a.sf = sf;
+  a.table = static_cast<ModeRec*>(table);
+  a.a_tab = const_cast<double*>(atab_of(table, M));
+  a.amp_tab = const_cast<float*>(amptab_of(table, M));
+  return a;
+}
+
 hipError_t launch_prep(int kind, int dim, int64_t M, const double* cov, const double* z1,
                        const double* z2, const double* sf, void* table, hipStream_t stream) {
   if (M <= 0) return hipSuccess;

@@ -240,9 +240,19 @@ T* at(void* base, size_t off) {
 }
 
 // ---- pre-pass: bounding box ------------------------------------------------
+// Blocks [0, kBoxBlocks): partial bounding boxes; the blocks after them (when
+// pa.ntiles > 0) prepare the mode table, 256 modes each (prep_mode).
 template <int D>
 __global__ void __launch_bounds__(256)
-k_box_partial(const double* __restrict__ pos, int64_t ld, int64_t N, double* __restrict__ box) {
+k_box_partial(const double* __restrict__ pos, int64_t ld, int64_t N, double* __restrict__ box,
+              PrepArgs pa) {
+  if (blockIdx.x >= kBoxBlocks) {  // uniform over the block
+    const int64_t m = static_cast<int64_t>(blockIdx.x - kBoxBlocks) * 256 + threadIdx.x;
+    if (m < static_cast<int64_t>(pa.ntiles) * 16)
+      prep_mode(pa.kind, pa.dim, pa.M, m, pa.cov, pa.z1, pa.z2, pa.sf, pa.table, pa.a_tab,
+                pa.amp_tab);
+    return;
+  }
   __shared__ double red[2 * D][256];
   double lo[D], hi[D];
 #pragma unroll
@@ -252,7 +262,7 @@ k_box_partial(const double* __restrict__ pos, int64_t ld, int64_t N, double* __r
   }
   // kBoxUnroll points in flight per thread; out-of-range ones repeat point
   // N - 1, which changes no extreme
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * 256;
+  const int64_t stride = static_cast<int64_t>(kBoxBlocks) * 256;
   for (int64_t pb = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; pb < N;
        pb += kBoxUnroll * stride) {
     double x[kBoxUnroll][D];
@@ -1507,8 +1517,15 @@ bool binned_supported(int dim, int64_t N) {
 hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table,
                                 const double* pos, int64_t pos_ld, void* out, int64_t out_ld,
                                 bool vec, int out_dtype, double scale, void* bin_work,
-                                hipStream_t s, hipEvent_t split) {
-  if (N <= 0) return hipSuccess;
+                                hipStream_t s, hipEvent_t split, const PrepArgs* prep) {
+  if (N <= 0) return prep ? launch_prep(prep->kind, prep->dim, prep->M, prep->cov, prep->z1,
+                                        prep->z2, prep->sf, prep->table, s)
+                          : hipSuccess;
+  // the mode table's preparation rides on the bounding-box launch
+  PrepArgs pa{};
+  if (prep) pa = *prep;
+  const unsigned box_grid = static_cast<unsigned>(
+      kBoxBlocks + (static_cast<int64_t>(pa.ntiles) * 16 + 255) / 256);
   const BinLayout L = bin_layout(M, N);
   BinParams* prm = at<BinParams>(bin_work, L.params);
   double* box = at<double>(bin_work, L.box);
@@ -1542,7 +1559,7 @@ hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table
   const bool f32 = out_dtype == 1;
 #define GSC_BIN(D)                                                                               \
   do {                                                                                           \
-    k_box_partial<D><<<kBoxBlocks, 256, 0, s>>>(pos, pos_ld, N, box);                            \
+    k_box_partial<D><<<box_grid, 256, 0, s>>>(pos, pos_ld, N, box, pa);                          \
     k_bin_params<D><<<1, 256, 0, s>>>(box, N, L.max_cells, modes, Mi, prm, btpl, btpl32, ampt,   \
                                       sm, rough);                                                \
     k_bin_hist<D><<<nblk, kSortThreads, 0, s>>>(pos, pos_ld, N, L.blk_pts, prm, bcount);         \

@@ -35,6 +35,21 @@ size_t mode_table_bytes(int dim, int64_t M);
 
 // Builds the mode table from dense device arrays cov (dim, M) row-major,
 // z1, z2 (M) and, for KIND_FOURIER, sf (M).
+// The mode-table preparation as arguments of a kernel that does it in
+// blocks of its own (dim <= 3): ntiles x 16 modes (prep_mode), table the
+// mode table of mode_table_bytes(dim, M) bytes.
+struct PrepArgs {
+  int kind, dim;
+  int64_t M;
+  int ntiles;
+  const double *cov, *z1, *z2, *sf;
+  ModeRec* table;
+  double* a_tab;
+  float* amp_tab;
+};
+PrepArgs prep_args(int kind, int dim, int64_t M, const double* cov, const double* z1,
+                   const double* z2, const double* sf, void* table);
+
 hipError_t launch_prep(int kind, int dim, int64_t M, const double* cov, const double* z1,
                        const double* z2, const double* sf, void* table, hipStream_t stream);
 
@@ -57,11 +72,14 @@ hipError_t launch_direct_incompr(int dim, int64_t M, int64_t N, const void* tabl
 bool binned_supported(int dim, int64_t N);
 size_t binned_work_bytes(int dim, int64_t M, int64_t N);
 // split: null, or an event recorded on `stream` between the cell sort and the
-// summation kernel (phase timing).
+// summation kernel (phase timing).  prep: null when `table` is prepared
+// already, else the preparation, done by extra blocks of the first sort
+// kernel (one launch less).
 hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table,
                                 const double* pos, int64_t pos_ld, void* out, int64_t out_ld,
                                 bool vec, int out_dtype, double scale, void* bin_work,
-                                hipStream_t stream, hipEvent_t split = nullptr);
+                                hipStream_t stream, hipEvent_t split = nullptr,
+                                const PrepArgs* prep = nullptr);
 
 // Seeded spectral-mode sampling (sampling.hip): Philox4x64-10 words of key
 // (seed, stream_id), Box-Muller normals, k = g / l (model 0, Gaussian) or
