@@ -812,8 +812,8 @@ __device__ __forceinline__ void cell_centre(const BinParams& P, int e, double (&
 //   the end of the last chunk, zero phase data past n_smooth;
 // - vector form: pairs (p0 | p1 << 16, p2) of the three bf16 parts, r0ld of
 //   them, zero past n_smooth.
-constexpr int kWaveSlices = 4;  // waves per cell in k_bin_waves, at most
-constexpr int kWavesMax = 8192; // waves of the k_bin_waves grid for extra slices
+constexpr int kWaveSlices = 4;  // waves per cell for its row, at most
+constexpr int kWavesMax = 8192; // waves for the cell rows, for extra slices
 constexpr int kSliceWaves = 2048;  // cells x slices up to which cells are sliced
 template <int D>
 __device__ __forceinline__ void cell_slice(const BinParams& P, int e, int slice, int slices,
@@ -878,22 +878,6 @@ __device__ __forceinline__ void cell_slice(const BinParams& P, int e, int slice,
   }
 }
 
-template <int D>
-__global__ void __launch_bounds__(256)
-k_bin_waves(const BinParams* __restrict__ prm, const int* __restrict__ counts,
-            const int* __restrict__ offsets, double* __restrict__ centre,
-            int* __restrict__ wave_cell, float4* __restrict__ recs,
-            const SmoothRec* __restrict__ sm, uint32_t* __restrict__ r0w, int64_t r0ld,
-            bool vec) {
-  const BinParams P = *prm;
-  const int lane = threadIdx.x & 63;
-  const int nw = static_cast<int>(gridDim.x) * 4;
-  const int slices = max(1, min(kWaveSlices, min(nw, kSliceWaves) / max(P.ncells, 1)));
-  const int items = P.ncells * slices;
-  for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < items; it += nw)  // uniform per wave
-    cell_slice<D>(P, it / slices, it % slices, slices, lane, counts, offsets, centre, wave_cell,
-                  recs, sm, r0w, r0ld, vec);
-}
 
 // Point block b again (same threads and rounds as k_bin_hist): ranks its
 // points within each cell with LDS atomics (starting from the block's base in
@@ -910,13 +894,33 @@ __device__ __forceinline__ void scatter_point(const BinParams& P, const double* 
   recs[atomicAdd(&rank[e], 1)] = make_rec(xi[0], xi[1], xi[2], static_cast<int>(p));
 }
 
+// The two placement steps in one launch; they write disjoint memory (the
+// scatter the slots below each cell's count, cell_slice the padding slots
+// above it, the centres, the wave -> cell map and the r0 rows).  Blocks
+// [0, nblk) scatter point block b; the blocks after them take the (cell,
+// slice) items of cell_slice, kSortThreads / 64 waves per block, in a
+// grid-stride loop over nwv waves.
 template <int D>
 __global__ void __launch_bounds__(kSortThreads)
-k_bin_scatter(const double* __restrict__ pos, int64_t ld, int64_t N, int64_t blk_pts,
+k_bin_scatter(const double* __restrict__ pos, int64_t ld, int64_t N, int64_t blk_pts, int nblk,
               const BinParams* __restrict__ prm, const int* __restrict__ bcount,
-              const int* __restrict__ offsets, float4* __restrict__ recs) {
+              const int* __restrict__ counts, const int* __restrict__ offsets,
+              float4* __restrict__ recs, double* __restrict__ centre, int* __restrict__ wave_cell,
+              const SmoothRec* __restrict__ sm, uint32_t* __restrict__ r0w, int64_t r0ld,
+              bool vec) {
   __shared__ int rank[kMaxCells];
   const BinParams P = *prm;
+  if (static_cast<int>(blockIdx.x) >= nblk) {  // uniform over the block
+    constexpr int kWpb = kSortThreads / 64;
+    const int nwv = (static_cast<int>(gridDim.x) - nblk) * kWpb;
+    const int slices = max(1, min(kWaveSlices, min(nwv, kSliceWaves) / max(P.ncells, 1)));
+    const int items = P.ncells * slices;
+    for (int it = (static_cast<int>(blockIdx.x) - nblk) * kWpb + static_cast<int>(threadIdx.x >> 6);
+         it < items; it += nwv)  // uniform per wave
+      cell_slice<D>(P, it / slices, it % slices, slices, threadIdx.x & 63, counts, offsets, centre,
+                    wave_cell, recs, sm, r0w, r0ld, vec);
+    return;
+  }
   const int* cbase = bcount + static_cast<int64_t>(blockIdx.x) * P.ncells;
   for (int e = threadIdx.x; e < P.ncells; e += kSortThreads) rank[e] = offsets[e] + cbase[e];
   __syncthreads();
@@ -1172,7 +1176,7 @@ __device__ __forceinline__ void sum_modes_vec(const ModeRec* __restrict__ modes,
 //
 // K slots 24 .. 26 (lane group kg = 3) carry r0: A holds 1 there and B the
 // three bf16 parts of the cell's r0, read from the per-cell table that
-// k_bin_waves writes, so the matrix instruction returns the whole local
+// cell_slice writes, so the matrix instruction returns the whole local
 // phase.  The amplitudes of kVecAmpChunk modes at a time sit in the block's
 // LDS.  Without a per-cell table (too large) k_direct_local_vec_eval below
 // evaluates r0 per wave pair and chunk instead.
@@ -1542,17 +1546,19 @@ hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table
   SmoothRec* sm = at<SmoothRec>(bin_work, L.sm);
   ModeRec* rough = at<ModeRec>(bin_work, L.rough);
   uint32_t* r0w = L.r0ld > 0 || L.r0vld > 0 ? at<uint32_t>(bin_work, L.r0w) : nullptr;
-  // the r0 rows k_bin_waves writes: the scalar or the vector form, if any
+  // the r0 rows cell_slice writes: the scalar or the vector form, if any
   uint32_t* r0rows = vec ? (L.r0vld > 0 ? r0w : nullptr) : (L.r0ld > 0 ? r0w : nullptr);
   const int64_t r0rld = vec ? L.r0vld : L.r0ld;
   const ModeRec* modes = static_cast<const ModeRec*>(table);
   const int nblk = static_cast<int>(L.nblk);
   const unsigned cb = static_cast<unsigned>((L.max_cells + kColCells - 1) / kColCells);
-  // k_bin_waves: a wave per cell at least, up to kWaveSlices while the grid
-  // stays within kWavesMax waves
-  const unsigned wb = static_cast<unsigned>(
-      (std::max<int64_t>(L.max_cells, std::min<int64_t>(kWavesMax, L.max_cells * kWaveSlices)) +
-       3) / 4);
+  // placement: the scatter's point blocks, then the cell rows' blocks (a
+  // wave per cell at least, up to kWaveSlices while within kWavesMax waves)
+  constexpr int kWpb = kSortThreads / 64;
+  const unsigned pb = static_cast<unsigned>(
+      nblk + (std::max<int64_t>(L.max_cells,
+                                std::min<int64_t>(kWavesMax, L.max_cells * kWaveSlices)) +
+              kWpb - 1) / kWpb);
   const unsigned mb = static_cast<unsigned>((L.max_waves + 3) / 4);
   const unsigned vb = static_cast<unsigned>((2 * L.max_waves + 3) / 4);
   const int Mi = static_cast<int>(M);
@@ -1565,10 +1571,9 @@ hipError_t launch_direct_binned(int dim, int64_t M, int64_t N, const void* table
     k_bin_hist<D><<<nblk, kSortThreads, 0, s>>>(pos, pos_ld, N, L.blk_pts, prm, bcount);         \
     k_col_scan<<<cb, 256, 0, s>>>(prm, nblk, bcount, counts);                                    \
     k_bin_offsets<<<1, kSortThreads, 0, s>>>(prm, counts, offsets);                              \
-    k_bin_waves<D><<<wb, 256, 0, s>>>(prm, counts, offsets, centre, wave_cell, recs, sm, r0rows, \
-                                      r0rld, vec);                                               \
-    k_bin_scatter<D><<<nblk, kSortThreads, 0, s>>>(pos, pos_ld, N, L.blk_pts, prm, bcount,       \
-                                                   offsets, recs);                               \
+    k_bin_scatter<D><<<pb, kSortThreads, 0, s>>>(pos, pos_ld, N, L.blk_pts, nblk, prm, bcount,   \
+                                                 counts, offsets, recs, centre, wave_cell, sm,   \
+                                                 r0rows, r0rld, vec);                            \
     if (split) (void)hipEventRecord(split, s);                                                   \
     if (vec && r0rows != nullptr && f32)                                                         \
       k_direct_local_vec<D, float><<<mb, 256, 0, s>>>(                                         \
